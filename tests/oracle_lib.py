@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(os.path.dirname(_HERE), "oracle", "liboracle.so")
+
+
+class Params(C.Structure):
+    _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dpLimit", C.c_double),
+                ("overshootTol", C.c_double), ("dtInit", C.c_double), ("rng", C.c_int),
+                ("controller", C.c_int), ("seed", C.c_ulonglong), ("stream", C.c_ulonglong)]
+
+
+class Run(C.Structure):
+    _fields_ = [("mode", C.c_int), ("dc", C.c_int), ("Dt", C.c_double), ("NDt", C.c_long), ("T", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        l = C.CDLL(_LIB)
+        l.orc_last_error.restype = C.c_char_p
+        l.orc_system_create.restype = C.c_void_p
+        l.orc_system_create.argtypes = [C.c_char_p]
+        l.orc_system_destroy.argtypes = [C.c_void_p]
+        l.orc_system_dim.restype = C.c_long
+        l.orc_system_dim.argtypes = [C.c_void_p]
+        l.orc_system_njumps.argtypes = [C.c_void_p]
+        l.orc_system_navr.argtypes = [C.c_void_p]
+        l.orc_system_highest_frequency.argtypes = [C.c_void_p]
+        l.orc_system_highest_frequency.restype = C.c_double
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        l.orc_apply_heff.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp]
+        l.orc_rates.argtypes = [C.c_void_p, dp, dp]
+        l.orc_act_jump.argtypes = [C.c_void_p, dp, C.c_int]
+        l.orc_averages.argtypes = [C.c_void_p, dp, dp]
+        l.orc_pcg64_uniform.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_long, dp, C.c_void_p]
+        l.orc_philox4x64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        l.orc_counter_uniform.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong]
+        l.orc_counter_uniform.restype = C.c_double
+        l.orc_mcwf_run.restype = C.c_long
+        l.orc_mcwf_run.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), dp, C.c_long,
+                                   C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
+        l.orc_ensemble_run.restype = C.c_long
+        l.orc_ensemble_run.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.c_long, dp, C.c_long, C.c_void_p]
+        l.orc_master_run.restype = C.c_long
+        l.orc_master_run.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), dp, C.c_long, C.c_void_p]
+        _lib = l
+    return _lib
+
+
+def _cplx(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+
+
+class System:
+    """An oracle system built from a spec string (same grammar as the product's)."""
+
+    def __init__(self, spec):
+        self.h = lib().orc_system_create(spec.encode())
+        if not self.h:
+            raise ValueError(lib().orc_last_error().decode())
+        self.dim = lib().orc_system_dim(self.h)
+        self.njumps = lib().orc_system_njumps(self.h)
+        self.navr = lib().orc_system_navr(self.h)
+        self.highest_frequency = lib().orc_system_highest_frequency(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_system_destroy(self.h)
+
+    def apply_heff(self, psi, t=0.0, t0=0.0, dpsidt=None):
+        out = np.zeros(self.dim, np.complex128) if dpsidt is None else np.array(dpsidt, np.complex128)
+        o = out.view(np.float64)
+        if lib().orc_apply_heff(self.h, t, t0, _cplx(psi), o):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return o.view(np.complex128)
+
+    def dense_heff(self):
+        d = self.dim
+        m = np.zeros((d, d), np.complex128)
+        for j in range(d):
+            e = np.zeros(d, np.complex128)
+            e[j] = 1
+            m[:, j] = self.apply_heff(e)
+        return m
+
+    def rates(self, psi):
+        out = np.zeros(self.njumps)
+        lib().orc_rates(self.h, _cplx(psi), out)
+        return out
+
+    def act_jump(self, psi, m):
+        p = _cplx(psi).copy()
+        lib().orc_act_jump(self.h, p, m)
+        return p.view(np.complex128)
+
+    def averages(self, psi):
+        out = np.zeros(self.navr)
+        lib().orc_averages(self.h, _cplx(psi), out)
+        return out
+
+
+def params(epsRel=1e-6, epsAbs=1e-12, dpLimit=0.01, overshootTol=10.0, dtInit=0.0, rng="philox",
+           controller="odeint", seed=1001, stream=1):
+    return Params(epsRel, epsAbs, dpLimit, overshootTol, dtInit, 0 if rng == "pcg64" else 1,
+                  0 if controller == "odeint" else 1, seed, stream)
+
+
+def run_spec(mode="dc", dc=10, Dt=0.1, NDt=0, T=1.0):
+    return Run({"dc": 0, "Dt": 1, "NDt": 2}[mode], dc, Dt, NDt, T)
+
+
+def mcwf_run(system, psi0, p, r, cap=100000, jcap=100000):
+    nav = system.navr
+    rows = np.zeros((cap, 2 + nav))
+    jt = np.zeros(jcap)
+    jc = np.zeros(jcap, np.int32)
+    cnt = np.zeros(8, np.int64)
+    psi = np.zeros(system.dim, np.complex128)
+    n = lib().orc_mcwf_run(system.h, _cplx(psi0), C.byref(p), C.byref(r), rows, cap,
+                           jt.ctypes.data, jc.ctypes.data, jcap, cnt.ctypes.data, psi.view(np.float64).ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    nj = int(cnt[7])
+    return dict(rows=rows[:min(n, cap)], jump_t=jt[:nj], jump_ch=jc[:nj], counters=cnt, psi=psi)
+
+
+def ensemble_run(system, psi0, p, r, ntraj, cap=100000):
+    rows = np.zeros((cap, 2 + system.navr))
+    tries = np.zeros(1, np.int64)
+    n = lib().orc_ensemble_run(system.h, _cplx(psi0), C.byref(p), C.byref(r), ntraj, rows, cap, tries.ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return rows[:min(n, cap)], int(tries[0])
+
+
+def master_run(system, rho0, p, r, cap=100000):
+    rows = np.zeros((cap, 2 + system.navr))
+    d = system.dim
+    rho = np.zeros((d, d), np.complex128)
+    n = lib().orc_master_run(system.h, _cplx(rho0), C.byref(p), C.byref(r), rows, cap, rho.view(np.float64).ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return rows[:min(n, cap)], rho
+
+
+def counter_uniform(seed, traj, step):
+    return lib().orc_counter_uniform(seed, traj, step)
+
+
+def philox4x64(ctr, key):
+    c = (C.c_ulonglong * 4)(*ctr)
+    k = (C.c_ulonglong * 2)(*key)
+    o = (C.c_ulonglong * 4)()
+    lib().orc_philox4x64(C.byref(c), C.byref(k), C.byref(o))
+    return list(o)
+
+
+def pcg64_uniform(seed, stream, n):
+    out = np.zeros(n)
+    raw = np.zeros(n, np.uint64)
+    lib().orc_pcg64_uniform(seed, stream, n, out, raw.ctypes.data)
+    return out, raw
