@@ -1,0 +1,334 @@
+"""cppqed_amd -- MI355X-native quantum-trajectory hot path (MCWF ensembles, Master sweeps).
+
+Thin Python mirror of the reference's element / trajectory-driver interface
+(C++QED: ``mode::make``, ``qbit::make``, ``jaynescummings::make``,
+``binary::make``, ``composite::make``, ``evolve``), over the C ABI in
+``include/qtraj.h`` implemented by ``libqtraj.so`` (HIP for gfx950 + host C++).
+The library must be built (``python -m cppqed_amd.build``); nothing here falls
+back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libqtraj.so")
+
+QT_FP64, QT_FP32_DERIV = 0, 1
+
+_lib = None
+
+
+class QtError(RuntimeError):
+    pass
+
+
+class SysInfo(C.Structure):
+    _fields_ = [("dim", C.c_long), ("rank", C.c_int), ("dims", C.c_int * 16), ("n_jumps", C.c_int),
+                ("n_avr", C.c_int), ("h_width", C.c_int), ("j_width", C.c_int), ("o_width", C.c_int),
+                ("time_dependent", C.c_int), ("highest_frequency", C.c_double), ("h_nnz", C.c_longlong)]
+
+
+class McwfParams(C.Structure):
+    _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dpLimit", C.c_double),
+                ("overshootTol", C.c_double), ("dtInit", C.c_double), ("precision", C.c_int),
+                ("controller", C.c_int), ("seed", C.c_ulonglong), ("traj0", C.c_ulonglong),
+                ("jump_log_cap", C.c_int), ("layout_rows", C.c_int), ("layout_waves", C.c_int)]
+
+
+def lib():
+    """Loads libqtraj.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise QtError("libqtraj.so is not built (python -m cppqed_amd.build)")
+        l = C.CDLL(LIBPATH)
+        vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+        l.qt_last_error.restype = C.c_char_p
+        l.qt_sys_build.argtypes = [C.c_char_p, C.POINTER(vp)]
+        l.qt_sys_destroy.argtypes = [vp]
+        l.qt_sys_info.argtypes = [vp, C.POINTER(SysInfo)]
+        l.qt_sys_table.argtypes = [vp, ip, C.POINTER(C.c_int), vp, vp, vp]
+        l.qt_apply_heff.argtypes = [vp, dp, dp, vp, vp, ip, ip, vp]
+        l.qt_mcwf_create.argtypes = [vp, ip, C.POINTER(McwfParams), vp, C.POINTER(vp)]
+        l.qt_mcwf_destroy.argtypes = [vp]
+        l.qt_mcwf_advance.argtypes = [vp, vp, ip, vp, vp]
+        l.qt_mcwf_advance_async.argtypes = [vp, vp, ip, vp]
+        l.qt_mcwf_reset.argtypes = [vp, vp]
+        l.qt_mcwf_traj_averages.argtypes = [vp, vp]
+        l.qt_mcwf_state.argtypes = [vp, ip, vp, vp]
+        l.qt_mcwf_jumplog.argtypes = [vp, ip, vp, vp, ip, C.POINTER(C.c_int)]
+        l.qt_mcwf_counters.argtypes = [vp, vp, vp]
+        l.qt_mcwf_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        l.qt_mcwf_plain_run.argtypes = [vp, C.POINTER(McwfParams), C.c_ulonglong, vp, vp, ip, vp, vp, vp, vp, ip, vp]
+        for name in ("qt_master_create", "qt_master_destroy", "qt_master_advance", "qt_master_rho", "qt_master_kernel_ms"):
+            if not hasattr(l, name):
+                continue
+        _lib = l
+    return _lib
+
+
+def check(rc):
+    if rc:
+        raise QtError(lib().qt_last_error().decode())
+
+
+def _cbuf(a):
+    """complex128 C-contiguous copy and its pointer."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    return a, a.ctypes.data
+
+
+# ---------------------------------------------------------------- elements (spec builders)
+
+class Element:
+    """A free element or an interaction, as a spec statement."""
+
+    def __init__(self, stmt, dim=None):
+        self.stmt = stmt
+        self.dim = dim
+
+
+class mode:  # noqa: N801  (mirrors the reference namespace mode::)
+    @staticmethod
+    def make(cutoff, delta=-10.0, kappa=0.0, eta=0.0, picture="Sch"):
+        """mode::make(ParsPumpedLossy, QM_Picture), CPPQEDelements/frees/Mode.tcc:40-60."""
+        eta = complex(eta)
+        return Element(f"free mode cutoff={int(cutoff)} delta={delta!r} kappa={kappa!r} "
+                       f"eta={eta.real!r} etaI={eta.imag!r} picture={picture}", int(cutoff))
+
+
+class qbit:  # noqa: N801
+    @staticmethod
+    def make(deltaA=-10.0, gamma=0.0, eta=0.0, picture="Sch"):
+        """qbit::make(ParsPumpedLossy, QM_Picture), CPPQEDelements/frees/Qbit.cc:217-246."""
+        eta = complex(eta)
+        return Element(f"free qbit deltaA={deltaA!r} gamma={gamma!r} eta={eta.real!r} etaI={eta.imag!r} "
+                       f"picture={picture}", 2)
+
+
+def pumped_two_level_atom_sch(gamma, deltaA, eta):
+    """PumpedTwoLevelAtomSch, CPPQEDelements/frees/PumpedTwoLevelAtom.cc:35-49."""
+    eta = complex(eta)
+    return Element(f"free ptla gamma={gamma!r} deltaA={deltaA!r} eta={eta.real!r} etaI={eta.imag!r}", 2)
+
+
+class Interaction:
+    def __init__(self, kind, frees, **pars):
+        self.kind, self.frees, self.pars = kind, list(frees), pars
+
+
+class jaynescummings:  # noqa: N801
+    @staticmethod
+    def make(q, m, g):
+        """jaynescummings::make, CPPQEDelements/interactions/JaynesCummings_.h:85-90."""
+        g = complex(g)
+        return Interaction("jc", [q, m], g=g.real, gI=g.imag)
+
+
+class coupledmodes:  # noqa: N801
+    @staticmethod
+    def make(m1, m2, u):
+        """coupledmodes (CM_XX_RWA), CPPQEDelements/interactions/CoupledModes.cc:36-41."""
+        u = complex(u)
+        return Interaction("bs", [m1, m2], g=u.real, gI=u.imag)
+
+
+def zz(q1, q2, J):
+    """J sigma_z (x) sigma_z between two qubits (new element: the reference lacks sigma_z)."""
+    return Interaction("zz", [q1, q2], J=J)
+
+
+class Act:
+    """composite::_<i,j>(interaction): the interaction acts on free indices (i, j)."""
+
+    def __init__(self, i, j, ia):
+        self.axes, self.ia = (i, j), ia
+
+
+def _ia_stmt(ia, axes):
+    kv = " ".join(f"{k}={v!r}" for k, v in ia.pars.items())
+    return f"ia {ia.kind} {axes[0]} {axes[1]} {kv}"
+
+
+class binary:  # noqa: N801
+    @staticmethod
+    def make(ia):
+        """binary::make, CPPQEDcore/composites/BinarySystem.cc:323-333."""
+        f0, f1 = ia.frees
+        return System("\n".join([f0.stmt, f1.stmt, _ia_stmt(ia, (0, 1)), "system binary"]))
+
+
+class composite:  # noqa: N801
+    @staticmethod
+    def make(frees, acts=()):
+        """composite::make(frees..., acts...), CPPQEDcore/composites/Composite.h:363-367."""
+        lines = [f.stmt for f in frees] + [_ia_stmt(a.ia, a.axes) for a in acts] + ["system composite"]
+        return System("\n".join(lines))
+
+
+# ---------------------------------------------------------------- system
+
+class System:
+    """A lowered system: host tables plus device copy (qt_sys)."""
+
+    def __init__(self, spec):
+        if isinstance(spec, Element):
+            spec = spec.stmt
+        self.spec = spec
+        h = C.c_void_p()
+        check(lib().qt_sys_build(spec.encode(), C.byref(h)))
+        self.h = h
+        info = SysInfo()
+        check(lib().qt_sys_info(self.h, C.byref(info)))
+        self.info = info
+        self.dim = info.dim
+        self.dims = tuple(info.dims[:info.rank])
+        self.n_jumps = info.n_jumps
+        self.n_avr = info.n_avr
+        self.highest_frequency = info.highest_frequency
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qt_sys_destroy(self.h)
+            self.h = None
+
+    def table(self, which=0):
+        """(cols[dim, w], vals[dim, w], freqs[dim, w]) of H_nH/i (0), J_m (1+m) or observable (1+n_jumps+q)."""
+        w = C.c_int()
+        check(lib().qt_sys_table(self.h, which, C.byref(w), None, None, None))
+        n, w = self.dim, w.value
+        cols = np.zeros(n * w, np.int32)
+        vals = np.zeros(n * w, np.complex128)
+        freqs = np.zeros(n * w, np.complex128)
+        check(lib().qt_sys_table(self.h, which, None, cols.ctypes.data, vals.ctypes.data, freqs.ctypes.data))
+        return cols.reshape(n, w), vals.reshape(n, w), freqs.reshape(n, w)
+
+    def dense(self, which=0):
+        """Dense matrix of table `which` assembled from its ELL rows (testing aid)."""
+        cols, vals, _ = self.table(which)
+        m = np.zeros((self.dim, self.dim), np.complex128)
+        rows = np.repeat(np.arange(self.dim), cols.shape[1])
+        np.add.at(m, (rows, cols.ravel()), vals.ravel())
+        return m
+
+    def apply_heff(self, psi_dev_ptr, dpsidt_dev_ptr, n=1, precision=QT_FP64, stream=None, t=0.0, t0=0.0):
+        check(lib().qt_apply_heff(self.h, t, t0, psi_dev_ptr, dpsidt_dev_ptr, n, precision, stream))
+
+
+# ---------------------------------------------------------------- run-loop intervals
+
+def run_intervals(T=None, Dt=0.1, NDt=0):
+    """Stream-interval lengths of trajectory::run in Dt mode (CPPQEDutils/Trajectory.h:560-595):
+    NDt > 0: NDt intervals of Dt; otherwise advance(min(Dt, T - t)) while t < T."""
+    if NDt:
+        return np.full(int(NDt), float(Dt))
+    out, t = [], 0.0
+    while t < T:
+        dt = min(Dt, T - t)
+        out.append(dt)
+        t = t + dt
+    return np.array(out)
+
+
+# ---------------------------------------------------------------- MCWF ensemble
+
+def mcwf_pars(epsRel=1e-6, epsAbs=1e-12, dpLimit=0.01, overshootTol=10.0, dtInit=0.0, precision=QT_FP64,
+              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0)):
+    """mcwf::Pars + ode_engine::Pars with the reference defaults (MCWF_Trajectory.h:28-47, ODE.h:31-63)."""
+    return McwfParams(epsRel, epsAbs, dpLimit, overshootTol, dtInit, precision, 0, seed, traj0,
+                      jump_log_cap, layout[0], layout[1])
+
+
+class Ensemble:
+    """Device-resident ensemble of MCWF trajectories (mcwf::makeEnsemble + trajectory::Ensemble)."""
+
+    def __init__(self, system, n_traj, psi0, pars=None):
+        self.system = system
+        self.n = int(n_traj)
+        self.pars = pars or mcwf_pars()
+        psi0, p = _cbuf(psi0)
+        if psi0.size != system.dim:
+            raise ValueError("initial state has the wrong dimension")
+        h = C.c_void_p()
+        check(lib().qt_mcwf_create(system.h, self.n, C.byref(self.pars), p, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qt_mcwf_destroy(self.h)
+            self.h = None
+
+    def advance(self, intervals):
+        """Advances through the stream intervals; returns rows (time, mean dtDid, processed averages)."""
+        iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
+        rows = np.zeros((len(iv), 2 + self.system.n_avr))
+        check(lib().qt_mcwf_advance(self.h, iv.ctypes.data, len(iv), rows.ctypes.data, None))
+        return rows
+
+    def reset(self, stream=None):
+        check(lib().qt_mcwf_reset(self.h, stream))
+
+    def advance_async(self, intervals, stream=None):
+        iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
+        check(lib().qt_mcwf_advance_async(self.h, iv.ctypes.data, len(iv), stream))
+
+    def traj_averages(self):
+        out = np.zeros((self.n, self.system.n_avr))
+        check(lib().qt_mcwf_traj_averages(self.h, out.ctypes.data))
+        return out
+
+    def state(self, i):
+        psi = np.zeros(self.system.dim, np.complex128)
+        t = C.c_double()
+        check(lib().qt_mcwf_state(self.h, i, psi.ctypes.data, C.byref(t)))
+        return psi, t.value
+
+    def jumplog(self, i, cap=4096):
+        t = np.zeros(cap)
+        ch = np.zeros(cap, np.int32)
+        n = C.c_int()
+        check(lib().qt_mcwf_jumplog(self.h, i, t.ctypes.data, ch.ctypes.data, cap, C.byref(n)))
+        m = min(n.value, cap, self.pars.jump_log_cap)
+        return t[:m], ch[:m], n.value
+
+    def counters(self):
+        per = np.zeros((self.n, 8), np.int64)
+        tot = np.zeros(8, np.int64)
+        check(lib().qt_mcwf_counters(self.h, per.ctypes.data, tot.ctypes.data))
+        return per, tot
+
+    def kernel_ms(self):
+        ms = C.c_double()
+        n = C.c_int()
+        check(lib().qt_mcwf_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+COUNTER_NAMES = ("mcwf_steps", "ode_steps", "failed", "derivs", "tries", "step_backs", "overshoots", "jumps")
+
+
+def mcwf_plain_run(system, psi0, intervals, pars=None, traj=0, jcap=4096):
+    """The unbatched GPU form: one trajectory, one launch per RKCK try (qt_mcwf_plain_run)."""
+    pars = pars or mcwf_pars()
+    psi0, p = _cbuf(psi0)
+    iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
+    obs = np.zeros((len(iv), system.n_avr))
+    psi = np.zeros(system.dim, np.complex128)
+    jt = np.zeros(jcap)
+    jch = np.zeros(jcap, np.int32)
+    cnt = np.zeros(8, np.int64)
+    check(lib().qt_mcwf_plain_run(system.h, C.byref(pars), traj, p, iv.ctypes.data, len(iv), obs.ctypes.data,
+                                  psi.ctypes.data, jt.ctypes.data, jch.ctypes.data, jcap, cnt.ctypes.data))
+    nj = int(min(cnt[7], jcap))
+    return dict(obs=obs, psi=psi, jump_t=jt[:nj], jump_ch=jch[:nj], counters=cnt)
+
+
+def process_averages(system_spec_obs_var, raw):
+    """Applies Averaged::process (VAR of the number operator) to raw linear averages."""
+    out = np.array(raw, dtype=np.float64, copy=True)
+    for q, v in enumerate(system_spec_obs_var):
+        if v >= 0:
+            out[..., q] -= out[..., v] ** 2
+    return out

@@ -1,0 +1,602 @@
+// qt_capi.cpp -- the C ABI (include/qtraj.h): system handles, device-resident MCWF
+// ensembles, the plain one-trajectory form.
+#include "../../include/qtraj.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qt_kernels.hpp"
+#include "qt_philox.h"
+#include "qt_system.hpp"
+
+namespace qt {
+bool layout_supported(int rows, int nw, int wh);
+hipError_t launch_batch(int rows, int nw, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st);
+hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
+                                double* out_dt, hipStream_t st);
+hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st);
+hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st);
+hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st);
+hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st);
+hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, const SysDev& S, const double2* psi, double2* dpsidt,
+                             int n, hipStream_t st);
+}  // namespace qt
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Fail(QT_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return QT_OK;
+  } catch (Fail& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (std::bad_alloc&) {
+    g_err = "out of host memory";
+    return QT_ENOMEM;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return QT_EINVAL;
+  }
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+    if (!count) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; throw Fail(QT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+    n = count;
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (!v.empty()) hipCheck(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
+};
+
+struct LayoutChoice { int rows, nw, wh; };
+
+// Layout table (rows per lane x wavefronts per group) by dimension; the ELL width is
+// rounded up to a compiled width.
+LayoutChoice pick_layout(long dim, int width, int rows_req, int nw_req) {
+  int wh = width <= 4 ? 4 : width <= 8 ? 8 : width <= 16 ? 16 : -1;
+  if (wh < 0) throw Fail(QT_ENOTSUP, "H_nH/i rows wider than 16 entries are not supported");
+  auto fits = [&](int r, int n) { return (long)r * 64 * n >= dim; };
+  if (rows_req > 0 && nw_req > 0) {
+    if (!fits(rows_req, nw_req)) throw Fail(QT_EINVAL, "requested layout does not cover the dimension");
+    for (int w : {wh, 8, 16}) if (w >= wh && qt::layout_supported(rows_req, nw_req, w)) return {rows_req, nw_req, w};
+    throw Fail(QT_ENOTSUP, "requested layout is not compiled in");
+  }
+  static const int cand[][2] = {{1, 1}, {2, 1}, {4, 1}, {1, 4}, {2, 4}, {4, 4}, {4, 8}, {2, 16}, {4, 16}};
+  for (auto& c : cand) {
+    if (!fits(c[0], c[1])) continue;
+    for (int w : {wh, 8, 16}) if (w >= wh && qt::layout_supported(c[0], c[1], w)) return {c[0], c[1], w};
+  }
+  throw Fail(QT_ENOTSUP, "dimension too large for the compiled layouts");
+}
+
+void check_params(const qt_mcwf_params* p) {
+  if (!p) throw Fail(QT_EINVAL, "null parameters");
+  if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
+  if (!(p->dpLimit > 0) || !(p->overshootTol >= 1)) throw Fail(QT_EINVAL, "dpLimit > 0 and overshootTol >= 1 required");
+  if (p->precision != QT_FP64 && p->precision != QT_FP32_DERIV) throw Fail(QT_EINVAL, "unknown precision");
+  if (p->controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "only the odeint controller runs on the device");
+}
+
+}  // namespace
+
+struct qt_sys {
+  std::unique_ptr<qt::System> h;
+  bool uploaded = false;
+  DevBuf<int> hcol, jcol, ocol, oimag;
+  DevBuf<double2> hval, jval, oval;
+  DevBuf<float2> hval32;
+  qt::SysDev dev{};
+
+  void upload() {
+    if (uploaded) return;
+    const qt::System& s = *h;
+    const int d = (int)s.dim;
+    auto colmajor = [d](const qt::Ell& e, std::vector<int>& c, std::vector<double2>& v) {
+      for (int w = 0; w < e.width; ++w)
+        for (int r = 0; r < d; ++r) {
+          c.push_back(e.cols[(size_t)r * e.width + w]);
+          const qt::cd z = e.vals[(size_t)r * e.width + w];
+          v.push_back(make_double2(z.real(), z.imag()));
+        }
+    };
+    std::vector<int> c; std::vector<double2> v;
+    colmajor(s.H, c, v);
+    hcol.upload(c); hval.upload(v);
+    std::vector<float2> v32;
+    for (auto& z : v) v32.push_back(make_float2((float)z.x, (float)z.y));
+    hval32.upload(v32);
+    int wj = 1;
+    for (auto& j : s.J) wj = std::max(wj, j.width);
+    c.clear(); v.clear();
+    for (auto& j : s.J) {
+      qt::Ell p = j;  // pad to the common width
+      if (p.width < wj) {
+        qt::Ell q; q.width = wj; q.cols.assign((size_t)d * wj, 0); q.vals.assign((size_t)d * wj, 0.);
+        for (int r = 0; r < d; ++r)
+          for (int w = 0; w < wj; ++w) {
+            q.cols[(size_t)r * wj + w] = w < p.width ? p.cols[(size_t)r * p.width + w] : r;
+            q.vals[(size_t)r * wj + w] = w < p.width ? p.vals[(size_t)r * p.width + w] : 0.;
+          }
+        p = q;
+      }
+      colmajor(p, c, v);
+    }
+    if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); }
+    jcol.upload(c); jval.upload(v);
+    int wo = 1;
+    for (auto& o : s.O) wo = std::max(wo, o.width);
+    c.clear(); v.clear();
+    std::vector<int> im;
+    for (size_t q = 0; q < s.O.size(); ++q) {
+      const qt::Ell& o = s.O[q];
+      for (int w = 0; w < wo; ++w)
+        for (int r = 0; r < d; ++r) {
+          c.push_back(w < o.width ? o.cols[(size_t)r * o.width + w] : r);
+          const qt::cd z = w < o.width ? o.vals[(size_t)r * o.width + w] : qt::cd(0.);
+          v.push_back(make_double2(z.real(), z.imag()));
+        }
+      im.push_back(s.obs[q].imagPart ? 1 : 0);
+    }
+    if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); im.push_back(0); }
+    ocol.upload(c); oval.upload(v); oimag.upload(im);
+    dev.dim = d; dev.wh = s.H.width;
+    dev.hcol = hcol.p; dev.hval = hval.p; dev.hval32 = hval32.p;
+    dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
+    dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p;
+    uploaded = true;
+  }
+};
+
+struct qt_ens {
+  qt_sys* sys = nullptr;
+  int n = 0;
+  qt_mcwf_params p{};
+  LayoutChoice lay{};
+  double time = 0.;           // common time of all trajectories (they all stop at each endTime)
+  DevBuf<double2> psi, psiInit;
+  DevBuf<double> t, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt;
+  DevBuf<unsigned long long> ord;
+  DevBuf<long long> cnt;
+  DevBuf<int> jch, work, errflag;
+  int lastIntervals = 0;
+  int grid = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double kernelMs = 0.;
+  int launches = 0;
+  ~qt_ens() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+
+  qt::EnsDev view() {
+    qt::EnsDev E{};
+    E.n_traj = n; E.psi = psi.p; E.t = t.p; E.dtDid = dtDid.p; E.dtTry = dtTry.p; E.ordinal = ord.p;
+    E.seed = p.seed; E.traj0 = p.traj0; E.counters = cnt.p; E.jt = jt.p; E.jch = jch.p; E.jcap = p.jump_log_cap;
+    E.epsAbs = p.epsRel; E.epsRel = p.epsAbs;  // ODE.h:205 hands (epsRel, epsAbs) to make_controlled(abs, rel, ...)
+    E.dpLimit = p.dpLimit; E.overshootTol = p.overshootTol;
+    E.work = work.p; E.err_flag = errflag.p;
+    return E;
+  }
+};
+
+namespace {
+
+// rates of a host state through the host ELL tables: ||J_m psi||^2
+std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& psi) {
+  std::vector<double> r;
+  for (auto& j : s.J) {
+    double acc = 0.;
+    for (long row = 0; row < s.dim; ++row) {
+      qt::cd y = 0.;
+      for (int w = 0; w < j.width; ++w) y += j.vals[row * j.width + w] * psi[j.cols[row * j.width + w]];
+      acc += std::norm(y);
+    }
+    r.push_back(acc);
+  }
+  return r;
+}
+
+int resident_groups(const LayoutChoice& L) {
+  int dev = 0;
+  hipCheck(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t prop;
+  hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  // up to 8 single-wave groups per SIMD-quad; registers bound the real residency,
+  // the work queue absorbs any excess
+  int per_cu = std::max(1, 32 / L.nw);
+  return prop.multiProcessorCount * per_cu;
+}
+
+void process_rows(const qt::System& s, double* av) {
+  for (size_t q = 0; q < s.processVarOf.size(); ++q)
+    if (s.processVarOf[q] >= 0) av[q] -= av[s.processVarOf[q]] * av[s.processVarOf[q]];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qt_last_error(void) { return g_err.c_str(); }
+
+int qt_sys_build(const char* spec, qt_sys** out) {
+  return guard([&] {
+    if (!spec || !out) throw Fail(QT_EINVAL, "null argument");
+    std::unique_ptr<qt_sys> s(new qt_sys);
+    s->h.reset(qt::buildFromSpec(spec));
+    *out = s.release();
+  });
+}
+
+int qt_sys_destroy(qt_sys* sys) { delete sys; return QT_OK; }
+
+int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
+  return guard([&] {
+    if (!sys || !info) throw Fail(QT_EINVAL, "null argument");
+    const qt::System& s = *sys->h;
+    std::memset(info, 0, sizeof(*info));
+    info->dim = s.dim;
+    info->rank = (int)s.dims.size();
+    for (size_t i = 0; i < s.dims.size() && i < 16; ++i) info->dims[i] = s.dims[i];
+    info->n_jumps = (int)s.J.size();
+    info->n_avr = (int)s.O.size();
+    info->h_width = s.H.width;
+    int wj = 0; for (auto& j : s.J) wj = std::max(wj, j.width);
+    int wo = 0; for (auto& o : s.O) wo = std::max(wo, o.width);
+    info->j_width = wj; info->o_width = wo;
+    info->time_dependent = s.H.timeDependent() || s.hasEx;
+    info->highest_frequency = s.highestFrequency;
+    long long nnz = 0;
+    for (auto& z : s.H.vals) if (z != qt::cd(0.)) ++nnz;
+    info->h_nnz = nnz;
+  });
+}
+
+int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* vals, double* freqs) {
+  return guard([&] {
+    const qt::System& s = *sys->h;
+    const qt::Ell* e = nullptr;
+    if (which == 0) e = &s.H;
+    else if (which - 1 < (int)s.J.size()) e = &s.J[which - 1];
+    else if (which - 1 - (int)s.J.size() < (int)s.O.size()) e = &s.O[which - 1 - s.J.size()];
+    else throw Fail(QT_EINVAL, "table index out of range");
+    if (width) *width = e->width;
+    size_t n = (size_t)s.dim * e->width;
+    for (size_t i = 0; i < n; ++i) {
+      if (cols) cols[i] = e->cols[i];
+      if (vals) { vals[2 * i] = e->vals[i].real(); vals[2 * i + 1] = e->vals[i].imag(); }
+      if (freqs) {
+        qt::cd f = e->freqs.empty() ? qt::cd(0.) : e->freqs[i];
+        freqs[2 * i] = f.real(); freqs[2 * i + 1] = f.imag();
+      }
+    }
+  });
+}
+
+int qt_apply_heff(qt_sys* sys, double t, double t0, const double* psi, double* dpsidt, int n, int precision,
+                  qt_stream stream) {
+  (void)t; (void)t0;
+  return guard([&] {
+    if (!sys || !psi || !dpsidt || n <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    if (sys->h->H.timeDependent() || sys->h->hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    sys->upload();
+    LayoutChoice L = pick_layout(sys->h->dim, sys->h->H.width, 0, 0);
+    hipCheck(qt::launch_apply_heff(L.rows, L.nw, L.wh, precision == QT_FP32_DERIV, sys->dev, (const double2*)psi,
+                                   (double2*)dpsidt, n, (hipStream_t)stream), "apply_heff launch");
+  });
+}
+
+int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const double* psi0, qt_ens** out) {
+  return guard([&] {
+    if (!sys || !psi0 || !out || n_traj <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    check_params(p);
+    const qt::System& s = *sys->h;
+    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
+    sys->upload();
+    std::unique_ptr<qt_ens> e(new qt_ens);
+    e->sys = sys; e->n = n_traj; e->p = *p;
+    if (e->p.jump_log_cap < 1) e->p.jump_log_cap = 1;
+    e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
+    const long d = s.dim;
+    std::vector<qt::cd> psi(d);
+    for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
+    double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
+    // constructor check (MCWF_Trajectory.h:114-117): dtTry <= dpLimit / total rate at t = 0
+    double dtTry = dtInit;
+    if (!s.J.empty()) {
+      auto r = host_rates(s, psi);
+      double tot = 0.; for (double x : r) tot += x;
+      dtTry = std::min(dtTry, p->dpLimit / tot);
+    }
+    e->psi.alloc((size_t)n_traj * d);
+    std::vector<double2> rep((size_t)n_traj * d);
+    for (int k = 0; k < n_traj; ++k)
+      for (long i = 0; i < d; ++i) rep[(size_t)k * d + i] = make_double2(psi[i].real(), psi[i].imag());
+    e->psi.upload(rep);
+    e->psiInit.upload(rep);
+    e->t.upload(std::vector<double>(n_traj, 0.));
+    e->dtDid.upload(std::vector<double>(n_traj, 0.));
+    e->dtTry.upload(std::vector<double>(n_traj, dtTry));
+    e->dtTryInit.upload(std::vector<double>(n_traj, dtTry));
+    e->ord.upload(std::vector<unsigned long long>(n_traj, 0ull));
+    e->cnt.upload(std::vector<long long>((size_t)n_traj * qt::kCounters, 0));
+    e->jt.alloc((size_t)n_traj * e->p.jump_log_cap);
+    e->jch.alloc((size_t)n_traj * e->p.jump_log_cap);
+    e->work.alloc(1);
+    e->errflag.upload(std::vector<int>(1, 0));
+    e->grid = std::min(n_traj, resident_groups(e->lay));
+    hipCheck(hipEventCreate(&e->e0), "hipEventCreate");
+    hipCheck(hipEventCreate(&e->e1), "hipEventCreate");
+    *out = e.release();
+  });
+}
+
+int qt_mcwf_destroy(qt_ens* ens) { delete ens; return QT_OK; }
+
+static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool timed) {
+  const qt::System& s = *e->sys->h;
+  if (n_int <= 0) throw Fail(QT_EINVAL, "no intervals");
+  for (int k = 0; k < n_int; ++k) if (!(intervals[k] >= 0) || !std::isfinite(intervals[k])) throw Fail(QT_EINVAL, "intervals must be finite and non-negative");
+  e->intervals.alloc(n_int);
+  hipCheck(hipMemcpyAsync(e->intervals.p, intervals, n_int * sizeof(double), hipMemcpyHostToDevice, st), "intervals H2D");
+  const int no = std::max(1, (int)s.O.size());
+  e->obs.alloc((size_t)e->n * n_int * no);
+  e->dtd.alloc((size_t)e->n * n_int);
+  hipCheck(hipMemsetAsync(e->work.p, 0, sizeof(int), st), "reset work queue");
+  qt::EnsDev E = e->view();
+  E.intervals = e->intervals.p; E.n_intervals = n_int;
+  E.obs_out = e->obs.p; E.dtdid_out = e->dtd.p;
+  if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
+  hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
+           "mcwf batch launch");
+  if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
+  e->lastIntervals = n_int;
+  for (int k = 0; k < n_int; ++k) e->time += intervals[k];
+}
+
+int qt_mcwf_advance(qt_ens* e, const double* intervals, int n_int, double* rows, qt_stream stream) {
+  return guard([&] {
+    if (!e || !intervals) throw Fail(QT_EINVAL, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const qt::System& s = *e->sys->h;
+    const double tStart = e->time;
+    ens_launch(e, intervals, n_int, st, true);
+    const int no = (int)s.O.size();
+    e->mean.alloc((size_t)n_int * std::max(1, no));
+    e->meanDt.alloc(n_int);
+    hipCheck(qt::launch_ensemble_mean(e->obs.p, e->dtd.p, e->n, n_int, no, e->mean.p, e->meanDt.p, st), "mean launch");
+    std::vector<double> m((size_t)n_int * std::max(1, no)), md(n_int);
+    hipCheck(hipMemcpyAsync(m.data(), e->mean.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(md.data(), e->meanDt.p, md.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    int flag = 0;
+    hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    float ms = 0.f;
+    hipCheck(hipEventElapsedTime(&ms, e->e0, e->e1), "hipEventElapsedTime");
+    e->kernelMs += ms; e->launches += 1;
+    if (flag) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+    if (rows) {
+      double tt = tStart;
+      for (int k = 0; k < n_int; ++k) {
+        tt += intervals[k];
+        double* r = rows + (size_t)k * (2 + no);
+        r[0] = tt; r[1] = md[k];
+        for (int q = 0; q < no; ++q) r[2 + q] = m[(size_t)k * no + q];
+        process_rows(s, r + 2);
+      }
+    }
+  });
+}
+
+int qt_mcwf_reset(qt_ens* e, qt_stream stream) {
+  return guard([&] {
+    if (!e) throw Fail(QT_EINVAL, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = e->n, d = e->sys->h->dim;
+    hipCheck(hipMemcpyAsync(e->psi.p, e->psiInit.p, n * d * sizeof(double2), hipMemcpyDeviceToDevice, st), "reset psi");
+    hipCheck(hipMemcpyAsync(e->dtTry.p, e->dtTryInit.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), "reset dtTry");
+    hipCheck(hipMemsetAsync(e->t.p, 0, n * sizeof(double), st), "reset t");
+    hipCheck(hipMemsetAsync(e->dtDid.p, 0, n * sizeof(double), st), "reset dtDid");
+    hipCheck(hipMemsetAsync(e->ord.p, 0, n * sizeof(unsigned long long), st), "reset ordinal");
+    hipCheck(hipMemsetAsync(e->cnt.p, 0, n * qt::kCounters * sizeof(long long), st), "reset counters");
+    e->time = 0.;
+  });
+}
+
+int qt_mcwf_advance_async(qt_ens* e, const double* intervals, int n_int, qt_stream stream) {
+  return guard([&] {
+    if (!e || !intervals) throw Fail(QT_EINVAL, "null argument");
+    ens_launch(e, intervals, n_int, (hipStream_t)stream, false);
+  });
+}
+
+int qt_mcwf_traj_averages(qt_ens* e, double* out) {
+  return guard([&] {
+    const qt::System& s = *e->sys->h;
+    const int no = (int)s.O.size();
+    if (!e->lastIntervals) throw Fail(QT_EINVAL, "no interval advanced yet");
+    std::vector<double> all((size_t)e->n * e->lastIntervals * std::max(1, no));
+    hipCheck(hipMemcpy(all.data(), e->obs.p, all.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    for (int i = 0; i < e->n; ++i)
+      for (int q = 0; q < no; ++q) out[(size_t)i * no + q] = all[((size_t)i * e->lastIntervals + e->lastIntervals - 1) * no + q];
+  });
+}
+
+int qt_mcwf_state(qt_ens* e, int i, double* psi, double* t) {
+  return guard([&] {
+    if (i < 0 || i >= e->n) throw Fail(QT_EINVAL, "trajectory index out of range");
+    const long d = e->sys->h->dim;
+    if (psi) hipCheck(hipMemcpy(psi, e->psi.p + (size_t)i * d, d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
+    if (t) hipCheck(hipMemcpy(t, e->t.p + i, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+  });
+}
+
+int qt_mcwf_jumplog(qt_ens* e, int i, double* t, int* ch, int cap, int* n) {
+  return guard([&] {
+    if (i < 0 || i >= e->n) throw Fail(QT_EINVAL, "trajectory index out of range");
+    long long c[qt::kCounters];
+    hipCheck(hipMemcpy(c, e->cnt.p + (size_t)i * qt::kCounters, sizeof(c), hipMemcpyDeviceToHost), "D2H");
+    int nj = (int)std::min<long long>(c[qt::kJumps], e->p.jump_log_cap);
+    int m = std::min(nj, cap);
+    if (m > 0) {
+      if (t) hipCheck(hipMemcpy(t, e->jt.p + (size_t)i * e->p.jump_log_cap, m * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+      if (ch) hipCheck(hipMemcpy(ch, e->jch.p + (size_t)i * e->p.jump_log_cap, m * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    }
+    if (n) *n = (int)c[qt::kJumps];
+  });
+}
+
+int qt_mcwf_counters(qt_ens* e, long long* per, long long* totals) {
+  return guard([&] {
+    std::vector<long long> c((size_t)e->n * qt::kCounters);
+    hipCheck(hipMemcpy(c.data(), e->cnt.p, c.size() * sizeof(long long), hipMemcpyDeviceToHost), "D2H");
+    if (per) std::memcpy(per, c.data(), c.size() * sizeof(long long));
+    if (totals) {
+      for (int k = 0; k < qt::kCounters; ++k) totals[k] = 0;
+      for (int i = 0; i < e->n; ++i) for (int k = 0; k < qt::kCounters; ++k) totals[k] += c[(size_t)i * qt::kCounters + k];
+    }
+  });
+}
+
+int qt_mcwf_kernel_ms(qt_ens* e, double* total, int* launches) {
+  if (total) *total = e->kernelMs;
+  if (launches) *launches = e->launches;
+  return QT_OK;
+}
+
+int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long traj, const double* psi0,
+                      const double* intervals, int n_int, double* obs_host, double* psi_out, double* jt, int* jch,
+                      int jcap, long long* counters) {
+  return guard([&] {
+    if (!sys || !psi0 || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    check_params(p);
+    const qt::System& s = *sys->h;
+    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
+    sys->upload();
+    LayoutChoice L = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
+    const long d = s.dim;
+    const int nj = (int)s.J.size(), no = (int)s.O.size();
+    hipStream_t st = nullptr;
+    DevBuf<double2> x, xn, xc;
+    DevBuf<double> err, post, obs;
+    x.alloc(d); xn.alloc(d); xc.alloc(d); err.alloc(1); post.alloc(1 + std::max(1, nj)); obs.alloc(std::max(1, no));
+    hipCheck(hipMemcpy(x.p, psi0, d * sizeof(double2), hipMemcpyHostToDevice), "H2D");
+    std::vector<qt::cd> psi(d);
+    for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
+    double t = 0., dtDid = 0., dtTry = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
+    const double epsAbs = p->epsRel, epsRel = p->epsAbs;  // ODE.h:205
+    long long c[qt::kCounters] = {0};
+    if (nj) {
+      auto r = host_rates(s, psi);
+      double tot = 0.; for (double v : r) tot += v;
+      dtTry = std::min(dtTry, p->dpLimit / tot);
+    }
+    unsigned long long ord = 0;
+    std::vector<double> hp(1 + std::max(1, nj));
+    auto coherent = [&](double Dt) {
+      const double timeBefore = t;
+      const double nextDtTry = (std::fabs(Dt) < std::fabs(dtTry / 10.) ? dtTry : 0.);
+      if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
+      if (std::fabs(dtTry) > std::fabs(Dt)) dtTry = Dt;
+      for (;;) {
+        qt::PlainTry P{(int)d, x.p, xn.p, err.p, t, dtTry, epsAbs, epsRel};
+        hipCheck(qt::launch_plain_try(L.rows, L.nw, L.wh, sys->dev, P, st), "plain try");
+        double e = 0.;
+        hipCheck(hipMemcpy(&e, err.p, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        c[qt::kTries]++; c[qt::kDerivs] += 6;
+        if (e > 1.0) { dtTry *= std::max(9. / 10. * std::pow(e, -1. / (4 - 1)), 1. / 5.); c[qt::kFailed]++; continue; }
+        t += dtTry;
+        if (e < 0.5) { e = std::max(std::pow(5.0, -5.0), e); dtTry *= 9. / 10. * std::pow(e, -1. / 5); }
+        std::swap(x.p, xn.p);
+        break;
+      }
+      dtDid = t - timeBefore;
+      if (nextDtTry != 0.) dtTry = nextDtTry;
+      c[qt::kOdeSteps]++;
+      hipCheck(qt::launch_plain_post(L.rows, L.nw, L.wh, sys->dev, x.p, post.p, st), "plain post");
+      hipCheck(hipMemcpy(hp.data(), post.p, (1 + nj) * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    };
+    double now = 0.;
+    for (int k = 0; k < n_int; ++k) {
+      const double endTime = now + intervals[k];
+      for (;;) {
+        const double Dt = endTime - t;
+        if (Dt == 0.) break;
+        hipCheck(hipMemcpy(xc.p, x.p, d * sizeof(double2), hipMemcpyDeviceToDevice), "D2D");
+        const double tCache = t, dtDidCache = dtDid;
+        coherent(Dt);
+        if (nj) {
+          for (;;) {
+            double tot = 0.;
+            for (int m = 0; m < nj; ++m) tot += hp[1 + m];
+            const double sugg = p->dpLimit / tot;
+            if (tot * dtDid > p->overshootTol * p->dpLimit) {
+              t = tCache; dtDid = dtDidCache; dtTry = sugg; c[qt::kStepBacks]++;
+              hipCheck(hipMemcpy(x.p, xc.p, d * sizeof(double2), hipMemcpyDeviceToDevice), "D2D");
+              coherent(Dt);
+              continue;
+            } else if (tot * dtTry > p->dpLimit) c[qt::kOvershoots]++;
+            dtTry = std::min(dtTry, sugg);
+            break;
+          }
+          double random = qt::jump_uniform(p->seed, traj, ord++) / dtDid;
+          int no_ = 0;
+          for (; random > 0. && no_ != nj; random -= hp[1 + no_++]) ;
+          if (random < 0.) {
+            --no_;
+            const double nf = std::sqrt(hp[1 + no_]);
+            if (!std::isfinite(nf)) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+            hipCheck(qt::launch_plain_jump(L.rows, L.nw, L.wh, sys->dev, x.p, no_, nf, st), "plain jump");
+            if (jt && c[qt::kJumps] < jcap) jt[c[qt::kJumps]] = t;
+            if (jch && c[qt::kJumps] < jcap) jch[c[qt::kJumps]] = no_;
+            c[qt::kJumps]++;
+          }
+        }
+        c[qt::kMcwfSteps]++;
+      }
+      now = endTime;
+      if (obs_host && no) {
+        hipCheck(qt::launch_plain_obs(L.rows, L.nw, L.wh, sys->dev, x.p, obs.p, st), "plain obs");
+        hipCheck(hipMemcpy(obs_host + (size_t)k * no, obs.p, no * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+      }
+    }
+    if (psi_out) hipCheck(hipMemcpy(psi_out, x.p, d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
+    if (counters) std::memcpy(counters, c, sizeof(c));
+  });
+}
+
+}  // extern "C"

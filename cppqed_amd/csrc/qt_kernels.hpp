@@ -1,0 +1,75 @@
+// qt_kernels.hpp -- argument blocks shared by the HIP kernels and the host launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qt {
+
+constexpr int kMaxJumps = 8;      // jump channels handled by the batched kernel
+constexpr int kCounters = 8;      // per-trajectory counters, see Counter
+
+enum Counter {
+  kMcwfSteps = 0,   // MCWF_Trajectory::step calls
+  kOdeSteps = 1,    // successful ODE steps (including redone ones after a step-back)
+  kFailed = 2,      // rejected RKCK tries
+  kDerivs = 3,      // H_nH/i applications
+  kTries = 4,       // RKCK try_step calls (accepted + rejected): the throughput unit
+  kStepBacks = 5,   // dpLimit overshoot step-backs (MCWF_Trajectory.h:304-308)
+  kOvershoots = 6,  // dtTry capped by dpLimit (MCWF_Trajectory.h:309-311)
+  kJumps = 7,
+};
+
+// Device view of a lowered system (ELL tables, channel-major for jumps/observables).
+struct SysDev {
+  int dim;
+  int wh;                  // H row width (padded)
+  const int* hcol;         // [wh][dim] (column-major ELL: lanes read consecutive rows)
+  const double2* hval;     // [wh][dim]
+  const float2* hval32;    // [wh][dim]
+  int nj, wj;
+  const int* jcol;         // [nj][wj][dim]
+  const double2* jval;
+  int no, wo;
+  const int* ocol;         // [no][wo][dim]
+  const double2* oval;
+  const int* oimag;        // [no]
+};
+
+// Device view of an MCWF ensemble and the per-launch work description.
+struct EnsDev {
+  int n_traj;
+  double2* psi;                 // [n_traj][dim]
+  double* t;                    // [n_traj]
+  double* dtDid;
+  double* dtTry;
+  unsigned long long* ordinal;  // RNG counter (MCWF step ordinal)
+  unsigned long long seed;
+  unsigned long long traj0;     // global index of trajectory 0 (RNG key)
+  long long* counters;          // [n_traj][kCounters]
+  double* jt;                   // [n_traj][jcap]
+  int* jch;
+  int jcap;
+  double epsAbs, epsRel;        // error-checker constants (already in odeint's order)
+  double dpLimit, overshootTol;
+  // per launch
+  const double* intervals;      // advance lengths (Trajectory.h:43-50 semantics)
+  int n_intervals;
+  double* obs_out;              // [n_traj][n_intervals][no]
+  double* dtdid_out;            // [n_traj][n_intervals]
+  int* work;                    // work-queue head (trajectories are claimed in order)
+  int status_bad;               // unused
+  int* err_flag;                // set to 1 when a trajectory hits a non-finite jump norm
+};
+
+// One RKCK try of one trajectory (the plain form): x -> xn, err
+struct PlainTry {
+  int dim;
+  const double2* x;
+  double2* xn;
+  double* err;       // [1]
+  double t, dt;
+  double epsAbs, epsRel;
+};
+
+}  // namespace qt

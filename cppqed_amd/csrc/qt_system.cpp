@@ -1,0 +1,414 @@
+// qt_system.cpp -- element construction and lowering to ELL tables (host).
+#include "qt_system.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+
+namespace qt {
+
+// ---------------------------------------------------------------------------
+// sparse local-operator algebra
+// ---------------------------------------------------------------------------
+
+LocalOp kron(const LocalOp& a, const LocalOp& b) {
+  LocalOp r;
+  r.dims = a.dims;
+  r.dims.insert(r.dims.end(), b.dims.begin(), b.dims.end());
+  int nb = b.size();
+  for (auto& x : a.entries)
+    for (auto& y : b.entries)
+      r.entries.push_back({x.row * nb + y.row, x.col * nb + y.col, x.val * y.val, x.freq + y.freq});
+  return r;
+}
+
+LocalOp adjoint(const LocalOp& a) {
+  LocalOp r;
+  r.dims = a.dims;
+  for (auto& x : a.entries) r.entries.push_back({x.col, x.row, std::conj(x.val), -x.freq});
+  return r;
+}
+
+LocalOp scale(const LocalOp& a, cd f) {
+  LocalOp r = a;
+  for (auto& x : r.entries) x.val = x.val * f;
+  return r;
+}
+
+LocalOp plus(const LocalOp& a, const LocalOp& b) {
+  if (a.dims != b.dims) throw std::invalid_argument("operator dimension mismatch");
+  LocalOp r = a;
+  r.entries.insert(r.entries.end(), b.entries.begin(), b.entries.end());
+  return r;
+}
+
+static LocalOp diagonal(int dim, const std::vector<cd>& d) {
+  LocalOp r; r.dims = {dim};
+  for (int n = 0; n < dim; ++n) r.entries.push_back({n, n, d[n], 0.});
+  return r;
+}
+
+// annihilation operator a = sum_n sqrt(n+1) |n><n+1|   (Mode.cc:277-282)
+static LocalOp destroy(int dim) {
+  LocalOp r; r.dims = {dim};
+  for (int n = 0; n + 1 < dim; ++n) r.entries.push_back({n, n + 1, cd(std::sqrt(n + 1.), 0.), 0.});
+  return r;
+}
+
+static bool nonzero(cd z) { return z.real() != 0. || z.imag() != 0.; }
+
+// H/i of a (possibly pumped, damped) mode: -z n  +  eta a^dagger - eta^* a   (Mode.cc:56-86)
+static LocalOp modeHOverI(cd z, cd eta, int dim, bool& empty) {
+  LocalOp r; r.dims = {dim};
+  empty = true;
+  if (nonzero(z)) {
+    std::vector<cd> d(dim);
+    for (int n = 0; n < dim; ++n) d[n] = cd((double)n, 0.) * (-z);
+    r = plus(r, diagonal(dim, d)); empty = false;
+  }
+  if (nonzero(eta)) {
+    LocalOp ad = scale(adjoint(destroy(dim)), eta);
+    r = plus(r, plus(ad, scale(adjoint(ad), -1.)));
+    empty = false;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// spec parsing
+// ---------------------------------------------------------------------------
+
+using KV = std::map<std::string, std::string>;
+
+static double num(const KV& kv, const std::string& k, double def) {
+  auto it = kv.find(k);
+  return it == kv.end() ? def : std::stod(it->second);
+}
+
+static std::string picture(const KV& kv) {
+  auto it = kv.find("picture");
+  std::string p = it == kv.end() ? "Sch" : it->second;
+  if (p != "Sch" && p != "IP" && p != "UIP") throw std::invalid_argument("unknown picture " + p);
+  return p;
+}
+
+struct Builder {
+  System& s;
+  explicit Builder(System& sys) : s(sys) {}
+  std::vector<std::vector<Term>> freeJumps;  // per free
+  std::vector<std::vector<Observable>> freeObs;
+  std::vector<std::vector<int>> freeVar;
+  std::vector<std::vector<Term>> iaJumps;
+
+  void avMode(int ax, int dim) {
+    std::vector<cd> n(dim), n2(dim);
+    for (int k = 0; k < dim; ++k) { n[k] = (double)k; n2[k] = (double)k * k; }
+    Observable on, on2, are, aim;
+    on.terms.push_back({{ax}, diagonal(dim, n)});
+    on2.terms.push_back({{ax}, diagonal(dim, n2)});
+    are.terms.push_back({{ax}, destroy(dim)});
+    aim = are; aim.imagPart = true;
+    freeObs.back() = {on, on2, are, aim};
+    freeVar.back() = {-1, 0, -1, -1};  // VAR(n) = <n^2> - <n>^2  (Mode.cc:198-201)
+  }
+  void avQbit(int ax) {
+    Observable p0, p1, sre, sim;
+    p0.terms.push_back({{ax}, diagonal(2, {1., 0.})});
+    p1.terms.push_back({{ax}, diagonal(2, {0., 1.})});
+    LocalOp sig; sig.dims = {2}; sig.entries.push_back({0, 1, 1., 0.});
+    sre.terms.push_back({{ax}, sig});
+    sim = sre; sim.imagPart = true;
+    freeObs.back() = {p0, p1, sre, sim};
+    freeVar.back() = {-1, -1, -1, -1};
+  }
+
+  // mode::make(ParsPumpedLossy, picture) for nTh = 0 (Mode.tcc:40-60, 160-197)
+  void mode(const KV& kv) {
+    int ax = (int)s.frees.size();
+    int dim = (int)num(kv, "cutoff", 10);
+    double delta = num(kv, "delta", -10), kappa = num(kv, "kappa", 0);
+    cd eta(num(kv, "eta", 0), num(kv, "etaI", 0));
+    std::string pic = picture(kv);
+    FreeInfo f; f.kind = "mode"; f.dim = dim; f.avKind = "mode";
+    cd z(kappa, -delta);
+    bool empty;
+    LocalOp h;
+    double hf = 0.;
+    if (pic == "Sch") {
+      h = modeHOverI(z, eta, dim, empty);
+      if (kappa && nonzero(eta)) hf = std::abs(std::conj(z) * (double)dim);
+      else if (kappa) hf = std::max(std::fabs(kappa * dim), std::fabs(delta * dim));
+      else hf = std::fabs(delta * dim);
+      if (nonzero(eta)) hf = std::max(hf, std::abs(eta * std::sqrt((double)dim)));
+    } else {
+      if (!kappa || !nonzero(eta)) throw std::invalid_argument("interaction-picture mode needs kappa and eta");
+      f.hasEx = true;
+      if (pic == "IP") { h = modeHOverI(0., eta, dim, empty); f.zI = z; hf = std::abs(std::conj(z)); }
+      else { h = modeHOverI(cd(kappa, 0.), eta, dim, empty); f.zI = cd(0., -delta); hf = std::max(std::fabs(kappa * dim), std::fabs(delta)); }
+      hf = std::max(hf, std::abs(eta * std::sqrt((double)dim)));
+    }
+    f.hasHa = true;  // every mode::Hamiltonian carries at least the (zero) Kerr term (Mode.cc:92-107)
+    f.highestFrequency = hf;
+    s.hTerms.push_back({{ax}, h});
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    if (kappa) {  // sqrt(2 kappa) a  (Mode.cc:118-125)
+      LocalOp j; j.dims = {dim};
+      double fact = std::sqrt(2. * kappa);
+      for (int n = 0; n + 1 < dim; ++n) j.entries.push_back({n, n + 1, cd(fact * std::sqrt(n + 1.), 0.), 0.});
+      freeJumps.back().push_back({{ax}, j});
+      f.nJumps = 1;
+    }
+    avMode(ax, dim);
+    s.frees.push_back(f);
+  }
+
+  // qbit::make(ParsPumpedLossy, picture): a two-level mode pumped with -eta (Qbit.cc:217-246, Qbit_.h:99-118)
+  void qbit(const KV& kv) {
+    int ax = (int)s.frees.size();
+    double delta = num(kv, "deltaA", -10), gamma = num(kv, "gamma", 0);
+    cd eta(num(kv, "eta", 0), num(kv, "etaI", 0));
+    std::string pic = picture(kv);
+    FreeInfo f; f.kind = "qbit"; f.dim = 2; f.avKind = "qbit";
+    cd z(gamma, -delta);
+    bool empty;
+    LocalOp h;
+    double hf = 0.;
+    if (pic == "Sch") {
+      h = modeHOverI(z, -eta, 2, empty);
+      if (gamma && nonzero(eta)) hf = std::max(std::abs(std::conj(z)), std::abs(eta));
+      else if (gamma) hf = std::max(std::fabs(gamma), std::fabs(delta));
+      else hf = std::max(std::fabs(delta), std::abs(eta));
+    } else {
+      if (!gamma || !nonzero(eta)) throw std::invalid_argument("interaction-picture qbit needs gamma and eta");
+      f.hasEx = true;
+      if (pic == "IP") { h = modeHOverI(0., -eta, 2, empty); f.zI = z; hf = std::abs(std::conj(z)); }
+      else { h = modeHOverI(cd(gamma, 0.), -eta, 2, empty); f.zI = cd(0., -delta); hf = std::max(std::fabs(gamma), std::fabs(delta)); }
+      hf = std::max(hf, std::abs(eta));
+    }
+    f.hasHa = true;
+    f.highestFrequency = hf;
+    s.hTerms.push_back({{ax}, h});
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    if (gamma) {
+      LocalOp j; j.dims = {2};
+      j.entries.push_back({0, 1, cd(std::sqrt(2. * gamma) * std::sqrt(1.), 0.), 0.});
+      freeJumps.back().push_back({{ax}, j});
+      f.nJumps = 1;
+    }
+    avQbit(ax);
+    s.frees.push_back(f);
+  }
+
+  // PumpedTwoLevelAtomSch: dense 2x2 H/i = [[0, eta^*], [-eta, za]], za = (-gamma, deltaA)
+  // (PumpedTwoLevelAtom.cc:13-49)
+  void ptla(const KV& kv) {
+    int ax = (int)s.frees.size();
+    double gamma = num(kv, "gamma", 10), delta = num(kv, "deltaA", -10);
+    cd eta(num(kv, "eta", 0), num(kv, "etaI", 0));
+    cd za(-gamma, delta);
+    FreeInfo f; f.kind = "ptla"; f.dim = 2; f.avKind = "qbit"; f.hasHa = true;
+    f.highestFrequency = std::max(std::abs(cd(gamma, delta)), std::abs(eta));
+    LocalOp h; h.dims = {2};
+    h.entries.push_back({0, 1, std::conj(eta), 0.});
+    h.entries.push_back({1, 0, -eta, 0.});
+    h.entries.push_back({1, 1, za, 0.});
+    s.hTerms.push_back({{ax}, h});
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    LocalOp j; j.dims = {2};
+    j.entries.push_back({0, 1, cd(std::sqrt(-2. * za.real()), 0.), 0.});
+    freeJumps.back().push_back({{ax}, j});
+    f.nJumps = 1;
+    avQbit(ax);
+    s.frees.push_back(f);
+  }
+
+  const FreeInfo& freeAt(int i) const {
+    if (i < 0 || i >= (int)s.frees.size()) throw std::invalid_argument("interaction refers to an undefined free");
+    return s.frees[i];
+  }
+
+  // JaynesCummings: conj(g) sigma (x) a^dagger - h.c.   (JaynesCummings_.h:54-66)
+  void jc(int q, int m, const KV& kv) {
+    const FreeInfo& fq = freeAt(q); const FreeInfo& fm = freeAt(m);
+    cd g(num(kv, "g", 0), num(kv, "gI", 0));
+    if (nonzero(g)) {
+      LocalOp t = kron(scale(destroy(fq.dim), std::conj(g)), adjoint(destroy(fm.dim)));
+      s.hTerms.push_back({{q, m}, plus(t, scale(adjoint(t), -1.))});
+    } else {
+      LocalOp t; t.dims = {fq.dim, fm.dim};
+      s.hTerms.push_back({{q, m}, t});
+    }
+    iaHF = std::max(iaHF, std::abs(g * std::sqrt((double)fq.dim * fm.dim)));
+  }
+
+  // CoupledModes<CM_XX_RWA>: conj(u) a_1 a_2^dagger - h.c.   (CoupledModes.cc:36-41)
+  void bs(int m1, int m2, const KV& kv) {
+    const FreeInfo& f1 = freeAt(m1); const FreeInfo& f2 = freeAt(m2);
+    cd u(num(kv, "g", 0), num(kv, "gI", 0));
+    LocalOp t = kron(scale(destroy(f1.dim), std::conj(u)), adjoint(destroy(f2.dim)));
+    s.hTerms.push_back({{m1, m2}, plus(t, scale(adjoint(t), -1.))});
+    iaHF = std::max(iaHF, std::abs(u * (f1.dim * std::sqrt((double)f2.dim))));
+  }
+
+  // J sigma_z (x) sigma_z with sigma_z = diag(-1, +1) (|1> excited); H/i = -i J sz sz.
+  // The reference declares but never defines qbit::sigmazop (Qbit_.h:68-70): this element is new.
+  void zz(int a, int b, const KV& kv) {
+    const FreeInfo& fa = freeAt(a); const FreeInfo& fb = freeAt(b);
+    if (fa.dim != 2 || fb.dim != 2) throw std::invalid_argument("zz couples two qubits");
+    double J = num(kv, "J", 0);
+    LocalOp sz = diagonal(2, {-1., 1.});
+    s.hTerms.push_back({{a, b}, scale(kron(sz, sz), cd(0., -J))});
+    iaHF = std::max(iaHF, std::fabs(J));
+  }
+
+  double iaHF = 0.;
+};
+
+System* buildFromSpec(const std::string& spec) {
+  std::unique_ptr<System> sp(new System);
+  System& s = *sp;
+  Builder b(s);
+  std::vector<std::string> lines;
+  {
+    std::string cur;
+    for (char c : spec) { if (c == ';' || c == '\n') { lines.push_back(cur); cur.clear(); } else cur += c; }
+    lines.push_back(cur);
+  }
+  int forced = -1;
+  int nIa = 0;
+  for (auto& ln : lines) {
+    std::istringstream is(ln);
+    std::string what, kind;
+    if (!(is >> what)) continue;
+    if (!(is >> kind)) throw std::invalid_argument("incomplete statement: " + ln);
+    std::vector<int> axes;
+    KV kv;
+    std::string tok;
+    while (is >> tok) {
+      auto p = tok.find('=');
+      if (p == std::string::npos) axes.push_back(std::stoi(tok));
+      else kv[tok.substr(0, p)] = tok.substr(p + 1);
+    }
+    if (what == "free") {
+      if (!axes.empty()) throw std::invalid_argument("free element takes no axes");
+      if (kind == "mode") b.mode(kv);
+      else if (kind == "qbit") b.qbit(kv);
+      else if (kind == "ptla") b.ptla(kv);
+      else throw std::invalid_argument("unknown free element " + kind);
+    } else if (what == "ia") {
+      if (axes.size() != 2) throw std::invalid_argument("interaction needs two axes");
+      if (axes[0] == axes[1]) throw std::invalid_argument("interaction axes must differ");
+      if (kind == "jc") b.jc(axes[0], axes[1], kv);
+      else if (kind == "bs") b.bs(axes[0], axes[1], kv);
+      else if (kind == "zz") b.zz(axes[0], axes[1], kv);
+      else throw std::invalid_argument("unknown interaction " + kind);
+      ++nIa;
+    } else if (what == "system") {
+      if (kind == "binary") forced = 1; else if (kind == "composite") forced = 0;
+      else throw std::invalid_argument("unknown system kind " + kind);
+    } else throw std::invalid_argument("unknown statement " + what);
+  }
+  if (s.frees.empty()) throw std::invalid_argument("system without free elements");
+  s.isBinary = forced >= 0 ? forced == 1 : (s.frees.size() == 2 && nIa <= 1);
+  for (auto& f : s.frees) s.dims.push_back(f.dim);
+  s.strides.assign(s.dims.size(), 1);
+  s.dim = 1;
+  for (int i = (int)s.dims.size() - 1; i >= 0; --i) { s.strides[i] = s.dim; s.dim *= s.dims[i]; }
+  // jump channels: frees first (BinarySystem.cc:217-239)
+  for (auto& js : b.freeJumps) for (auto& t : js) s.jumps.push_back({t});
+  for (size_t i = 0; i < b.freeObs.size(); ++i) {
+    int base = (int)s.obs.size();
+    for (size_t k = 0; k < b.freeObs[i].size(); ++k) {
+      s.obs.push_back(b.freeObs[i][k]);
+      s.processVarOf.push_back(b.freeVar[i][k] < 0 ? -1 : base + b.freeVar[i][k]);
+    }
+  }
+  // highest frequency (DynamicsBase.cc:49-66); binary systems include the interaction's (Q11)
+  for (auto& f : s.frees) s.highestFrequency = std::max(s.highestFrequency, f.highestFrequency);
+  if (s.isBinary) s.highestFrequency = std::max(s.highestFrequency, b.iaHF);
+  for (auto& f : s.frees) s.hasEx = s.hasEx || f.hasEx;
+  if (s.hasEx) {
+    s.exactZ.assign(s.dim, 0.);
+    for (long r = 0; r < s.dim; ++r)
+      for (size_t a = 0; a < s.frees.size(); ++a)
+        if (s.frees[a].hasEx) s.exactZ[r] += s.frees[a].zI * (double)((r / s.strides[a]) % s.dims[a]);
+  }
+  s.lower();
+  return sp.release();
+}
+
+// ---------------------------------------------------------------------------
+// embedding: local operators on axes V -> row-gather ELL over the full space
+// ---------------------------------------------------------------------------
+
+Ell embed(const System& s, const std::vector<Term>& terms) {
+  struct E { int col; cd val; cd freq; };
+  std::vector<std::vector<E>> rows(s.dim);
+  for (auto& t : terms) {
+    const LocalOp& op = t.op;
+    int r = (int)t.axes.size();
+    if ((int)op.dims.size() != r) throw std::invalid_argument("operator rank does not match its axes");
+    for (int i = 0; i < r; ++i)
+      if (t.axes[i] < 0 || t.axes[i] >= (int)s.dims.size() || op.dims[i] != s.dims[t.axes[i]])
+        throw std::invalid_argument("operator axes do not match the system dimensions");
+    std::vector<int> lstr(r, 1);
+    for (int i = r - 2; i >= 0; --i) lstr[i] = lstr[i + 1] * op.dims[i + 1];
+    // group entries by local row
+    std::vector<std::vector<const LocalOp::Entry*>> byRow(op.size());
+    for (auto& e : op.entries) byRow[e.row].push_back(&e);
+    for (long row = 0; row < s.dim; ++row) {
+      int lr = 0;
+      for (int i = 0; i < r; ++i) lr += (int)((row / s.strides[t.axes[i]]) % s.dims[t.axes[i]]) * lstr[i];
+      for (const LocalOp::Entry* e : byRow[lr]) {
+        long col = row;
+        cd freq = 0.;
+        for (int i = 0; i < r; ++i) {
+          int ax = t.axes[i];
+          int rc = (e->row / lstr[i]) % op.dims[i], cc = (e->col / lstr[i]) % op.dims[i];
+          col += (long)(cc - rc) * s.strides[ax];
+          // interaction-picture phase of |r><c| under U = exp(-zI n t): exp(zI (r - c) (t - t0))
+          if (s.frees[ax].hasEx) freq += s.frees[ax].zI * (double)(rc - cc);
+        }
+        auto& rv = rows[row];
+        auto it = std::find_if(rv.begin(), rv.end(), [&](const E& x) { return x.col == (int)col && x.freq == freq; });
+        if (it == rv.end()) rv.push_back({(int)col, e->val, freq});
+        else it->val += e->val;
+      }
+    }
+  }
+  Ell ell;
+  size_t w = 1;
+  for (auto& rv : rows) w = std::max(w, rv.size());
+  ell.width = (int)w;
+  ell.cols.assign(s.dim * w, 0);
+  ell.vals.assign(s.dim * w, 0.);
+  bool td = false;
+  for (auto& rv : rows) for (auto& e : rv) if (e.freq != cd(0.)) td = true;
+  if (td) ell.freqs.assign(s.dim * w, 0.);
+  for (long row = 0; row < s.dim; ++row) {
+    auto& rv = rows[row];
+    // keep the diagonal first (stable otherwise): a fixed order across rows helps the gather
+    std::stable_sort(rv.begin(), rv.end(), [&](const E& a, const E& b) { return (a.col == row) > (b.col == row); });
+    for (size_t k = 0; k < w; ++k) {
+      if (k < rv.size()) {
+        ell.cols[row * w + k] = rv[k].col; ell.vals[row * w + k] = rv[k].val;
+        if (td) ell.freqs[row * w + k] = rv[k].freq;
+      } else {
+        ell.cols[row * w + k] = (int)row;  // padding: zero weight on the row itself
+      }
+    }
+  }
+  return ell;
+}
+
+void System::lower() {
+  H = embed(*this, hTerms);
+  J.clear();
+  for (auto& j : jumps) J.push_back(embed(*this, j));
+  O.clear();
+  for (auto& o : obs) O.push_back(embed(*this, o.terms));
+}
+
+}  // namespace qt

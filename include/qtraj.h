@@ -1,0 +1,135 @@
+/* qtraj.h -- C ABI of the MI355X quantum-trajectory hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status (0 = ok),
+ * thread-local error text from qt_last_error().  Complex arrays are
+ * interleaved (re, im) doubles, row-major over the subsystem dimensions
+ * (the reference's Blitz++ StateVectorLow / DensityOperatorLow layout).
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the vukics/cppqed repository).
+ */
+#ifndef QTRAJ_H
+#define QTRAJ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qt_sys qt_sys;          /* a lowered quantum system (host tables + device copy) */
+typedef struct qt_ens qt_ens;          /* a device-resident MCWF ensemble */
+typedef struct qt_master qt_master;    /* a device-resident batch of density operators */
+typedef void* qt_stream;               /* hipStream_t; NULL = the default stream */
+
+enum { QT_OK = 0, QT_EINVAL = 1, QT_EDEVICE = 2, QT_ENOMEM = 3, QT_ENUMERIC = 4, QT_ENOTSUP = 5 };
+enum { QT_FP64 = 0, QT_FP32_DERIV = 1 };
+enum { QT_CONTROLLER_ODEINT = 0, QT_CONTROLLER_GSL = 1 };
+
+const char* qt_last_error(void);
+
+/* ---------------------------------------------------------------- systems */
+
+/* Builds a system from an element spec (one statement per line or ';'):
+ *   free mode cutoff=N delta=D kappa=K eta=E [etaI=..] [picture=Sch|IP|UIP]
+ *   free qbit deltaA=D gamma=G eta=E [etaI=..] [picture=..]
+ *   free ptla gamma=G deltaA=D eta=E
+ *   ia jc i j g=G | ia bs i j g=U | ia zz i j J=J
+ *   system binary|composite
+ * Replaces the element makers and composition of the reference:
+ *   mode::make  CPPQEDelements/frees/Mode.tcc:40-60,   qbit::make  Qbit.cc:217-246,
+ *   PumpedTwoLevelAtomSch  PumpedTwoLevelAtom.cc:35-49, jaynescummings::make  JaynesCummings_.h:85-90,
+ *   binary::make  CPPQEDcore/composites/BinarySystem.cc:323-333, composite::make  Composite.h:363-367.
+ * The device copy of the tables is made on first use. */
+int qt_sys_build(const char* spec, qt_sys** out);
+int qt_sys_destroy(qt_sys* sys);
+
+typedef struct {
+  long dim;                 /* total Hilbert-space dimension */
+  int rank;
+  int dims[16];
+  int n_jumps;              /* Liouvillean::nAvr<LA_Li>  (Liouvillean.h:54) */
+  int n_avr;                /* Averaged::nAvr            (streamed columns after time, dtDid) */
+  int h_width, j_width, o_width;
+  int time_dependent;       /* 1 in interaction picture */
+  double highest_frequency; /* QuantumSystem::highestFrequency, QuantumSystem.h:21-41 */
+  long long h_nnz;          /* stored nonzeros of H_nH/i (merged) */
+} qt_sys_info_t;
+int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info);
+
+/* Host copies of the lowered tables (testing aid): *width receives the row width;
+ * cols[dim*width] row-major, vals/freqs[2*dim*width] (any may be NULL).
+ * which: 0 = H_nH/i, 1+m = J_m, 1+n_jumps+q = observable q. */
+int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* vals, double* freqs);
+
+/* dpsidt += (H_nH/i) psi for n state vectors, device pointers, on `stream`.
+ * Replaces structure::Hamiltonian<RANK>::addContribution(t, psi, dpsidt, t0)
+ * (CPPQEDcore/structure/Hamiltonian.h:38-43) as dispatched by
+ * binary::Hamiltonian::addContribution_v (BinarySystem.cc:194-207) /
+ * composite (Composite.tcc:178-197).  Schroedinger-picture systems only. */
+int qt_apply_heff(qt_sys* sys, double t, double t0, const double* psi_dev, double* dpsidt_dev, int n,
+                  int precision, qt_stream stream);
+
+/* ------------------------------------------------------------ MCWF ensemble */
+
+typedef struct {
+  double epsRel, epsAbs;     /* user values: the error checker gets (abs=epsRel, rel=epsAbs), ODE.h:205 */
+  double dpLimit, overshootTol;   /* mcwf::Pars, MCWF_Trajectory.h:28-47 */
+  double dtInit;             /* <= 0: 1/(10*highest_frequency), Trajectory.h:126 */
+  int precision;             /* QT_FP64 | QT_FP32_DERIV */
+  int controller;            /* QT_CONTROLLER_ODEINT (only) */
+  unsigned long long seed;   /* counter-RNG key word 0 */
+  unsigned long long traj0;  /* RNG key word 1 of trajectory 0 (trajectory k uses traj0+k) */
+  int jump_log_cap;          /* jump records kept per trajectory */
+  int layout_rows, layout_waves;  /* 0 = pick automatically (autotuned table) */
+} qt_mcwf_params;
+
+/* nTraj copies of psi0 (host) on one system: mcwf::makeEnsemble, MCWF_Trajectory.h:213-234,
+ * including the constructor's initial dpLimit check (:114-117). */
+int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const double* psi0_host, qt_ens** out);
+int qt_mcwf_destroy(qt_ens* ens);
+
+/* Advances every trajectory through n_intervals stream intervals of the given lengths
+ * (cppqedutils::advance, Trajectory.h:43-50, for each trajectory: Ensemble::advance,
+ * StochasticTrajectory.h:220) in one persistent launch.  At each interval end the
+ * ensemble-averaged streamed row is written to rows_host[k*(2+n_avr)]: time, mean dtDid,
+ * processed averages (AverageTrajectories + DensityOperatorStreamer,
+ * MCWF_Trajectory.h:406-420).  rows_host may be NULL. */
+int qt_mcwf_advance(qt_ens* ens, const double* intervals, int n_intervals, double* rows_host, qt_stream stream);
+
+/* Asynchronous form for serving/graph replay: no host synchronisation, no rows. */
+int qt_mcwf_advance_async(qt_ens* ens, const double* intervals, int n_intervals, qt_stream stream);
+
+/* Returns every trajectory to its state right after qt_mcwf_create (t = 0, psi0, initial dtTry,
+ * RNG ordinal 0, counters 0), device-side, asynchronously on `stream`. */
+int qt_mcwf_reset(qt_ens* ens, qt_stream stream);
+
+/* Per-trajectory raw (unprocessed) averages at the last streamed point: out[n_traj*n_avr]. */
+int qt_mcwf_traj_averages(qt_ens* ens, double* out);
+/* State of trajectory i (host copy) and its time. */
+int qt_mcwf_state(qt_ens* ens, int i, double* psi_host, double* t);
+/* Jump record of trajectory i (MCWF_TrajectoryLogger jump list). */
+int qt_mcwf_jumplog(qt_ens* ens, int i, double* t, int* ch, int cap, int* n);
+/* Per-trajectory counters [n_traj][8]: MCWF steps, ODE steps, failed tries, derivative
+ * calls, RKCK tries, step-backs, dpLimit overshoots, jumps (MCWF_TrajectoryLogger.h:59-101,
+ * ODE.h:99-122).  totals[8] (may be NULL) receives their sums. */
+int qt_mcwf_counters(qt_ens* ens, long long* per_traj, long long* totals);
+/* Average duration (ms) of the batched kernel launches so far, from HIP events on the launch stream. */
+int qt_mcwf_kernel_ms(qt_ens* ens, double* total_ms, int* launches);
+
+/* -------------------------------------------------------------- plain form */
+
+/* The unbatched GPU form: one trajectory at a time, the MCWF/ODE control on the host
+ * (as MCWF_Trajectory::step, MCWF_Trajectory.h:347-374), one device launch per RKCK
+ * try and per post-step phase, FP64.  Advances trajectory psi through the intervals,
+ * writing the raw averages after each interval into obs_host[k*n_avr] and the jump
+ * record into jt/jch (cap jcap).  counters[8] as qt_mcwf_counters. */
+int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long traj, const double* psi0_host,
+                      const double* intervals, int n_intervals, double* obs_host, double* psi_out,
+                      double* jt, int* jch, int jcap, long long* counters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QTRAJ_H */

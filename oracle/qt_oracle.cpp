@@ -423,8 +423,10 @@ Free makeMode(const std::map<std::string, std::string>& kv) {
   cd zSch(kappa, -delta);
   if (p == Picture::Sch) {
     modeHamiltonian(f, false, zSch, 0., eta, cutoff);
-    // frequencies: PumpedLossyModeSch {TUPLE_kappadelta(0),TUPLE_eta} (Mode.tcc:90-93)
-    if (kappa) f.complexFreqs.push_back({std::conj(zSch), (double)cutoff});
+    // frequencies (Mode.tcc:90-93, 127, 160, 193): PumpedLossyModeSch {kappadelta,eta},
+    // LossyModeSch {kappa,delta}, PumpedModeSch {delta,eta}, ModeSch {delta}
+    if (kappa && isNonZero(eta)) f.complexFreqs.push_back({std::conj(zSch), (double)cutoff});
+    else if (kappa) { f.realFreqs.push_back({kappa, (double)cutoff}); f.realFreqs.push_back({delta, (double)cutoff}); }
     else f.realFreqs.push_back({delta, (double)cutoff});
     if (isNonZero(eta)) f.complexFreqs.push_back({eta, std::sqrt((double)cutoff)});
   } else {

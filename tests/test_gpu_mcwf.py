@@ -1,0 +1,124 @@
+"""GPU parity of the MCWF hot path against the CPU oracle (same counter-based draws)."""
+import numpy as np
+import pytest
+
+import cppqed_amd as Q
+import oracle_lib as O
+from sysdefs import JC60, PLM20, PTLA, coherent, ground
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+@pytest.mark.parametrize("spec", [JC60, PLM20, PTLA])
+@pytest.mark.parametrize("prec,tol", [(Q.QT_FP64, 1e-14), (Q.QT_FP32_DERIV, 2e-6)])
+def test_apply_heff_matches_oracle(spec, prec, tol):
+    sq, so = Q.System(spec), O.System(spec)
+    rng = np.random.default_rng(7)
+    n = 33
+    psi = rng.normal(size=(n, sq.dim)) + 1j * rng.normal(size=(n, sq.dim))
+    base = rng.normal(size=(n, sq.dim)) + 1j * rng.normal(size=(n, sq.dim))
+    dpsi = _dev(base.copy())
+    x = _dev(psi)
+    sq.apply_heff(x.data_ptr(), dpsi.data_ptr(), n=n, precision=prec)
+    torch.cuda.synchronize()
+    got = dpsi.cpu().numpy()
+    for i in range(n):
+        ref = so.apply_heff(psi[i], dpsidt=base[i])
+        scale = np.abs(ref).max()
+        assert np.abs(got[i] - ref).max() <= tol * scale, (i, np.abs(got[i] - ref).max(), scale)
+
+
+def _oracle_traj(so, psi0, k, seed, intervals, **kw):
+    p = O.params(rng="philox", seed=seed, stream=k, **kw)
+    r = O.run_spec("NDt", Dt=intervals[0], NDt=len(intervals))
+    return O.mcwf_run(so, psi0, p, r)
+
+
+def _assert_traj_parity(res_o, psi_g, jt_g, jch_g, nj_g, cnt_g=None):
+    assert nj_g == len(res_o["jump_t"]), (nj_g, len(res_o["jump_t"]))
+    np.testing.assert_array_equal(jch_g, res_o["jump_ch"])
+    np.testing.assert_allclose(jt_g, res_o["jump_t"], rtol=1e-10, atol=1e-12)
+    err = np.linalg.norm(psi_g - res_o["psi"]) / np.linalg.norm(res_o["psi"])
+    assert err <= 1e-9, err
+    if cnt_g is not None:
+        # MCWF steps, ODE steps, failed tries, tries, step-backs, jumps
+        for c in (0, 1, 2, 4, 5, 7):
+            assert cnt_g[c] == res_o["counters"][c], (c, cnt_g[c], res_o["counters"][c])
+
+
+@pytest.mark.parametrize("name,spec,psi0,T,ntraj", [
+    ("jc60", JC60, ground(60), 3.0, 48),
+    ("plm20", PLM20, coherent(0.5, 20), 2.0, 32),
+    ("ptla", PTLA, ground(2), 1.0, 32),
+])
+def test_batched_fp64_trajectories_match_oracle(name, spec, psi0, T, ntraj):
+    print(f"[gpu] batched fp64 parity {name} ntraj={ntraj} T={T}", flush=True)
+    sq, so = Q.System(spec), O.System(spec)
+    seed = 20240601
+    Dt = 0.1 if name != "ptla" else 0.05
+    iv = np.full(int(round(T / Dt)), Dt)
+    ens = Q.Ensemble(sq, ntraj, psi0, Q.mcwf_pars(seed=seed, traj0=0, jump_log_cap=256))
+    rows = ens.advance(iv)
+    per, tot = ens.counters()
+    njumps = 0
+    for k in range(ntraj):
+        res_o = _oracle_traj(so, psi0, k, seed, iv)
+        psi_g, t_g = ens.state(k)
+        jt, jch, nj = ens.jumplog(k)
+        _assert_traj_parity(res_o, psi_g, jt, jch, nj, per[k])
+        njumps += nj
+    assert tot[7] == njumps
+    # ensemble rows against the oracle's ensemble
+    rows_o, tries_o = O.ensemble_run(so, psi0, O.params(rng="philox", seed=seed, stream=0),
+                                     O.run_spec("NDt", Dt=Dt, NDt=len(iv)), ntraj)
+    np.testing.assert_allclose(rows[:, 0], rows_o[1:, 0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rows[:, 2:], rows_o[1:, 2:], rtol=1e-9, atol=1e-9)
+    assert tot[4] == tries_o
+
+
+def test_plain_form_matches_oracle():
+    print("[gpu] plain form parity jc60", flush=True)
+    sq, so = Q.System(JC60), O.System(JC60)
+    seed = 77
+    iv = np.full(10, 0.1)
+    for k in range(3):
+        res_g = Q.mcwf_plain_run(sq, ground(60), iv, Q.mcwf_pars(seed=seed), traj=k)
+        res_o = _oracle_traj(so, ground(60), k, seed, iv)
+        _assert_traj_parity(res_o, res_g["psi"], res_g["jump_t"], res_g["jump_ch"], int(res_g["counters"][7]),
+                            res_g["counters"])
+
+
+def test_fp32_derivative_observable_error():
+    """FP32 derivative evaluations under FP64 control: ensemble <a^dagger a> error vs the FP64 run."""
+    print("[gpu] fp32-derivative observable error jc60", flush=True)
+    sq = Q.System(JC60)
+    iv = np.full(20, 0.1)
+    n = 1024
+    r64 = Q.Ensemble(sq, n, ground(60), Q.mcwf_pars(seed=5)).advance(iv)
+    r32 = Q.Ensemble(sq, n, ground(60), Q.mcwf_pars(seed=5, precision=Q.QT_FP32_DERIV)).advance(iv)
+    # column 6 = <n> of the mode (after time, dtDid, 4 qubit columns)
+    err = np.abs(r32[:, 6] - r64[:, 6]).max()
+    assert err < 1e-3, err
+
+
+def test_layouts_agree():
+    """Every compiled layout that covers the dimension gives the same trajectories."""
+    print("[gpu] layout agreement jc60", flush=True)
+    sq = Q.System(JC60)
+    iv = np.full(5, 0.1)
+    ref = None
+    for lay in [(1, 1), (2, 1), (4, 1), (1, 4)]:
+        ens = Q.Ensemble(sq, 16, ground(60), Q.mcwf_pars(seed=9, layout=lay))
+        rows = ens.advance(iv)
+        states = np.array([ens.state(k)[0] for k in range(16)])
+        if ref is None:
+            ref = (rows, states)
+        else:
+            np.testing.assert_allclose(rows[:, 2:], ref[0][:, 2:], rtol=1e-11, atol=1e-12)
+            assert np.abs(states - ref[1]).max() < 1e-10
