@@ -34,7 +34,8 @@ class McwfParams(C.Structure):
     _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dpLimit", C.c_double),
                 ("overshootTol", C.c_double), ("dtInit", C.c_double), ("precision", C.c_int),
                 ("controller", C.c_int), ("seed", C.c_ulonglong), ("traj0", C.c_ulonglong),
-                ("jump_log_cap", C.c_int), ("layout_rows", C.c_int), ("layout_waves", C.c_int)]
+                ("jump_log_cap", C.c_int), ("layout_rows", C.c_int), ("layout_waves", C.c_int),
+                ("layout_occupancy", C.c_int), ("layout_group_lanes", C.c_int)]
 
 
 def lib():
@@ -235,10 +236,10 @@ def run_intervals(T=None, Dt=0.1, NDt=0):
 # ---------------------------------------------------------------- MCWF ensemble
 
 def mcwf_pars(epsRel=1e-6, epsAbs=1e-12, dpLimit=0.01, overshootTol=10.0, dtInit=0.0, precision=QT_FP64,
-              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0)):
+              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0), occupancy=0, group_lanes=0):
     """mcwf::Pars + ode_engine::Pars with the reference defaults (MCWF_Trajectory.h:28-47, ODE.h:31-63)."""
     return McwfParams(epsRel, epsAbs, dpLimit, overshootTol, dtInit, precision, 0, seed, traj0,
-                      jump_log_cap, layout[0], layout[1])
+                      jump_log_cap, layout[0], layout[1], occupancy, group_lanes)
 
 
 class Ensemble:

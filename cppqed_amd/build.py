@@ -17,7 +17,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC"]
 
-SOURCES = [("qt_system.cpp", "cxx"), ("qt_capi.cpp", "hip"), ("qt_mcwf.hip", "hip"), ("qt_master.hip", "hip")]
+SOURCES = [("qt_system.cpp", "cxx"), ("qt_capi.cpp", "hip"), ("qt_mcwf.hip", "hip"), ("qt_subgroup.hip", "hip"),
+           ("qt_master.hip", "hip")]
 
 
 def _run(cmd):

@@ -18,7 +18,15 @@
 
 namespace qt {
 bool layout_supported(int rows, int nw, int wh);
-hipError_t launch_batch(int rows, int nw, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st);
+bool batch_supported(int rows, int nw, int wh, int minw);
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32);
+bool sub_supported(int g, int rows, int wh);
+int sub_blocks_per_cu(int g, int rows, int wh, bool f32);
+hipError_t launch_sub(int g, int rows, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st);
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E,
+                        hipStream_t st);
+hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
+                             int n_traj, int cap, double* draws, hipStream_t st);
 hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
                                 double* out_dt, hipStream_t st);
 hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st);
@@ -81,7 +89,7 @@ struct DevBuf {
   }
 };
 
-struct LayoutChoice { int rows, nw, wh; };
+struct LayoutChoice { int rows, nw, wh, occ = 1, glanes = 0; };  // glanes > 0: sub-wave groups
 
 // Layout table (rows per lane x wavefronts per group) by dimension; the ELL width is
 // rounded up to a compiled width.
@@ -100,6 +108,33 @@ LayoutChoice pick_layout(long dim, int width, int rows_req, int nw_req) {
     for (int w : {wh, 8, 16}) if (w >= wh && qt::layout_supported(c[0], c[1], w)) return {c[0], c[1], w};
   }
   throw Fail(QT_ENOTSUP, "dimension too large for the compiled layouts");
+}
+
+// sub-wave group width by dimension (measured on MI355X)
+int default_group_lanes(long dim) { return dim <= 64 ? 16 : 64; }
+
+// occupancy hint per layout (measured on MI355X; 1 = no hint)
+int default_occupancy(const LayoutChoice& L) {
+  // one trajectory per wavefront at dim <= 64: asking for 4 waves/SIMD schedules the
+  // kernel for latency hiding (JC-60: 453 vs 413 M tries/s FP32-deriv, 398 vs 387 FP64)
+  if (L.rows == 1 && L.nw == 1 && L.wh == 4) return 4;
+  return 1;
+}
+
+// sub-wave group layout: G lanes per trajectory, rows per lane covering the dimension
+LayoutChoice pick_sub(long dim, int width, int g_req, int rows_req) {
+  int wh = width <= 4 ? 4 : width <= 8 ? 8 : width <= 16 ? 16 : -1;
+  if (wh < 0) throw Fail(QT_ENOTSUP, "H_nH/i rows wider than 16 entries are not supported");
+  for (int g : {16, 32, 64}) {
+    if (g_req > 0 && g != g_req) continue;
+    for (int r : {1, 2, 4}) {
+      if (rows_req > 0 && r != rows_req) continue;
+      if ((long)r * g < dim) continue;
+      for (int w : {wh, 8, 16})
+        if (w >= wh && qt::sub_supported(g, r, w)) return LayoutChoice{r, 1, w, 1, g};
+    }
+  }
+  throw Fail(QT_ENOTSUP, "no compiled sub-wave layout covers this system");
 }
 
 void check_params(const qt_mcwf_params* p) {
@@ -187,7 +222,7 @@ struct qt_ens {
   LayoutChoice lay{};
   double time = 0.;           // common time of all trajectories (they all stop at each endTime)
   DevBuf<double2> psi, psiInit;
-  DevBuf<double> t, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt;
+  DevBuf<double> t, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt, draws;
   DevBuf<unsigned long long> ord;
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
@@ -196,6 +231,7 @@ struct qt_ens {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double kernelMs = 0.;
   int launches = 0;
+  double dtMinSeen = 0.02;  // typical accepted step, sizes the jump-draw budget
   ~qt_ens() {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -229,14 +265,13 @@ std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& p
   return r;
 }
 
-int resident_groups(const LayoutChoice& L) {
+int resident_groups(const LayoutChoice& L, bool f32) {
   int dev = 0;
   hipCheck(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
   hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  // up to 8 single-wave groups per SIMD-quad; registers bound the real residency,
-  // the work queue absorbs any excess
-  int per_cu = std::max(1, 32 / L.nw);
+  int per_cu = L.glanes > 0 ? qt::sub_blocks_per_cu(L.glanes, L.rows, L.wh, f32) : qt::batch_blocks_per_cu(L.rows, L.nw, L.wh, L.occ, f32);
+  if (per_cu < 1) per_cu = 1;
   return prop.multiProcessorCount * per_cu;
 }
 
@@ -329,7 +364,16 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     std::unique_ptr<qt_ens> e(new qt_ens);
     e->sys = sys; e->n = n_traj; e->p = *p;
     if (e->p.jump_log_cap < 1) e->p.jump_log_cap = 1;
-    e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
+    const bool useSub = p->layout_group_lanes > 0;
+    if (useSub) {
+      e->lay = pick_sub(s.dim, s.H.width, p->layout_group_lanes > 0 ? p->layout_group_lanes : default_group_lanes(s.dim),
+                        p->layout_rows);
+    } else {
+      e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
+      e->lay.occ = p->layout_occupancy > 0 ? p->layout_occupancy : default_occupancy(e->lay);
+      if (!qt::batch_supported(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ))
+        throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
+    }
     const long d = s.dim;
     std::vector<qt::cd> psi(d);
     for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
@@ -357,7 +401,8 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->jch.alloc((size_t)n_traj * e->p.jump_log_cap);
     e->work.alloc(1);
     e->errflag.upload(std::vector<int>(1, 0));
-    e->grid = std::min(n_traj, resident_groups(e->lay));
+    const int perGroup = e->lay.glanes > 0 ? 64 / e->lay.glanes : 1;  // trajectories per work group
+    e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV));
     hipCheck(hipEventCreate(&e->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&e->e1), "hipEventCreate");
     *out = e.release();
@@ -379,8 +424,22 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
   qt::EnsDev E = e->view();
   E.intervals = e->intervals.p; E.n_intervals = n_int;
   E.obs_out = e->obs.p; E.dtdid_out = e->dtd.p;
+  // jump draws for the launch, generated in bulk (one Philox per lane); trajectories that take
+  // more steps than the budget evaluate Philox in the kernel
+  if (!e->sys->h->J.empty() && e->lay.glanes == 0) {
+    double span = 0.; for (int k = 0; k < n_int; ++k) span += intervals[k];
+    const int cap = (int)std::min(2048.0, std::max(64.0, std::ceil(span / std::max(e->dtMinSeen, 1e-6) * 1.25) + 8));
+    e->draws.alloc((size_t)e->n * cap);
+    hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st), "jump draws");
+    E.draws = e->draws.p; E.draw_cap = cap;
+  }
   if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
-  hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
+  if (e->lay.glanes > 0)
+    hipCheck(qt::launch_sub(e->lay.glanes, e->lay.rows, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
+             "mcwf sub-wave launch");
+  else
+  hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV, e->grid,
+                            e->sys->dev, E, st),
            "mcwf batch launch");
   if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
   e->lastIntervals = n_int;

@@ -1,0 +1,128 @@
+// qt_device.hpp -- device helpers shared by the MCWF kernels (complex FMA, DPP lane
+// reductions, the Cash-Karp tableau, the step-size controller's power function).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace qt {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+template <bool F32> struct CT;
+template <> struct CT<false> { using T = double2; using R = double; };
+template <> struct CT<true> { using T = float2; using R = float; };
+
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+  c.x = fma(a.x, b.x, c.x); c.x = fma(-a.y, b.y, c.x);
+  c.y = fma(a.x, b.y, c.y); c.y = fma(a.y, b.x, c.y);
+  return c;
+}
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
+  c.x = fmaf(a.x, b.x, c.x); c.x = fmaf(-a.y, b.y, c.x);
+  c.y = fmaf(a.x, b.y, c.y); c.y = fmaf(a.y, b.x, c.y);
+  return c;
+}
+__device__ __forceinline__ double2 to_d(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <typename T> __device__ __forceinline__ T from_d(double2 a);
+template <> __device__ __forceinline__ double2 from_d<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_d<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ double2 axpy(double s, double2 k, double2 acc) {  // acc + s*k
+  acc.x = fma(s, k.x, acc.x); acc.y = fma(s, k.y, acc.y); return acc;
+}
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// Cross-lane moves through DPP (VALU latency, no LDS round trip).  Controls (GFX9 encoding):
+// quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum / max over the 64 lanes of a wavefront; every lane gets the bitwise-identical result
+// (each butterfly step adds two equal-order partial sums, addition being commutative).
+// All 64 lanes must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp<0xB1>(v); v += dpp<0x4E>(v); v += dpp<0x141>(v); v += dpp<0x140>(v);
+  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+__device__ __forceinline__ double wave_max(double v) {
+  v = fmax(v, dpp<0xB1>(v)); v = fmax(v, dpp<0x4E>(v)); v = fmax(v, dpp<0x141>(v)); v = fmax(v, dpp<0x140>(v));
+  return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+}
+
+template <int NW> struct GroupReduce {
+  // sum / max over all lanes of the group; every lane gets the result.
+  __device__ static double sum(double v, double* red) {
+    v = wave_sum(v);
+    if constexpr (NW == 1) { (void)red; return v; }
+    else {
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      __syncthreads();
+      if (lane == 0) red[w] = v;
+      __syncthreads();
+      double s = 0.;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) s += red[i];
+      return s;
+    }
+  }
+  __device__ static double max(double v, double* red) {
+    v = wave_max(v);
+    if constexpr (NW == 1) { (void)red; return v; }
+    else {
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      __syncthreads();
+      if (lane == 0) red[w] = v;
+      __syncthreads();
+      double s = red[0];
+#pragma unroll
+      for (int i = 1; i < NW; ++i) s = fmax(s, red[i]);
+      return s;
+    }
+  }
+};
+
+// x^y for the step-size controller, 0 < x, as exp(y log x): a few ulp from pow, which
+// moves dt by a few ulp (Boost's default_step_adjuster calls std::pow).
+__device__ __forceinline__ double ctrl_pow(double x, double y) { return exp(y * log(x)); }
+
+// x^(-1/N) for N = 3, 5 (the controller's only exponents): an FP32 estimate refined by two
+// Newton steps on r^N x = 1 in FP64 (quadratic convergence: 1e-7 -> 1e-14 -> rounding level),
+// a few ulp from std::pow and far cheaper than FP64 log/exp.  Valid for 1e-30 < x < 1e30,
+// which covers the controller's range (errors in [5^-5, 1e30]).
+template <int N>
+__device__ __forceinline__ double inv_root(double x) {
+  double r = (double)exp2f(-__log2f((float)x) * (1.0f / N));
+  #pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    double rn = r;
+    #pragma unroll
+    for (int k = 1; k < N; ++k) rn *= r;        // r^N
+    const double e = fma(-rn, x, 1.0);         // 1 - x r^N
+    r = fma(r * (1.0 / N), e, r);              // r (1 + e/N)
+  }
+  return r;
+}
+
+// Cash-Karp RK4(5) tableau (as in Boost.Odeint's runge_kutta_cash_karp54)
+struct CK {
+  static constexpr double c2 = 1. / 5, c3 = 3. / 10, c4 = 3. / 5, c5 = 1., c6 = 7. / 8;
+  static constexpr double a21 = 1. / 5;
+  static constexpr double a31 = 3. / 40, a32 = 9. / 40;
+  static constexpr double a41 = 3. / 10, a42 = -9. / 10, a43 = 6. / 5;
+  static constexpr double a51 = -11. / 54, a52 = 5. / 2, a53 = -70. / 27, a54 = 35. / 27;
+  static constexpr double a61 = 1631. / 55296, a62 = 175. / 512, a63 = 575. / 13824, a64 = 44275. / 110592, a65 = 253. / 4096;
+  static constexpr double b1 = 37. / 378, b3 = 250. / 621, b4 = 125. / 594, b6 = 512. / 1771;
+  static constexpr double e1 = 37. / 378 - 2825. / 27648, e3 = 250. / 621 - 18575. / 48384,
+                          e4 = 125. / 594 - 13525. / 55296, e5 = -277. / 14336, e6 = 512. / 1771 - 1. / 4;
+};
+
+}  // namespace qt

@@ -60,6 +60,8 @@ struct EnsDev {
   int* work;                    // work-queue head (trajectories are claimed in order)
   int status_bad;               // unused
   int* err_flag;                // set to 1 when a trajectory hits a non-finite jump norm
+  const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals ordinal[traj] + i (may be null)
+  int draw_cap;
 };
 
 // One RKCK try of one trajectory (the plain form): x -> xn, err

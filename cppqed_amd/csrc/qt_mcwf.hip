@@ -17,85 +17,13 @@
 // Work distribution: a global atomic counter hands trajectories to groups, so
 // uneven trajectory costs (jumps, step-backs, rejected tries) balance out.
 
+#include "qt_device.hpp"
 #include "qt_kernels.hpp"
 #include "qt_philox.h"
 
+#include <algorithm>
+
 namespace qt {
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-
-template <bool F32> struct CT;
-template <> struct CT<false> { using T = double2; using R = double; };
-template <> struct CT<true> { using T = float2; using R = float; };
-
-__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
-  c.x = fma(a.x, b.x, c.x); c.x = fma(-a.y, b.y, c.x);
-  c.y = fma(a.x, b.y, c.y); c.y = fma(a.y, b.x, c.y);
-  return c;
-}
-__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
-  c.x = fmaf(a.x, b.x, c.x); c.x = fmaf(-a.y, b.y, c.x);
-  c.y = fmaf(a.x, b.y, c.y); c.y = fmaf(a.y, b.x, c.y);
-  return c;
-}
-__device__ __forceinline__ double2 to_d(double2 a) { return a; }
-__device__ __forceinline__ double2 to_d(float2 a) { return make_double2((double)a.x, (double)a.y); }
-template <typename T> __device__ __forceinline__ T from_d(double2 a);
-template <> __device__ __forceinline__ double2 from_d<double2>(double2 a) { return a; }
-template <> __device__ __forceinline__ float2 from_d<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
-__device__ __forceinline__ double2 axpy(double s, double2 k, double2 acc) {  // acc + s*k
-  acc.x = fma(s, k.x, acc.x); acc.y = fma(s, k.y, acc.y); return acc;
-}
-__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
-
-template <int NW> struct GroupReduce {
-  // sum / max over all lanes of the group; every lane gets the result.
-  __device__ static double sum(double v, double* red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if constexpr (NW == 1) { (void)red; return v; }
-    else {
-      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-      __syncthreads();
-      if (lane == 0) red[w] = v;
-      __syncthreads();
-      double s = 0.;
-#pragma unroll
-      for (int i = 0; i < NW; ++i) s += red[i];
-      return s;
-    }
-  }
-  __device__ static double max(double v, double* red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if constexpr (NW == 1) { (void)red; return v; }
-    else {
-      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-      __syncthreads();
-      if (lane == 0) red[w] = v;
-      __syncthreads();
-      double s = red[0];
-#pragma unroll
-      for (int i = 1; i < NW; ++i) s = fmax(s, red[i]);
-      return s;
-    }
-  }
-};
-
-// Cash-Karp RK4(5) tableau (as in Boost.Odeint's runge_kutta_cash_karp54)
-struct CK {
-  static constexpr double c2 = 1. / 5, c3 = 3. / 10, c4 = 3. / 5, c5 = 1., c6 = 7. / 8;
-  static constexpr double a21 = 1. / 5;
-  static constexpr double a31 = 3. / 40, a32 = 9. / 40;
-  static constexpr double a41 = 3. / 10, a42 = -9. / 10, a43 = 6. / 5;
-  static constexpr double a51 = -11. / 54, a52 = 5. / 2, a53 = -70. / 27, a54 = 35. / 27;
-  static constexpr double a61 = 1631. / 55296, a62 = 175. / 512, a63 = 575. / 13824, a64 = 44275. / 110592, a65 = 253. / 4096;
-  static constexpr double b1 = 37. / 378, b3 = 250. / 621, b4 = 125. / 594, b6 = 512. / 1771;
-  static constexpr double e1 = 37. / 378 - 2825. / 27648, e3 = 250. / 621 - 18575. / 48384,
-                          e4 = 125. / 594 - 13525. / 55296, e5 = -277. / 14336, e6 = 512. / 1771 - 1. / 4;
-};
 
 // ---------------------------------------------------------------------------
 // the per-group trajectory engine
@@ -168,14 +96,14 @@ struct Engine {
   __device__ __forceinline__ void stage(const double2 v[ROWS]) const {
     __syncthreads();  // previous readers of xs are done
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (valid(r)) xs[r * G + lane] = from_d<T>(v[r]);
+    for (int r = 0; r < ROWS; ++r) xs[r * G + lane] = from_d<T>(v[r]);
     __syncthreads();
   }
 
   __device__ __forceinline__ void stagePsi(const double2 v[ROWS]) const {
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (valid(r)) ps[r * G + lane] = v[r];
+    for (int r = 0; r < ROWS; ++r) ps[r * G + lane] = v[r];
     __syncthreads();
   }
 
@@ -219,12 +147,18 @@ struct Engine {
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
         xe = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, xe))));
         if (valid(r)) {
-          const double ax = sqrt(cabs2(x[r])), ak = sqrt(cabs2(d1)), ae = sqrt(cabs2(xe));
-          e = fmax(e, ae / (epsAbs + epsRel * (ax + dt * ak)));
+          // odeint's rel_error: |xerr| / (epsAbs + epsRel (|x| + dt |dxdt|)); the maximum is taken
+          // over squared ratios and rooted once
+          const double ax = sqrt(cabs2(x[r])), ak = sqrt(cabs2(d1));
+          const double D = epsAbs + epsRel * (ax + dt * ak);
+          double rc = __builtin_amdgcn_rcp(D);
+          rc = fma(rc, fma(-D, rc, 1.0), rc);
+          rc = fma(rc, fma(-D, rc, 1.0), rc);
+          e = fmax(e, cabs2(xe) * (rc * rc));
         }
       }
     }
-    return GroupReduce<NW>::max(e, red);
+    return sqrt(GroupReduce<NW>::max(e, red));
   }
 
   __device__ __forceinline__ void derivative(const double2 x[ROWS], T k[ROWS]) const { stage(x); gather(k); }
@@ -268,11 +202,14 @@ struct Engine {
 // batched persistent MCWF kernel
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32>
-__global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E) {
+template <int ROWS, int NW, int WH, bool F32, int MINW>
+__global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
   using Eng = Engine<ROWS, NW, WH, F32>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
+  // kernel arguments live in LDS: read where used instead of pinning ~80 scalar registers
+  __shared__ SysDev S;
+  __shared__ EnsDev E;
   __shared__ T xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
   // psi at the start of the MCWF step (step-back restore): LDS when it fits, else registers
@@ -284,8 +221,10 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
   __shared__ long long cnt[kCounters];     // this trajectory's counters (lane 0 updates)
   __shared__ int shTraj;
 
-  Eng eng(S, xs, ps, red);
   const int lane = threadIdx.x;
+  if (lane == 0) { S = Sarg; E = Earg; }
+  __syncthreads();
+  Eng eng(S, xs, ps, red);
   auto bump = [&](int c, long long v) { if (lane == 0) cnt[c] += v; };
 
   for (;;) {
@@ -301,10 +240,13 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
     for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? E.psi[(size_t)traj * S.dim + r * G + lane] : make_double2(0., 0.);
     double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
     unsigned long long ord = E.ordinal[traj];
+    const unsigned long long ord0 = ord;
+    const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
     if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
     __syncthreads();
     const unsigned long long key = E.traj0 + (unsigned long long)traj;
     const int nj = S.nj;
+    const double epsAbs = E.epsAbs, epsRel = E.epsRel;
 
     // coherent part of one MCWF step (MCWF_Trajectory.h:250-276 with ODE.h:234-262)
     auto coherent = [&](double Dt) {
@@ -316,17 +258,16 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
       long long derivs = 1, tries = 0, failed = 0;
       for (;;) {
         ++tries;
-        const double err = eng.tryStep(x, k1, dtTry, E.epsAbs, E.epsRel, xn, derivs);
+        const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs);
         if (err > 1.0) {
-          dtTry *= fmax(9. / 10. * pow(err, -1. / (4 - 1)), 1. / 5.);
+          // default_step_adjuster::decrease_step: dt *= max(0.9 err^(-1/(error_order-1)), 1/5)
+          dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
           ++failed;
           continue;
         }
         t += dtTry;
-        if (err < 0.5) {
-          const double e = fmax(pow(5.0, -5.0), err);
-          dtTry *= 9. / 10. * pow(e, -1. / 5);
-        }
+        // increase_step: if err < 0.5, dt *= 0.9 max(5^-stepper_order, err)^(-1/stepper_order)
+        if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
         break;
       }
 #pragma unroll
@@ -338,9 +279,9 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
       double s = 0.;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s += cabs2(x[r]);
-      const double nrm = sqrt(GroupReduce<NW>::sum(s, red));
+      const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s, red));
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) { x[r].x /= nrm; x[r].y /= nrm; }
+      for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
     };
 
     auto computeRates = [&]() {
@@ -361,9 +302,12 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
         const double Dt = endTime - t;
         if (Dt == 0.) break;
         // MCWF_Trajectory::step (MCWF_Trajectory.h:347-374)
+        // the step's jump draw, fetched early so the load completes behind the RK work
+        const unsigned long long di = ord - ord0;
+        const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-          if constexpr (kPcLds) { if (eng.valid(r)) pc[r * G + lane] = x[r]; }
+          if constexpr (kPcLds) pc[r * G + lane] = x[r];
           else pcr[r] = x[r];
         }
         const double tCache = t, dtDidCache = dtDid;
@@ -379,7 +323,7 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
               bump(kStepBacks, 1);
 #pragma unroll
               for (int r = 0; r < ROWS; ++r) {
-                if constexpr (kPcLds) x[r] = eng.valid(r) ? pc[r * G + lane] : make_double2(0., 0.);
+                if constexpr (kPcLds) x[r] = pc[r * G + lane];
                 else x[r] = pcr[r];
               }
               coherent(Dt);
@@ -389,19 +333,21 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
             dtTry = fmin(dtTry, sugg);
             break;
           }
-          // performJump (MCWF_Trajectory.h:320-344)
-          double random = jump_uniform(E.seed, key, ord++) / dtDid;
+          // performJump (MCWF_Trajectory.h:320-344): one draw per step
+          const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);
+          ++ord;
+          double random = u / dtDid;
           int no = 0;
           for (; random > 0. && no != nj; random -= rts[no++]) ;
           if (random < 0.) {
             --no;
             const double nf = sqrt(rts[no]);
             if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
+            const double inv = 1.0 / nf;
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) {
-              if (!eng.valid(r)) continue;
               const double2 y = eng.jumpRow(no, r);
-              x[r] = make_double2(y.x / nf, y.y / nf);
+              x[r] = make_double2(y.x * inv, y.y * inv);
             }
             if (lane == 0) {
               const long long nJ = cnt[kJumps];
@@ -430,6 +376,25 @@ __global__ void __launch_bounds__(64 * NW) mcwf_batch_kernel(SysDev S, EnsDev E)
     if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
     if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
   }
+}
+
+// jump draws for the next `cap` MCWF steps of every trajectory, one Philox evaluation per lane:
+// draws[traj][i] = jump_uniform(seed, traj0 + traj, ordinal[traj] + i)
+__global__ void jump_draws_kernel(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
+                                  int n_traj, int cap, double* draws) {
+  const size_t total = (size_t)n_traj * cap;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const int traj = (int)(i / cap), k = (int)(i % cap);
+    draws[i] = jump_uniform(seed, traj0 + (unsigned long long)traj, ordinal[traj] + (unsigned long long)k);
+  }
+}
+
+hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
+                             int n_traj, int cap, double* draws, hipStream_t st) {
+  const size_t total = (size_t)n_traj * cap;
+  const int grid = (int)std::min<size_t>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(jump_draws_kernel, dim3(grid), dim3(256), 0, st, ordinal, seed, traj0, n_traj, cap, draws);
+  return hipGetLastError();
 }
 
 // ensemble mean over trajectories of the streamed raw averages (deterministic order):
@@ -572,6 +537,17 @@ struct Layout { int rows, nw, wh; };
   X(1, 4, 8) X(1, 4, 16) X(2, 4, 8) X(2, 4, 16) X(4, 4, 16) \
   X(4, 8, 16) X(2, 16, 16) X(4, 16, 16)
 
+// Batched-kernel variants: (rows, waves, ELL width, minimum waves per SIMD requested from the
+// register allocator).  The occupancy hint trades registers (and spills) for latency hiding.
+#define QT_BATCH(X) \
+  X(1, 1, 4, 1) X(1, 1, 4, 4) X(1, 1, 4, 5) X(1, 1, 4, 6) X(1, 1, 4, 8) \
+  X(1, 1, 8, 1) X(1, 1, 8, 4) X(1, 1, 8, 5) \
+  X(1, 1, 16, 1) \
+  X(2, 1, 4, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) \
+  X(4, 1, 8, 1) X(4, 1, 16, 1) \
+  X(1, 4, 8, 1) X(1, 4, 16, 1) X(2, 4, 8, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) \
+  X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
+
 bool layout_supported(int rows, int nw, int wh) {
 #define QT_CASE(R, N, W) if (rows == R && nw == N && wh == W) return true;
   QT_LAYOUTS(QT_CASE)
@@ -579,14 +555,36 @@ bool layout_supported(int rows, int nw, int wh) {
   return false;
 }
 
-hipError_t launch_batch(int rows, int nw, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st) {
-#define QT_CASE(R, N, W)                                                                         \
-  if (rows == R && nw == N && wh == W) {                                                         \
-    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true>), dim3(grid), dim3(64 * N), 0, st, S, E);  \
-    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
-    return hipGetLastError();                                                                    \
+bool batch_supported(int rows, int nw, int wh, int minw) {
+#define QT_CASE(R, N, W, M) if (rows == R && nw == N && wh == W && minw == M) return true;
+  QT_BATCH(QT_CASE)
+#undef QT_CASE
+  return false;
+}
+
+// resident work groups of a batched variant per compute unit (registers and LDS permitting)
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32) {
+  int n = 0;
+#define QT_CASE(R, N, W, M)                                                                                  \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                        \
+    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M>, 64 * N, 0); \
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M>, 64 * N, 0);    \
+    return n;                                                                                                \
   }
-  QT_LAYOUTS(QT_CASE)
+  QT_BATCH(QT_CASE)
+#undef QT_CASE
+  return 0;
+}
+
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E,
+                        hipStream_t st) {
+#define QT_CASE(R, N, W, M)                                                                            \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                  \
+    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M>), dim3(grid), dim3(64 * N), 0, st, S, E); \
+    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
+    return hipGetLastError();                                                                          \
+  }
+  QT_BATCH(QT_CASE)
 #undef QT_CASE
   return hipErrorInvalidValue;
 }

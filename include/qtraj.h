@@ -82,6 +82,8 @@ typedef struct {
   unsigned long long traj0;  /* RNG key word 1 of trajectory 0 (trajectory k uses traj0+k) */
   int jump_log_cap;          /* jump records kept per trajectory */
   int layout_rows, layout_waves;  /* 0 = pick automatically (autotuned table) */
+  int layout_occupancy;      /* minimum waves per SIMD asked of the register allocator; 0 = table */
+  int layout_group_lanes;    /* 16 | 32 | 64: several trajectories per wavefront (G lanes each); 0 = table */
 } qt_mcwf_params;
 
 /* nTraj copies of psi0 (host) on one system: mcwf::makeEnsemble, MCWF_Trajectory.h:213-234,

@@ -52,18 +52,19 @@ def _assert_traj_parity(res_o, psi_g, jt_g, jch_g, nj_g, cnt_g=None):
             assert cnt_g[c] == res_o["counters"][c], (c, cnt_g[c], res_o["counters"][c])
 
 
+@pytest.mark.parametrize("glanes", [0, 64])
 @pytest.mark.parametrize("name,spec,psi0,T,ntraj", [
     ("jc60", JC60, ground(60), 3.0, 48),
     ("plm20", PLM20, coherent(0.5, 20), 2.0, 32),
     ("ptla", PTLA, ground(2), 1.0, 32),
 ])
-def test_batched_fp64_trajectories_match_oracle(name, spec, psi0, T, ntraj):
-    print(f"[gpu] batched fp64 parity {name} ntraj={ntraj} T={T}", flush=True)
+def test_batched_fp64_trajectories_match_oracle(name, spec, psi0, T, ntraj, glanes):
+    print(f"[gpu] batched fp64 parity {name} ntraj={ntraj} T={T} group_lanes={glanes}", flush=True)
     sq, so = Q.System(spec), O.System(spec)
     seed = 20240601
     Dt = 0.1 if name != "ptla" else 0.05
     iv = np.full(int(round(T / Dt)), Dt)
-    ens = Q.Ensemble(sq, ntraj, psi0, Q.mcwf_pars(seed=seed, traj0=0, jump_log_cap=256))
+    ens = Q.Ensemble(sq, ntraj, psi0, Q.mcwf_pars(seed=seed, traj0=0, jump_log_cap=256, group_lanes=glanes))
     rows = ens.advance(iv)
     per, tot = ens.counters()
     njumps = 0
@@ -113,8 +114,11 @@ def test_layouts_agree():
     sq = Q.System(JC60)
     iv = np.full(5, 0.1)
     ref = None
-    for lay in [(1, 1), (2, 1), (4, 1), (1, 4)]:
-        ens = Q.Ensemble(sq, 16, ground(60), Q.mcwf_pars(seed=9, layout=lay))
+    variants = [dict(layout=(1, 1)), dict(layout=(2, 1)), dict(layout=(4, 1)), dict(layout=(1, 4)),
+                dict(group_lanes=16), dict(group_lanes=32), dict(group_lanes=64)]
+    for var in variants:
+        print(f"[gpu] layout {var}", flush=True)
+        ens = Q.Ensemble(sq, 16, ground(60), Q.mcwf_pars(seed=9, **var))
         rows = ens.advance(iv)
         states = np.array([ens.state(k)[0] for k in range(16)])
         if ref is None:
