@@ -99,6 +99,12 @@ def main():
     t_min = f32_flops / P_FP32 + f64_flops / P_FP64
     peak = (f32_flops + f64_flops) / t_min / 1e12
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "mcwf_batch_jc60_16k_traffic.json")
+    if os.path.exists(tpath) and args.ntraj == 16384 and args.T == 20.0:
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu and args.cpu_traj > 0:
         import oracle_lib as O
@@ -127,7 +133,7 @@ def main():
                                "Dt=0.1 streams, FP32 derivatives + FP64 control",
                    "n_traj": args.ntraj, "T": args.T, "dim": 60, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
                      "note": "F = 6*8*354 FP32 + 124*60 FP64 flop per RK try; peak = mix of 157.3 FP32 / 78.6 FP64 vector"},
         "batched_fp64": {"value": v64, "seconds": b64["seconds"], "tries": b64["tries"]},
         "plain_fp64": plain,
