@@ -38,6 +38,10 @@ class McwfParams(C.Structure):
                 ("layout_occupancy", C.c_int), ("layout_group_lanes", C.c_int)]
 
 
+class MasterParams(C.Structure):
+    _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dtInit", C.c_double), ("ss_tol", C.c_double)]
+
+
 def lib():
     """Loads libqtraj.so; raises if it has not been built."""
     global _lib
@@ -63,9 +67,12 @@ def lib():
         l.qt_mcwf_counters.argtypes = [vp, vp, vp]
         l.qt_mcwf_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         l.qt_mcwf_plain_run.argtypes = [vp, C.POINTER(McwfParams), C.c_ulonglong, vp, vp, ip, vp, vp, vp, vp, ip, vp]
-        for name in ("qt_master_create", "qt_master_destroy", "qt_master_advance", "qt_master_rho", "qt_master_kernel_ms"):
-            if not hasattr(l, name):
-                continue
+        l.qt_master_create.argtypes = [vp, ip, C.POINTER(MasterParams), vp, C.POINTER(vp)]
+        l.qt_master_destroy.argtypes = [vp]
+        l.qt_master_advance.argtypes = [vp, vp, ip, vp, vp, vp]
+        l.qt_master_rho.argtypes = [vp, ip, vp]
+        l.qt_master_status.argtypes = [vp, vp, vp, vp, vp]
+        l.qt_master_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = l
     return _lib
 
@@ -333,3 +340,55 @@ def process_averages(system_spec_obs_var, raw):
         if v >= 0:
             out[..., q] -= out[..., v] ** 2
     return out
+
+
+# ---------------------------------------------------------------- Master sweep
+
+class MasterSweep:
+    """A batch of density operators on device, point k under systems[k] (master::make + Master)."""
+
+    def __init__(self, systems, rho0, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0):
+        self.systems = list(systems)
+        self.n = len(self.systems)
+        self.dim = self.systems[0].dim
+        self.n_avr = self.systems[0].n_avr
+        rho0 = np.ascontiguousarray(np.asarray(rho0, dtype=np.complex128))
+        if rho0.shape != (self.dim, self.dim):
+            raise ValueError("rho0 has the wrong shape")
+        arr = (C.c_void_p * self.n)(*[s_.h.value if hasattr(s_.h, "value") else s_.h for s_ in self.systems])
+        self.pars = MasterParams(epsRel, epsAbs, dtInit, ss_tol)
+        h = C.c_void_p()
+        check(lib().qt_master_create(arr, self.n, C.byref(self.pars), rho0.ctypes.data, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qt_master_destroy(self.h)
+            self.h = None
+
+    def advance(self, intervals):
+        """rows[point, k] = (time, dtDid, processed averages); dnorm[point, k] = ||drho/dt||_F."""
+        iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
+        rows = np.zeros((self.n, len(iv), 2 + self.n_avr))
+        dn = np.zeros((self.n, len(iv)))
+        check(lib().qt_master_advance(self.h, iv.ctypes.data, len(iv), rows.ctypes.data, dn.ctypes.data, None))
+        return rows, dn
+
+    def rho(self, point):
+        r = np.zeros((self.dim, self.dim), np.complex128)
+        check(lib().qt_master_rho(self.h, point, r.ctypes.data))
+        return r
+
+    def status(self):
+        done = np.zeros(self.n, np.int32)
+        tss = np.zeros(self.n)
+        t = np.zeros(self.n)
+        cnt = np.zeros((self.n, 8), np.int64)
+        check(lib().qt_master_status(self.h, done.ctypes.data, tss.ctypes.data, t.ctypes.data, cnt.ctypes.data))
+        return done, tss, t, cnt
+
+    def kernel_ms(self):
+        ms = C.c_double()
+        n = C.c_int()
+        check(lib().qt_master_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -35,7 +35,12 @@ hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2*
 hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st);
 hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, const SysDev& S, const double2* psi, double2* dpsidt,
                              int n, hipStream_t st);
+struct MasterDev;
+bool master_supported(int nb);
+hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st);
 }  // namespace qt
+
+#include "qt_master_dev.hpp"
 
 namespace {
 
@@ -245,6 +250,24 @@ struct qt_ens {
     E.dpLimit = p.dpLimit; E.overshootTol = p.overshootTol;
     E.work = work.p; E.err_flag = errflag.p;
     return E;
+  }
+};
+
+struct qt_master {
+  int n = 0, d = 0, nb = 0, nj = 0, wj = 1;
+  qt_master_params p{};
+  qt_sys* sys0 = nullptr;
+  double time = 0.;
+  DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn;
+  DevBuf<int> jcol, done;
+  DevBuf<double2> jval, x, k;
+  DevBuf<long long> cnt;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double kernelMs = 0.;
+  int launches = 0;
+  ~qt_master() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
   }
 };
 
@@ -553,6 +576,164 @@ int qt_mcwf_counters(qt_ens* e, long long* per, long long* totals) {
 int qt_mcwf_kernel_ms(qt_ens* e, double* total, int* launches) {
   if (total) *total = e->kernelMs;
   if (launches) *launches = e->launches;
+  return QT_OK;
+}
+
+int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_params* p, const double* rho0,
+                     qt_master** out) {
+  return guard([&] {
+    if (!systems || !p || !rho0 || !out || n_points <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
+    const qt::System& s0 = *systems[0]->h;
+    const int d = (int)s0.dim;
+    const int NB = (d + 15) / 16, DP = 16 * NB, KC = DP / 4, NN = DP * DP;
+    if (!qt::master_supported(NB)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
+    const int nj = (int)s0.J.size();
+    int wj = 1;
+    for (int k = 0; k < n_points; ++k) {
+      const qt::System& s = *systems[k]->h;
+      if (s.dim != d || (int)s.J.size() != nj) throw Fail(QT_EINVAL, "sweep points must share dimension and jump channels");
+      if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+      for (auto& j : s.J) wj = std::max(wj, j.width);
+    }
+    std::unique_ptr<qt_master> m(new qt_master);
+    m->n = n_points; m->d = d; m->nb = NB; m->p = *p;
+    m->sys0 = systems[0];
+    // H_nH/i A fragments: [point][NB][KC][64] {re, im}, H[16 r + l%16][4 kk + l/16]
+    std::vector<double> hf((size_t)n_points * NB * KC * 64 * 2, 0.);
+    std::vector<int> jc((size_t)n_points * std::max(1, nj) * wj * d, 0);
+    std::vector<double2> jv((size_t)n_points * std::max(1, nj) * wj * d, make_double2(0., 0.));
+    std::vector<double> dtTry(n_points);
+    for (int k = 0; k < n_points; ++k) {
+      const qt::System& s = *systems[k]->h;
+      std::vector<qt::cd> H((size_t)DP * DP, 0.);
+      for (int r = 0; r < d; ++r)
+        for (int w = 0; w < s.H.width; ++w) H[(size_t)r * DP + s.H.cols[(size_t)r * s.H.width + w]] += s.H.vals[(size_t)r * s.H.width + w];
+      for (int rb = 0; rb < NB; ++rb)
+        for (int kk = 0; kk < KC; ++kk)
+          for (int l = 0; l < 64; ++l) {
+            const qt::cd z = H[(size_t)(16 * rb + l % 16) * DP + 4 * kk + l / 16];
+            const size_t o = ((((size_t)k * NB + rb) * KC + kk) * 64 + l) * 2;
+            hf[o] = z.real(); hf[o + 1] = z.imag();
+          }
+      for (int mI = 0; mI < nj; ++mI) {
+        const qt::Ell& e = s.J[mI];
+        for (int w = 0; w < e.width; ++w)
+          for (int r = 0; r < d; ++r) {
+            const size_t o = (((size_t)k * nj + mI) * wj + w) * d + r;
+            jc[o] = e.cols[(size_t)r * e.width + w];
+            const qt::cd z = e.vals[(size_t)r * e.width + w];
+            jv[o] = make_double2(z.real(), z.imag());
+          }
+      }
+      dtTry[k] = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
+    }
+    m->hfrag.upload(hf); m->jcol.upload(jc); m->jval.upload(jv);
+    m->nj = nj; m->wj = wj;
+    systems[0]->upload();  // observables of point 0
+    // rho0 -> tile order: index (4 t + v) 64 + l holds (16 r + 4 v + l/16, 16 c + l%16), t = r NB + c
+    std::vector<double2> x0(NN, make_double2(0., 0.)), xs((size_t)n_points * NN);
+    for (int t = 0; t < NB * NB; ++t)
+      for (int v = 0; v < 4; ++v)
+        for (int l = 0; l < 64; ++l) {
+          const int i = 16 * (t / NB) + 4 * v + l / 16, j = 16 * (t % NB) + l % 16;
+          if (i < d && j < d) x0[(4 * t + v) * 64 + l] = make_double2(rho0[2 * ((size_t)i * d + j)], rho0[2 * ((size_t)i * d + j) + 1]);
+        }
+    for (int k = 0; k < n_points; ++k) std::copy(x0.begin(), x0.end(), xs.begin() + (size_t)k * NN);
+    m->x.upload(xs);
+    m->k.alloc((size_t)n_points * 7 * NN);
+    m->t.upload(std::vector<double>(n_points, 0.));
+    m->dtDid.upload(std::vector<double>(n_points, 0.));
+    m->dtTry.upload(dtTry);
+    m->done.upload(std::vector<int>(n_points, 0));
+    m->tss.upload(std::vector<double>(n_points, -1.));
+    m->cnt.upload(std::vector<long long>((size_t)n_points * qt::kCounters, 0));
+    hipCheck(hipEventCreate(&m->e0), "hipEventCreate");
+    hipCheck(hipEventCreate(&m->e1), "hipEventCreate");
+    *out = m.release();
+  });
+}
+
+int qt_master_destroy(qt_master* m) { delete m; return QT_OK; }
+
+int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* rows, double* dnorm, qt_stream stream) {
+  return guard([&] {
+    if (!m || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    const qt::System& s0 = *m->sys0->h;
+    const int no = (int)s0.O.size();
+    m->intervals.alloc(n_int);
+    hipCheck(hipMemcpyAsync(m->intervals.p, intervals, n_int * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+    m->obs.alloc((size_t)m->n * n_int * std::max(1, no));
+    m->dtd.alloc((size_t)m->n * n_int);
+    m->dn.alloc((size_t)m->n * n_int);
+    qt::MasterDev M{};
+    M.n_points = m->n; M.dim = m->d; M.hfrag = m->hfrag.p;
+    M.nj = m->nj; M.wj = m->wj; M.jcol = m->jcol.p; M.jval = m->jval.p;
+    M.no = no; M.wo = m->sys0->dev.wo; M.ocol = m->sys0->dev.ocol; M.oval = m->sys0->dev.oval; M.oimag = m->sys0->dev.oimag;
+    M.x = m->x.p; M.k = m->k.p; M.t = m->t.p; M.dtDid = m->dtDid.p; M.dtTry = m->dtTry.p;
+    M.done = m->done.p; M.tss = m->tss.p; M.counters = m->cnt.p;
+    M.epsAbs = m->p.epsRel; M.epsRel = m->p.epsAbs;  // ODE.h:205
+    M.ss_tol = m->p.ss_tol;
+    M.intervals = m->intervals.p; M.n_intervals = n_int;
+    M.obs_out = m->obs.p; M.dtdid_out = m->dtd.p; M.dnorm_out = m->dn.p;
+    hipCheck(hipEventRecord(m->e0, st), "hipEventRecord");
+    hipCheck(qt::launch_master(m->nb, M, st), "master launch");
+    hipCheck(hipEventRecord(m->e1, st), "hipEventRecord");
+    std::vector<double> ob((size_t)m->n * n_int * std::max(1, no)), dd((size_t)m->n * n_int), dnv((size_t)m->n * n_int);
+    hipCheck(hipMemcpyAsync(ob.data(), m->obs.p, ob.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(dd.data(), m->dtd.p, dd.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(dnv.data(), m->dn.p, dnv.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    float ms = 0.f;
+    hipCheck(hipEventElapsedTime(&ms, m->e0, m->e1), "hipEventElapsedTime");
+    m->kernelMs += ms; m->launches += 1;
+    const double tStart = m->time;
+    for (int k = 0; k < n_int; ++k) m->time += intervals[k];
+    for (int pt = 0; pt < m->n; ++pt) {
+      double tt = tStart;
+      for (int k = 0; k < n_int; ++k) {
+        tt += intervals[k];
+        if (rows) {
+          double* r = rows + ((size_t)pt * n_int + k) * (2 + no);
+          r[0] = tt; r[1] = dd[(size_t)pt * n_int + k];
+          for (int q = 0; q < no; ++q) r[2 + q] = ob[((size_t)pt * n_int + k) * no + q];
+          process_rows(s0, r + 2);
+        }
+        if (dnorm) dnorm[(size_t)pt * n_int + k] = dnv[(size_t)pt * n_int + k];
+      }
+    }
+  });
+}
+
+int qt_master_rho(qt_master* m, int point, double* rho) {
+  return guard([&] {
+    if (!m || point < 0 || point >= m->n || !rho) throw Fail(QT_EINVAL, "bad arguments");
+    const int d = m->d, NB = m->nb, DP = 16 * NB, NN = DP * DP;
+    std::vector<double2> x(NN);
+    hipCheck(hipMemcpy(x.data(), m->x.p + (size_t)point * NN, NN * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
+    for (int t = 0; t < NB * NB; ++t)
+      for (int v = 0; v < 4; ++v)
+        for (int l = 0; l < 64; ++l) {
+          const int i = 16 * (t / NB) + 4 * v + l / 16, j = 16 * (t % NB) + l % 16;
+          if (i < d && j < d) { rho[2 * ((size_t)i * d + j)] = x[(4 * t + v) * 64 + l].x; rho[2 * ((size_t)i * d + j) + 1] = x[(4 * t + v) * 64 + l].y; }
+        }
+  });
+}
+
+int qt_master_status(qt_master* m, int* done, double* tss, double* t, long long* counters) {
+  return guard([&] {
+    if (!m) throw Fail(QT_EINVAL, "null argument");
+    if (done) hipCheck(hipMemcpy(done, m->done.p, m->n * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    if (tss) hipCheck(hipMemcpy(tss, m->tss.p, m->n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    if (t) hipCheck(hipMemcpy(t, m->t.p, m->n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    if (counters) hipCheck(hipMemcpy(counters, m->cnt.p, (size_t)m->n * qt::kCounters * sizeof(long long), hipMemcpyDeviceToHost), "D2H");
+  });
+}
+
+int qt_master_kernel_ms(qt_master* m, double* total, int* launches) {
+  if (total) *total = m->kernelMs;
+  if (launches) *launches = m->launches;
   return QT_OK;
 }
 

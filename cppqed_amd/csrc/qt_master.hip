@@ -1,0 +1,355 @@
+// qt_master.hip -- batched Master-equation sweep on FP64 MFMA (gfx950).
+//
+// One work group of four wavefronts per sweep point.  The density operator rho (d x d
+// complex FP64, d padded to DP = 16 NB) is cut into NB x NB tiles of 16 x 16; tile
+// t = r NB + c belongs to wavefront t % 4, which accumulates its block of
+// M = (H_nH/i) S with v_mfma_f64_16x16x4_f64 (four real MFMAs per complex 16x16x4 block):
+// the A operand is the H_nH/i row block r (read from the point's fragment-ordered copy,
+// L2-resident), the B operand the RK stage matrix S staged in LDS as real / imaginary
+// planes.  Then, as Master::step's derivative (Master.h:157-184),
+//     drho = M + M^dagger + sum_m J_m S J_m^dagger,
+// formed as N = M + J S J^dagger / 2 (the jump term is Hermitian) and N + N^dagger through an
+// LDS transpose.  Step-size control is ode_engine::Base::step with Boost.Odeint's RKCK54
+// controller (ODE.h:234-262); after each step rho is made Hermitian (2 Re, CMatrix.cc:14-30)
+// and renormalised by its trace (Master.h:196-206).
+//
+// MFMA lane layout (measured on gfx950): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane 16 (i % 4) + j, component i / 4.  Lane l of the tile's wavefront owns the
+// tile elements (4v + l/16, l%16), v < 4; matrices in HBM are stored tile by tile in that
+// order: index (4 t + v) 64 + l.
+
+#include "qt_device.hpp"
+#include "qt_kernels.hpp"
+#include "qt_master_dev.hpp"
+
+#include <algorithm>
+
+namespace qt {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMasterWaves = 4;
+
+// K = M + M^dagger + sum_m J_m S J_m^dagger for the stage matrix S in LDS (Sr, Si planes);
+// kept out of line so that the six call sites per RK try share one register allocation.
+template <int NB>
+__device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout) {
+  constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB;
+  constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int d = M.dim;
+  const double* const hf = M.hfrag + (size_t)p * NB * KC * 64 * 2;
+  double* const SrL = Sr + (lane >> 4) * LD + (lane & 15);
+  double* const SiL = Si + (lane >> 4) * LD + (lane & 15);
+  const double* const SrT = Sr + (lane & 15) * LD + (lane >> 4);
+  const double* const SiT = Si + (lane & 15) * LD + (lane >> 4);
+  d4 cr[TW], ci[TW];
+#pragma unroll
+  for (int tw = 0; tw < TW; ++tw) { cr[tw] = d4{0., 0., 0., 0.}; ci[tw] = d4{0., 0., 0., 0.}; }
+  // M = (H_nH/i) S on FP64 MFMA
+#pragma unroll 1
+  for (int kk = 0; kk < KC; ++kk) {
+#pragma unroll
+    for (int tw = 0; tw < TW; ++tw) {
+      const int t = w + kMasterWaves * tw;
+      if (t >= NT) continue;
+      const int r = t / NB, c = t % NB;
+      const double* a = hf + ((size_t)(r * KC + kk) * 64 + lane) * 2;
+      const double ar = a[0], ai = a[1];
+      const double br = SrL[4 * kk * LD + 16 * c], bi = SiL[4 * kk * LD + 16 * c];
+      cr[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr[tw], 0, 0, 0);
+      cr[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, cr[tw], 0, 0, 0);
+      ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci[tw], 0, 0, 0);
+      ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci[tw], 0, 0, 0);
+    }
+  }
+  // N = M + (sum_m J_m S J_m^dagger) / 2, S still in LDS
+  if (M.nj > 0) {
+#pragma unroll
+    for (int tw = 0; tw < TW; ++tw) {
+      const int t = w + kMasterWaves * tw;
+      if (t >= NT) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * (t / NB) + 4 * v + (lane >> 4), j = 16 * (t % NB) + (lane & 15);
+        if (i >= d || j >= d) continue;
+        double ax = 0., ay = 0.;
+        for (int m = 0; m < M.nj; ++m) {
+          const size_t base = ((size_t)p * M.nj + m) * M.wj * d;
+          for (int aa = 0; aa < M.wj; ++aa) {
+            const double2 ja = M.jval[base + (size_t)aa * d + i];
+            if (ja.x == 0. && ja.y == 0.) continue;
+            const int ca = M.jcol[base + (size_t)aa * d + i];
+            for (int b = 0; b < M.wj; ++b) {
+              const double2 jb = M.jval[base + (size_t)b * d + j];
+              if (jb.x == 0. && jb.y == 0.) continue;
+              const int cb = M.jcol[base + (size_t)b * d + j];
+              const double sx = Sr[ca * LD + cb], sy = Si[ca * LD + cb];
+              const double ux = ja.x * sx - ja.y * sy, uy = ja.x * sy + ja.y * sx;  // J_ia S_{ca,cb}
+              ax += ux * jb.x + uy * jb.y;                                          // ... conj(J_jb)
+              ay += uy * jb.x - ux * jb.y;
+            }
+          }
+        }
+        cr[tw][v] += 0.5 * ax;
+        ci[tw][v] += 0.5 * ay;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int tw = 0; tw < TW; ++tw) {
+    const int t = w + kMasterWaves * tw;
+    if (t >= NT) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int o = (16 * (t / NB) + 4 * v) * LD + 16 * (t % NB);
+      SrL[o] = cr[tw][v]; SiL[o] = ci[tw][v];
+    }
+  }
+  __syncthreads();
+  // drho_ij = N_ij + conj(N_ji)
+#pragma unroll
+  for (int tw = 0; tw < TW; ++tw) {
+    const int t = w + kMasterWaves * tw;
+    if (t >= NT) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int o = 16 * (t % NB) * LD + 16 * (t / NB) + 4 * v;
+      Kout[(4 * t + v) * 64 + lane] = make_double2(cr[tw][v] + SrT[o], ci[tw][v] - SiT[o]);
+    }
+  }
+}
+
+
+template <int NB>
+__global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) {
+  constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB, NN = DP * DP;
+  constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;  // tiles per wavefront (at most)
+  __shared__ double Sr[DP * LD];
+  __shared__ double Si[DP * LD];
+  __shared__ double red[kMasterWaves];
+  __shared__ long long cnt[kCounters];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int p = blockIdx.x;
+  if (p >= M.n_points) return;
+  const int d = M.dim;
+  const double* const hf = M.hfrag + (size_t)p * NB * KC * 64 * 2;
+
+  double2* const X = M.x + (size_t)p * NN;
+  double2* const K = M.k + (size_t)p * 7 * NN;
+  double2* const K1 = K;
+  double2* const K2 = K + NN;
+  double2* const K3 = K + 2 * NN;
+  double2* const K4 = K + 3 * NN;
+  double2* const K5 = K + 4 * NN;
+  double2* const K6 = K + 5 * NN;
+  double2* const XN = K + 6 * NN;
+
+  // element e = (tw, v) of this lane: tile t = w + 4 tw, row 16 r + 4 v + lane/16, column 16 c + lane%16
+  auto tileOf = [&](int tw) { return w + kMasterWaves * tw; };
+  auto has = [&](int tw) { return tileOf(tw) < NT; };
+  auto gidx = [&](int tw, int v) { return (4 * tileOf(tw) + v) * 64 + lane; };
+  auto rowOf = [&](int tw, int v) { return 16 * (tileOf(tw) / NB) + 4 * v + (lane >> 4); };
+  auto colOf = [&](int tw) { return 16 * (tileOf(tw) % NB) + (lane & 15); };
+  double* const SrL = Sr + (lane >> 4) * LD + (lane & 15);   // + (16 r + 4 v) LD + 16 c
+  double* const SiL = Si + (lane >> 4) * LD + (lane & 15);
+  const double* const SrT = Sr + (lane & 15) * LD + (lane >> 4);  // + 16 c LD + 16 r + 4 v
+  const double* const SiT = Si + (lane & 15) * LD + (lane >> 4);
+  auto lofs = [&](int tw, int v) { const int t = tileOf(tw); return (16 * (t / NB) + 4 * v) * LD + 16 * (t % NB); };
+  auto tofs = [&](int tw, int v) { const int t = tileOf(tw); return 16 * (t % NB) * LD + 16 * (t / NB) + 4 * v; };
+
+  auto bsum = [&](double v) {
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+  };
+  auto bmax = [&](double v) {
+    v = wave_max(v);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  };
+
+  // Kout = F(S); S given element-wise by stage(tw, v)
+  auto derivative = [&](auto&& stage, double2* Kout) {
+    __syncthreads();
+    for (int tw = 0; tw < TW; ++tw) {
+      if (!has(tw)) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 sv = stage(tw, v);
+        SrL[lofs(tw, v)] = sv.x; SiL[lofs(tw, v)] = sv.y;
+      }
+    }
+    __syncthreads();
+    master_apply<NB>(M, p, Sr, Si, Kout);
+  };
+  auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
+
+  double t = M.t[p], dtDid = M.dtDid[p], dtTry = M.dtTry[p];
+  if (tid < kCounters) cnt[tid] = M.counters[(size_t)p * kCounters + tid];
+  __syncthreads();
+  int done = M.done[p];
+
+  for (int kI = 0; kI < M.n_intervals; ++kI) {
+    const double endTime = t + M.intervals[kI];
+    if (!done) {
+      for (;;) {
+        const double Dt = endTime - t;
+        if (Dt == 0.) break;
+        // Master::step -> ode_engine::Base::step(Dt)
+        const double timeBefore = t;
+        const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
+        if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
+        if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
+        derivative([&](int tw, int v) { return ld(X, tw, v); }, K1);  // k1 at (x, t); unchanged on retries
+        long long tries = 0, failed = 0;
+        for (;;) {
+          ++tries;
+          const double dt = dtTry;
+          derivative([&](int tw, int v) { return axpy(CK::a21 * dt, ld(K1, tw, v), ld(X, tw, v)); }, K2);
+          derivative([&](int tw, int v) {
+            return axpy(CK::a32 * dt, ld(K2, tw, v), axpy(CK::a31 * dt, ld(K1, tw, v), ld(X, tw, v)));
+          }, K3);
+          derivative([&](int tw, int v) {
+            return axpy(CK::a43 * dt, ld(K3, tw, v), axpy(CK::a42 * dt, ld(K2, tw, v),
+                   axpy(CK::a41 * dt, ld(K1, tw, v), ld(X, tw, v))));
+          }, K4);
+          derivative([&](int tw, int v) {
+            return axpy(CK::a54 * dt, ld(K4, tw, v), axpy(CK::a53 * dt, ld(K3, tw, v), axpy(CK::a52 * dt, ld(K2, tw, v),
+                   axpy(CK::a51 * dt, ld(K1, tw, v), ld(X, tw, v)))));
+          }, K5);
+          derivative([&](int tw, int v) {
+            return axpy(CK::a65 * dt, ld(K5, tw, v), axpy(CK::a64 * dt, ld(K4, tw, v), axpy(CK::a63 * dt, ld(K3, tw, v),
+                   axpy(CK::a62 * dt, ld(K2, tw, v), axpy(CK::a61 * dt, ld(K1, tw, v), ld(X, tw, v))))));
+          }, K6);
+          double emax = 0.;
+          for (int tw = 0; tw < TW; ++tw) {
+            if (!has(tw)) break;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const double2 x = ld(X, tw, v), k1 = ld(K1, tw, v), k3 = ld(K3, tw, v), k4 = ld(K4, tw, v),
+                            k5 = ld(K5, tw, v), k6 = ld(K6, tw, v);
+              XN[gidx(tw, v)] = axpy(CK::b6 * dt, k6, axpy(CK::b4 * dt, k4, axpy(CK::b3 * dt, k3, axpy(CK::b1 * dt, k1, x))));
+              double2 xe = make_double2(CK::e1 * dt * k1.x, CK::e1 * dt * k1.y);
+              xe = axpy(CK::e6 * dt, k6, axpy(CK::e5 * dt, k5, axpy(CK::e4 * dt, k4, axpy(CK::e3 * dt, k3, xe))));
+              const double D = M.epsAbs + M.epsRel * (sqrt(cabs2(x)) + dt * sqrt(cabs2(k1)));
+              emax = fmax(emax, cabs2(xe) / (D * D));
+            }
+          }
+          const double err = sqrt(bmax(emax));
+          if (err > 1.0) {
+            dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
+            ++failed;
+            continue;
+          }
+          t += dtTry;
+          if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
+          break;
+        }
+        dtDid = t - timeBefore;
+        if (nextDtTry != 0.) dtTry = nextDtTry;
+        // calculateTwoTimesRealPartOfSelf (CMatrix.cc:14-30), then trace renormalisation
+        __syncthreads();
+        for (int tw = 0; tw < TW; ++tw) {
+          if (!has(tw)) break;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { const double2 y = ld(XN, tw, v); SrL[lofs(tw, v)] = y.x; SiL[lofs(tw, v)] = y.y; }
+        }
+        __syncthreads();
+        double tr = 0.;
+        for (int tw = 0; tw < TW; ++tw) {
+          if (!has(tw)) break;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = rowOf(tw, v), j = colOf(tw);
+            if (i == j && i < d) tr += 2. * SrL[lofs(tw, v)];
+          }
+        }
+        const double inv = 1.0 / bsum(tr);
+        for (int tw = 0; tw < TW; ++tw) {
+          if (!has(tw)) break;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = rowOf(tw, v), j = colOf(tw);
+            const double xr = SrL[lofs(tw, v)], xi = SiL[lofs(tw, v)], yr = SrT[tofs(tw, v)], yi = SiT[tofs(tw, v)];
+            double2 y;
+            if (i == j) y = make_double2(2. * xr, 0.);
+            else if (i < j) y = make_double2(xr + yr, xi - yi);
+            else y = make_double2(yr + xr, -(yi - xi));
+            X[gidx(tw, v)] = make_double2(y.x * inv, y.y * inv);
+          }
+        }
+        if (tid == 0) {
+          cnt[kMcwfSteps] += 1; cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed;
+          cnt[kDerivs] += 6 * tries;
+        }
+      }
+    }
+    // stream point: averages Tr(rho O_q), ||drho/dt||_F, steady-state check
+    __syncthreads();
+    for (int tw = 0; tw < TW; ++tw) {
+      if (!has(tw)) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { const double2 y = ld(X, tw, v); SrL[lofs(tw, v)] = y.x; SiL[lofs(tw, v)] = y.y; }
+    }
+    __syncthreads();
+    for (int q = 0; q < M.no; ++q) {
+      double part = 0.;
+      const bool im = M.oimag[q] != 0;
+      for (int i = tid; i < d; i += 64 * kMasterWaves) {
+        for (int a = 0; a < M.wo; ++a) {
+          const double2 o = M.oval[((size_t)q * M.wo + a) * d + i];
+          const int cidx = M.ocol[((size_t)q * M.wo + a) * d + i];
+          // Tr(rho O) = sum_i sum_a O_{i,c} rho_{c,i}
+          const double rr = Sr[cidx * LD + i], ri = Si[cidx * LD + i];
+          part += im ? (o.x * ri + o.y * rr) : (o.x * rr - o.y * ri);
+        }
+      }
+      const double v = bsum(part);
+      if (tid == 0) M.obs_out[((size_t)p * M.n_intervals + kI) * M.no + q] = v;
+    }
+    derivative([&](int tw, int v) { return ld(X, tw, v); }, K1);
+    double n2 = 0.;
+    for (int tw = 0; tw < TW; ++tw) {
+      if (!has(tw)) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) n2 += cabs2(ld(K1, tw, v));
+    }
+    const double dn = sqrt(bsum(n2));
+    if (tid == 0) {
+      M.dtdid_out[(size_t)p * M.n_intervals + kI] = dtDid;
+      M.dnorm_out[(size_t)p * M.n_intervals + kI] = dn;
+    }
+    if (!done && M.ss_tol > 0. && dn < M.ss_tol) {
+      done = 1;
+      if (tid == 0) { M.done[p] = 1; M.tss[p] = t; }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) { M.t[p] = t; M.dtDid[p] = dtDid; M.dtTry[p] = dtTry; }
+  if (tid < kCounters) M.counters[(size_t)p * kCounters + tid] = cnt[tid];
+}
+
+bool master_supported(int nb) { return nb >= 1 && nb <= 6; }
+
+hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st) {
+  const dim3 block(64 * kMasterWaves);
+  switch (nb) {
+    case 1: hipLaunchKernelGGL(master_kernel<1>, dim3(M.n_points), block, 0, st, M); break;
+    case 2: hipLaunchKernelGGL(master_kernel<2>, dim3(M.n_points), block, 0, st, M); break;
+    case 3: hipLaunchKernelGGL(master_kernel<3>, dim3(M.n_points), block, 0, st, M); break;
+    case 4: hipLaunchKernelGGL(master_kernel<4>, dim3(M.n_points), block, 0, st, M); break;
+    case 5: hipLaunchKernelGGL(master_kernel<5>, dim3(M.n_points), block, 0, st, M); break;
+    case 6: hipLaunchKernelGGL(master_kernel<6>, dim3(M.n_points), block, 0, st, M); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qt

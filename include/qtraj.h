@@ -119,6 +119,33 @@ int qt_mcwf_counters(qt_ens* ens, long long* per_traj, long long* totals);
 /* Average duration (ms) of the batched kernel launches so far, from HIP events on the launch stream. */
 int qt_mcwf_kernel_ms(qt_ens* ens, double* total_ms, int* launches);
 
+/* ------------------------------------------------------- Master-equation sweep */
+
+typedef struct {
+  double epsRel, epsAbs;   /* user values, as for MCWF (tolerance swap of ODE.h:205 applied inside) */
+  double dtInit;           /* <= 0: 1/(10*highest_frequency) of each point's system */
+  double ss_tol;           /* stop a point once ||drho/dt||_F < ss_tol at a stream point (<= 0: never) */
+} qt_master_params;
+
+/* A batch of density operators, point k evolving under systems[k] (same dimension and jump
+ * structure), all from rho0_host (d*d complex, row-major): master::make + Master
+ * (CPPQEDcore/quantumtrajectory/Master.h:130-136, 50-208) for every sweep point.  The
+ * (H_nH/i) rho products run on FP64 MFMA; dimensions up to 96. */
+int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_params* p, const double* rho0_host,
+                     qt_master** out);
+int qt_master_destroy(qt_master* m);
+/* Advances every point through the stream intervals (Master::step via trajectory::run's Dt
+ * loop, Trajectory.h:560-595); rows_host[(point*n_intervals + k)*(2+n_avr)] = time, dtDid,
+ * processed averages (DensityOperatorStreamer); dnorm_host[point*n_intervals + k] =
+ * ||drho/dt||_F at that stream point.  Either may be NULL. */
+int qt_master_advance(qt_master* m, const double* intervals, int n_intervals, double* rows_host, double* dnorm_host,
+                      qt_stream stream);
+/* rho of one point (d*d complex, row-major) */
+int qt_master_rho(qt_master* m, int point, double* rho_host);
+/* per point: steady-state flag, the time it was reached, current time, counters[8] */
+int qt_master_status(qt_master* m, int* done, double* t_ss, double* t, long long* counters);
+int qt_master_kernel_ms(qt_master* m, double* total_ms, int* launches);
+
 /* -------------------------------------------------------------- plain form */
 
 /* The unbatched GPU form: one trajectory at a time, the MCWF/ODE control on the host

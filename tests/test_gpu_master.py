@@ -1,0 +1,44 @@
+"""GPU parity of the FP64-MFMA Master sweep against the CPU oracle and the reference's golden output."""
+import numpy as np
+import pytest
+
+import cppqed_amd as Q
+import oracle_lib as O
+from sysdefs import PTLA, jc_master_spec
+from test_oracle import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def test_master_ptla_golden():
+    """Testing/expected/run_PTLA_CPPQED_dt_master replayed by the device Master (d=2, one tile)."""
+    print("[gpu] master ptla golden", flush=True)
+    s = Q.System(PTLA)
+    m = Q.MasterSweep([s], np.diag([1.0, 0.0]), epsRel=1e-6, epsAbs=1e-5)
+    iv = Q.run_intervals(T=1.0, Dt=0.1)
+    rows, _ = m.advance(iv)
+    g = load_golden("run_PTLA_CPPQED_dt_master")
+    np.testing.assert_allclose(rows[0], g[1:], rtol=1e-5, atol=1e-8)
+
+
+@pytest.mark.parametrize("cutoff,etas,T", [(4, [0.6, 2.0], 1.0), (8, [1.0, 3.0, 5.0], 0.5), (40, [0.5, 7.0], 0.2)])
+def test_master_sweep_matches_oracle(cutoff, etas, T):
+    print(f"[gpu] master sweep cutoff={cutoff} etas={etas}", flush=True)
+    specs = [jc_master_spec(cutoff, eta=e, g=10.0).replace("gamma=0", "gamma=0.3") for e in etas]
+    sq = [Q.System(sp) for sp in specs]
+    d = sq[0].dim
+    rho0 = np.zeros((d, d), complex)
+    rho0[0, 0] = 1
+    iv = np.full(int(round(T / 0.05)), 0.05)
+    m = Q.MasterSweep(sq, rho0)
+    rows, dn = m.advance(iv)
+    _, _, _, cnt = m.status()
+    for k, sp in enumerate(specs):
+        so = O.System(sp)
+        rows_o, rho_o = O.master_run(so, rho0, O.params(), O.run_spec("NDt", Dt=0.05, NDt=len(iv)))
+        rho_g = m.rho(k)
+        err = np.linalg.norm(rho_g - rho_o) / np.linalg.norm(rho_o)
+        assert err < 1e-9, (k, err)
+        np.testing.assert_allclose(rows[k][:, 2:], rows_o[1:, 2:], rtol=1e-8, atol=1e-10)
+        assert np.allclose(rho_g, rho_g.conj().T, atol=0)
+        assert abs(np.trace(rho_g) - 1) < 1e-13
