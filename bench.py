@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--cpu-traj", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-master", action="store_true")
+    ap.add_argument("--master-points", type=int, default=512)
+    ap.add_argument("--master-T", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -116,6 +119,40 @@ def main():
         cpu = {"value": ctries / cel, "unit": "RK try-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_traj} JC-60 trajectories to T={args.T}, FP64, 1 thread ({ctries} tries, {cel:.1f} s)"}
 
+    master = None
+    if not args.no_master:
+        from sysdefs import jc_master_spec
+        etas = 0.5 + np.arange(args.master_points) * 14.5 / max(1, args.master_points - 1)
+        msys = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in etas]
+        rho0 = np.zeros((80, 80), complex)
+        rho0[0, 0] = 1
+        sweep = Q.MasterSweep(msys, rho0, epsRel=1e-12, epsAbs=1e-12, ss_tol=1e-8)
+        t0 = time.perf_counter()
+        chunks = 0
+        while chunks < args.master_T // 5:
+            _, dn = sweep.advance(np.full(5, 1.0))
+            chunks += 1
+            if sweep.status()[0].all():
+                break
+        mel = time.perf_counter() - t0
+        done, tss, tm, mcnt = sweep.status()
+        mtries = int(mcnt[:, 4].sum())
+        mms, _ = sweep.kernel_ms()
+        d = 80
+        f_gemm = 6 * 8 * d ** 3                  # dense complex H rho per RHS, on FP64 MFMA
+        f_try = 6 * (8 * d ** 3 + 2 * 8 * d * d + 4 * d * d) + 124 * d * d
+        m_ach = f_gemm * mtries / (mms / 1e3) / 1e12
+        master = {"workload": f"{args.master_points} driven Jaynes-Cummings points, mode cutoff 40 (d=80), g=10, kappa=1, "
+                              "eta in [0.5, 15], epsRel=epsAbs=1e-12, stop at ||drho/dt||_F < 1e-8",
+                  "tries": mtries, "seconds": mel, "tries_per_s": mtries / mel,
+                  "converged": int(done.sum()), "t_end": float(tm.max()),
+                  "residual_median": float(np.median(dn[:, -1])), "residual_max": float(dn[:, -1].max()),
+                  "t_steady_median": float(np.median(tss[tss > 0])) if (tss > 0).any() else None,
+                  "roofline": {"bound": "mfma", "achieved": m_ach, "peak": 78.6, "unit": "TFLOP/s", "frac": m_ach / 78.6,
+                               "traffic": None,
+                               "note": "8 d^3 flop of dense complex (H_nH/i) rho per derivative, 6 per try; "
+                                       f"{f_try / 1e6:.1f} MFLOP per try in all, sparse-equivalent 1.82 MFLOP"}}
+
     out = {
         "metric": "MCWF RK steps/s (JC dim 60, 16k traj) batched vs plain; p99 latency",
         "value": value,
@@ -141,6 +178,7 @@ def main():
         "batched_fp64_over_plain": (v64 / plain["value"]) if plain else None,
         "kernel": {"launches": b32["launches"], "avg_ms": 1e3 * b32["kernel_s"] / max(1, b32["launches"])},
         "cpu_baseline": cpu,
+        "master_sweep": master,
     }
     print(json.dumps(out))
 
