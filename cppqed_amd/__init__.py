@@ -42,6 +42,10 @@ class MasterParams(C.Structure):
     _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dtInit", C.c_double), ("ss_tol", C.c_double)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("variant", C.c_int), ("team", C.c_int * 16)]
+
+
 def lib():
     """Loads libqtraj.so; raises if it has not been built."""
     global _lib
@@ -67,6 +71,13 @@ def lib():
         l.qt_mcwf_counters.argtypes = [vp, vp, vp]
         l.qt_mcwf_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         l.qt_mcwf_plain_run.argtypes = [vp, C.POINTER(McwfParams), C.c_ulonglong, vp, vp, ip, vp, vp, vp, vp, ip, vp]
+        l.qt_batch_variants.argtypes = [vp, ip]
+        l.qt_batch_create.argtypes = [vp, ip, C.POINTER(BatchParams), C.POINTER(vp)]
+        l.qt_batch_destroy.argtypes = [vp]
+        l.qt_batch_layout.argtypes = [vp, C.POINTER(C.c_int), vp]
+        l.qt_batch_advance.argtypes = [vp, vp, vp, vp, vp]
+        l.qt_batch_autotune.argtypes = [vp, dp, ip, C.POINTER(C.c_int), vp, ip]
+        l.qt_batch_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         l.qt_master_create.argtypes = [vp, ip, C.POINTER(MasterParams), vp, C.POINTER(vp)]
         l.qt_master_destroy.argtypes = [vp]
         l.qt_master_advance.argtypes = [vp, vp, ip, vp, vp, vp]
@@ -314,6 +325,65 @@ class Ensemble:
         return ms.value, n.value
 
 
+def batch_variants():
+    """Compiled mixed-batch layouts: (waves per work group, rows/lane and ELL register width of the
+    work-group team, rows/lane and width of the wavefront team)."""
+    n = lib().qt_batch_variants(None, 0)
+    t = np.zeros(5 * n, np.int32)
+    lib().qt_batch_variants(t.ctypes.data, n)
+    return [tuple(int(x) for x in t[5 * i:5 * i + 5]) for i in range(n)]
+
+
+class Batch:
+    """Several ensembles (different systems) advanced by one persistent launch (qt_batch_*)."""
+
+    def __init__(self, members, variant=-1, team=None):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        p = BatchParams(variant, (C.c_int * 16)(*([-1] * 16)))
+        if team is not None:
+            for i, t in enumerate(team):
+                p.team[i] = t
+        h = C.c_void_p()
+        check(lib().qt_batch_create(arr, len(self.members), C.byref(p), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qt_batch_destroy(self.h)
+            self.h = None
+
+    def layout(self):
+        v = C.c_int()
+        team = np.zeros(len(self.members), np.int32)
+        check(lib().qt_batch_layout(self.h, C.byref(v), team.ctypes.data))
+        return v.value, team.tolist()
+
+    def advance(self, intervals):
+        """intervals: one list for every member, or a list of per-member lists; returns per-member rows."""
+        if np.ndim(intervals[0]) == 0:
+            intervals = [intervals] * len(self.members)
+        ivs = [np.ascontiguousarray(np.asarray(iv, dtype=np.float64)) for iv in intervals]
+        rows = [np.zeros((len(iv), 2 + m.system.n_avr)) for iv, m in zip(ivs, self.members)]
+        ivp = (C.c_void_p * len(ivs))(*[iv.ctypes.data for iv in ivs])
+        nis = np.array([len(iv) for iv in ivs], np.int32)
+        rp = (C.c_void_p * len(rows))(*[r.ctypes.data for r in rows])
+        check(lib().qt_batch_advance(self.h, ivp, nis.ctypes.data, rp, None))
+        return rows
+
+    def autotune(self, probe_span=0.5, probe_traj=2048):
+        v = C.c_int()
+        ms = np.zeros(64)
+        check(lib().qt_batch_autotune(self.h, probe_span, probe_traj, C.byref(v), ms.ctypes.data, 64))
+        return v.value, ms[ms > 0]
+
+    def kernel_ms(self):
+        ms = C.c_double()
+        n = C.c_int()
+        check(lib().qt_batch_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
 COUNTER_NAMES = ("mcwf_steps", "ode_steps", "failed", "derivs", "tries", "step_backs", "overshoots", "jumps")
 
 
@@ -331,15 +401,6 @@ def mcwf_plain_run(system, psi0, intervals, pars=None, traj=0, jcap=4096):
                                   psi.ctypes.data, jt.ctypes.data, jch.ctypes.data, jcap, cnt.ctypes.data))
     nj = int(min(cnt[7], jcap))
     return dict(obs=obs, psi=psi, jump_t=jt[:nj], jump_ch=jch[:nj], counters=cnt)
-
-
-def process_averages(system_spec_obs_var, raw):
-    """Applies Averaged::process (VAR of the number operator) to raw linear averages."""
-    out = np.array(raw, dtype=np.float64, copy=True)
-    for q, v in enumerate(system_spec_obs_var):
-        if v >= 0:
-            out[..., q] -= out[..., v] ** 2
-    return out
 
 
 # ---------------------------------------------------------------- Master sweep

@@ -27,6 +27,11 @@ hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, 
                         hipStream_t st);
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
                              int n_traj, int cap, double* draws, hipStream_t st);
+bool mixed_supported(int nwb, int rb, int whb, int rw, int whw);
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32);
+int mixed_variants(int* table, int cap);
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st);
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
 hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
                                 double* out_dt, hipStream_t st);
 hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st);
@@ -155,7 +160,7 @@ void check_params(const qt_mcwf_params* p) {
 struct qt_sys {
   std::unique_ptr<qt::System> h;
   bool uploaded = false;
-  DevBuf<int> hcol, jcol, ocol, oimag;
+  DevBuf<int> hcol, jcol, ocol, oimag, owid;
   DevBuf<double2> hval, jval, oval;
   DevBuf<float2> hval32;
   qt::SysDev dev{};
@@ -199,9 +204,10 @@ struct qt_sys {
     int wo = 1;
     for (auto& o : s.O) wo = std::max(wo, o.width);
     c.clear(); v.clear();
-    std::vector<int> im;
+    std::vector<int> im, ow;
     for (size_t q = 0; q < s.O.size(); ++q) {
       const qt::Ell& o = s.O[q];
+      ow.push_back(o.width);
       for (int w = 0; w < wo; ++w)
         for (int r = 0; r < d; ++r) {
           c.push_back(w < o.width ? o.cols[(size_t)r * o.width + w] : r);
@@ -210,12 +216,12 @@ struct qt_sys {
         }
       im.push_back(s.obs[q].imagPart ? 1 : 0);
     }
-    if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); im.push_back(0); }
-    ocol.upload(c); oval.upload(v); oimag.upload(im);
+    if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); im.push_back(0); ow.push_back(0); }
+    ocol.upload(c); oval.upload(v); oimag.upload(im); owid.upload(ow);
     dev.dim = d; dev.wh = s.H.width;
     dev.hcol = hcol.p; dev.hval = hval.p; dev.hval32 = hval32.p;
     dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
-    dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p;
+    dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p; dev.owid = owid.p;
     uploaded = true;
   }
 };
@@ -300,7 +306,10 @@ int resident_groups(const LayoutChoice& L, bool f32) {
 
 void process_rows(const qt::System& s, double* av) {
   for (size_t q = 0; q < s.processVarOf.size(); ++q)
-    if (s.processVarOf[q] >= 0) av[q] -= av[s.processVarOf[q]] * av[s.processVarOf[q]];
+    if (s.processVarOf[q] >= 0) {
+      av[q] -= av[s.processVarOf[q]] * av[s.processVarOf[q]];
+      if (s.processSqrt[q]) av[q] = std::sqrt(av[q]);
+    }
 }
 
 }  // namespace
@@ -434,7 +443,9 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
 
 int qt_mcwf_destroy(qt_ens* ens) { delete ens; return QT_OK; }
 
-static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool timed) {
+// Per-launch preparation of one ensemble: uploads the intervals, sizes the stream outputs,
+// generates the launch's jump draws; returns the kernel's view of the ensemble.
+static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool bulkDraws) {
   const qt::System& s = *e->sys->h;
   if (n_int <= 0) throw Fail(QT_EINVAL, "no intervals");
   for (int k = 0; k < n_int; ++k) if (!(intervals[k] >= 0) || !std::isfinite(intervals[k])) throw Fail(QT_EINVAL, "intervals must be finite and non-negative");
@@ -443,63 +454,74 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
   const int no = std::max(1, (int)s.O.size());
   e->obs.alloc((size_t)e->n * n_int * no);
   e->dtd.alloc((size_t)e->n * n_int);
-  hipCheck(hipMemsetAsync(e->work.p, 0, sizeof(int), st), "reset work queue");
   qt::EnsDev E = e->view();
   E.intervals = e->intervals.p; E.n_intervals = n_int;
   E.obs_out = e->obs.p; E.dtdid_out = e->dtd.p;
   // jump draws for the launch, generated in bulk (one Philox per lane); trajectories that take
   // more steps than the budget evaluate Philox in the kernel
-  if (!e->sys->h->J.empty() && e->lay.glanes == 0) {
+  if (!s.J.empty() && bulkDraws) {
     double span = 0.; for (int k = 0; k < n_int; ++k) span += intervals[k];
     const int cap = (int)std::min(2048.0, std::max(64.0, std::ceil(span / std::max(e->dtMinSeen, 1e-6) * 1.25) + 8));
     e->draws.alloc((size_t)e->n * cap);
     hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st), "jump draws");
     E.draws = e->draws.p; E.draw_cap = cap;
   }
+  return E;
+}
+
+static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool timed) {
+  qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0);
+  hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
   if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
   if (e->lay.glanes > 0)
     hipCheck(qt::launch_sub(e->lay.glanes, e->lay.rows, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
              "mcwf sub-wave launch");
   else
-  hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV, e->grid,
-                            e->sys->dev, E, st),
-           "mcwf batch launch");
+    hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV, e->grid,
+                              e->sys->dev, E, st),
+             "mcwf batch launch");
   if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
   e->lastIntervals = n_int;
   for (int k = 0; k < n_int; ++k) e->time += intervals[k];
+}
+
+// Ensemble means of the last launch's stream outputs -> processed rows (time, dtDid, averages)
+// on the host; synchronises the stream.  Throws on a non-finite jump norm.
+static void ens_rows(qt_ens* e, double tStart, const double* intervals, int n_int, double* rows, hipStream_t st) {
+  const qt::System& s = *e->sys->h;
+  const int no = (int)s.O.size();
+  e->mean.alloc((size_t)n_int * std::max(1, no));
+  e->meanDt.alloc(n_int);
+  hipCheck(qt::launch_ensemble_mean(e->obs.p, e->dtd.p, e->n, n_int, no, e->mean.p, e->meanDt.p, st), "mean launch");
+  std::vector<double> m((size_t)n_int * std::max(1, no)), md(n_int);
+  hipCheck(hipMemcpyAsync(m.data(), e->mean.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+  hipCheck(hipMemcpyAsync(md.data(), e->meanDt.p, md.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+  int flag = 0;
+  hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (flag) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+  if (rows) {
+    double tt = tStart;
+    for (int k = 0; k < n_int; ++k) {
+      tt += intervals[k];
+      double* r = rows + (size_t)k * (2 + no);
+      r[0] = tt; r[1] = md[k];
+      for (int q = 0; q < no; ++q) r[2 + q] = m[(size_t)k * no + q];
+      process_rows(s, r + 2);
+    }
+  }
 }
 
 int qt_mcwf_advance(qt_ens* e, const double* intervals, int n_int, double* rows, qt_stream stream) {
   return guard([&] {
     if (!e || !intervals) throw Fail(QT_EINVAL, "null argument");
     hipStream_t st = (hipStream_t)stream;
-    const qt::System& s = *e->sys->h;
     const double tStart = e->time;
     ens_launch(e, intervals, n_int, st, true);
-    const int no = (int)s.O.size();
-    e->mean.alloc((size_t)n_int * std::max(1, no));
-    e->meanDt.alloc(n_int);
-    hipCheck(qt::launch_ensemble_mean(e->obs.p, e->dtd.p, e->n, n_int, no, e->mean.p, e->meanDt.p, st), "mean launch");
-    std::vector<double> m((size_t)n_int * std::max(1, no)), md(n_int);
-    hipCheck(hipMemcpyAsync(m.data(), e->mean.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-    hipCheck(hipMemcpyAsync(md.data(), e->meanDt.p, md.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-    int flag = 0;
-    hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
-    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    ens_rows(e, tStart, intervals, n_int, rows, st);
     float ms = 0.f;
     hipCheck(hipEventElapsedTime(&ms, e->e0, e->e1), "hipEventElapsedTime");
     e->kernelMs += ms; e->launches += 1;
-    if (flag) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
-    if (rows) {
-      double tt = tStart;
-      for (int k = 0; k < n_int; ++k) {
-        tt += intervals[k];
-        double* r = rows + (size_t)k * (2 + no);
-        r[0] = tt; r[1] = md[k];
-        for (int q = 0; q < no; ++q) r[2 + q] = m[(size_t)k * no + q];
-        process_rows(s, r + 2);
-      }
-    }
   });
 }
 
@@ -578,6 +600,253 @@ int qt_mcwf_kernel_ms(qt_ens* e, double* total, int* launches) {
   if (launches) *launches = e->launches;
   return QT_OK;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ mixed batch
+
+namespace {
+
+struct MixVariant { int nwb, rb, whb, rw, whw; };
+
+std::vector<MixVariant> mix_variants() {
+  int n = qt::mixed_variants(nullptr, 0);
+  std::vector<int> t(5 * n);
+  qt::mixed_variants(t.data(), n);
+  std::vector<MixVariant> v;
+  for (int i = 0; i < n; ++i) v.push_back({t[5 * i], t[5 * i + 1], t[5 * i + 2], t[5 * i + 3], t[5 * i + 4]});
+  return v;
+}
+
+// can member (dim, width) run as team `team` under variant V?  Register-cached ELL rows need
+// width <= the variant's width; uncached ones read any width from the table.
+bool team_fits(const MixVariant& V, int team, long dim, int width) {
+  if (team == 1) return dim <= 64L * V.rw && (V.rw * V.whw > 24 || width <= V.whw);
+  return dim <= 64L * V.nwb * V.rb && (V.rb * V.whb > 24 || width <= V.whb);
+}
+
+// default team: one wavefront per trajectory when the state fits a wavefront's rows
+bool assign_teams(const MixVariant& V, const std::vector<qt_ens*>& mem, const int* req, std::vector<int>& team) {
+  team.assign(mem.size(), 0);
+  for (size_t m = 0; m < mem.size(); ++m) {
+    const qt::System& s = *mem[m]->sys->h;
+    const int want = req ? req[m] : -1;
+    if (want == 0 || want == 1) {
+      if (!team_fits(V, want, s.dim, s.H.width)) return false;
+      team[m] = want;
+    } else if (team_fits(V, 1, s.dim, s.H.width)) team[m] = 1;
+    else if (team_fits(V, 0, s.dim, s.H.width)) team[m] = 0;
+    else return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct qt_batch_s {
+  std::vector<qt_ens*> mem;
+  int variant = 0;
+  std::vector<int> team;
+  bool f32 = false;
+  DevBuf<qt::MixDev> mix;
+  DevBuf<int> queue;
+  qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
+  int grid = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double kernelMs = 0.;
+  int launches = 0;
+  ~qt_batch_s() {
+    if (host) (void)hipHostFree(host);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  void choose(int v, const int* req) {
+    auto V = mix_variants();
+    if (v >= (int)V.size()) throw Fail(QT_EINVAL, "unknown batch layout variant");
+    if (v >= 0) {
+      if (!assign_teams(V[v], mem, req, team)) throw Fail(QT_ENOTSUP, "the requested layout does not fit every member");
+      variant = v;
+    } else {
+      int found = -1;
+      for (size_t i = 0; i < V.size() && found < 0; ++i) if (assign_teams(V[i], mem, req, team)) found = (int)i;
+      if (found < 0) throw Fail(QT_ENOTSUP, "no compiled mixed-batch layout fits every member");
+      variant = found;
+    }
+    const MixVariant& L = V[variant];
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    int per = qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, f32);
+    grid = prop.multiProcessorCount * std::max(1, per);
+  }
+};
+
+extern "C" {
+
+int qt_batch_variants(int* table, int cap) { return qt::mixed_variants(table, cap); }
+
+int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params* p, qt_batch** out) {
+  return guard([&] {
+    if (!members || !out || n_members <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    if (n_members > qt::kMaxMembers) throw Fail(QT_ENOTSUP, "at most 16 members per batch");
+    std::unique_ptr<qt_batch> b(new qt_batch);
+    for (int m = 0; m < n_members; ++m) {
+      if (!members[m]) throw Fail(QT_EINVAL, "null member");
+      b->mem.push_back(members[m]);
+    }
+    b->f32 = members[0]->p.precision == QT_FP32_DERIV;
+    for (auto* e : b->mem)
+      if ((e->p.precision == QT_FP32_DERIV) != b->f32) throw Fail(QT_EINVAL, "members must share one precision");
+    b->choose(p ? p->variant : -1, p ? p->team : nullptr);
+    b->mix.alloc(1);
+    b->queue.alloc(2);
+    hipCheck(hipHostMalloc((void**)&b->host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
+    hipCheck(hipEventCreate(&b->e0), "hipEventCreate");
+    hipCheck(hipEventCreate(&b->e1), "hipEventCreate");
+    *out = b.release();
+  });
+}
+
+int qt_batch_destroy(qt_batch* b) { delete b; return QT_OK; }
+
+int qt_batch_layout(const qt_batch* b, int* variant, int* team) {
+  if (!b) return QT_EINVAL;
+  if (variant) *variant = b->variant;
+  if (team) for (size_t m = 0; m < b->mem.size(); ++m) team[m] = b->team[m];
+  return QT_OK;
+}
+
+int qt_batch_kernel_ms(const qt_batch* b, double* total, int* launches) {
+  if (!b) return QT_EINVAL;
+  if (total) *total = b->kernelMs;
+  if (launches) *launches = b->launches;
+  return QT_OK;
+}
+
+}  // extern "C"
+
+// One mixed launch: prepares every member, uploads the descriptor, runs the kernel between
+// the batch's events.  Does not synchronise.
+static void batch_launch(qt_batch* b, const double* const* intervals, const int* n_int, hipStream_t st) {
+  const auto V = mix_variants();
+  const MixVariant& L = V[b->variant];
+  qt::MixDev& M = *b->host;
+  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");  // the staging buffer may still feed a copy
+  std::memset(&M, 0, sizeof(M));
+  M.n_members = (int)b->mem.size();
+  M.queue = b->queue.p;
+  int items[2] = {0, 0};
+  for (size_t m = 0; m < b->mem.size(); ++m) {
+    qt_ens* e = b->mem[m];
+    if (!intervals[m]) throw Fail(QT_EINVAL, "null interval list");
+    M.E[m] = ens_prepare(e, intervals[m], n_int[m], st, true);
+    M.S[m] = e->sys->dev;
+    M.team[m] = b->team[m];
+    M.first[m] = items[b->team[m]];
+    items[b->team[m]] += e->n;
+  }
+  M.block_items = items[0];
+  M.wave_items = items[1];
+  hipCheck(hipMemcpyAsync(b->mix.p, &M, sizeof(M), hipMemcpyHostToDevice, st), "batch descriptor H2D");
+  hipCheck(qt::launch_fill_int(b->queue.p, 2, 0, st), "reset queues");
+  const int groupsForWaves = (items[1] + L.nwb - 1) / L.nwb;
+  const int grid = std::max(1, std::min(b->grid, items[0] + groupsForWaves));
+  hipCheck(hipEventRecord(b->e0, st), "hipEventRecord");
+  hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, b->f32, grid, b->mix.p, st), "mixed launch");
+  hipCheck(hipEventRecord(b->e1, st), "hipEventRecord");
+  for (size_t m = 0; m < b->mem.size(); ++m) {
+    b->mem[m]->lastIntervals = n_int[m];
+  }
+}
+
+extern "C" {
+
+int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_int, double* const* rows,
+                     qt_stream stream) {
+  return guard([&] {
+    if (!b || !intervals || !n_int) throw Fail(QT_EINVAL, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> t0(b->mem.size());
+    for (size_t m = 0; m < b->mem.size(); ++m) t0[m] = b->mem[m]->time;
+    batch_launch(b, intervals, n_int, st);
+    for (size_t m = 0; m < b->mem.size(); ++m) {
+      qt_ens* e = b->mem[m];
+      for (int k = 0; k < n_int[m]; ++k) e->time += intervals[m][k];
+      ens_rows(e, t0[m], intervals[m], n_int[m], rows ? rows[m] : nullptr, st);
+    }
+    float ms = 0.f;
+    hipCheck(hipEventElapsedTime(&ms, b->e0, b->e1), "hipEventElapsedTime");
+    b->kernelMs += ms; b->launches += 1;
+  });
+}
+
+int qt_batch_autotune(qt_batch* b, double probe_span, int probe_traj, int* variant_out, double* ms_out, int ms_cap) {
+  return guard([&] {
+    if (!b || !(probe_span > 0) || probe_traj <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    long total = 0;
+    for (auto* e : b->mem) total += e->n;
+    const double f = std::min(1.0, (double)probe_traj / (double)total);
+    // probe members: fresh ensembles of the same systems, parameters and initial states
+    std::vector<std::unique_ptr<qt_ens, int (*)(qt_ens*)>> probes;
+    std::vector<qt_ens*> pm;
+    for (auto* e : b->mem) {
+      const long d = e->sys->h->dim;
+      std::vector<double2> psi0(d);
+      hipCheck(hipMemcpy(psi0.data(), e->psiInit.p, d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
+      qt_ens* pe = nullptr;
+      const int n = std::max(1, (int)std::lround(e->n * f));
+      int rc = qt_mcwf_create(e->sys, n, &e->p, reinterpret_cast<const double*>(psi0.data()), &pe);
+      if (rc) throw Fail(rc, g_err);
+      probes.emplace_back(pe, qt_mcwf_destroy);
+      pm.push_back(pe);
+    }
+    const auto V = mix_variants();
+    std::vector<int> teamSave = b->team;
+    const int variantSave = b->variant;
+    double best = 1e300;
+    int bestV = -1;
+    std::vector<int> bestTeam;
+    int nc = 0;
+    const double iv = probe_span;
+    std::vector<const double*> ivs(pm.size(), &iv);
+    std::vector<int> nis(pm.size(), 1);
+    for (size_t v = 0; v < V.size(); ++v) {
+      // candidate team assignments: by dimension, and everything on work groups
+      for (int mode = 0; mode < 2; ++mode) {
+        std::vector<int> req(pm.size(), mode == 0 ? -1 : 0);
+        std::vector<int> tm;
+        if (!assign_teams(V[v], pm, req.data(), tm)) continue;
+        if (mode == 1) {
+          std::vector<int> t0;
+          if (assign_teams(V[v], pm, nullptr, t0) && t0 == tm) continue;  // same as mode 0
+        }
+        qt_batch probe;
+        probe.mem = pm; probe.f32 = b->f32;
+        probe.choose((int)v, req.data());
+        probe.mix.alloc(1); probe.queue.alloc(2);
+        hipCheck(hipHostMalloc((void**)&probe.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
+        hipCheck(hipEventCreate(&probe.e0), "hipEventCreate");
+        hipCheck(hipEventCreate(&probe.e1), "hipEventCreate");
+        for (auto* e : pm) { int rc = qt_mcwf_reset(e, nullptr); if (rc) throw Fail(rc, g_err); }
+        batch_launch(&probe, ivs.data(), nis.data(), nullptr);
+        hipCheck(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        float ms = 0.f;
+        hipCheck(hipEventElapsedTime(&ms, probe.e0, probe.e1), "hipEventElapsedTime");
+        if (ms_out && nc < ms_cap) ms_out[nc] = ms;
+        ++nc;
+        if (ms < best) { best = ms; bestV = (int)v; bestTeam = probe.team; }
+      }
+    }
+    if (bestV < 0) { b->team = teamSave; b->variant = variantSave; throw Fail(QT_ENOTSUP, "no layout fits the members"); }
+    b->choose(bestV, bestTeam.data());
+    if (variant_out) *variant_out = bestV;
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_params* p, const double* rho0,
                      qt_master** out) {
@@ -670,7 +939,7 @@ int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* 
     qt::MasterDev M{};
     M.n_points = m->n; M.dim = m->d; M.hfrag = m->hfrag.p;
     M.nj = m->nj; M.wj = m->wj; M.jcol = m->jcol.p; M.jval = m->jval.p;
-    M.no = no; M.wo = m->sys0->dev.wo; M.ocol = m->sys0->dev.ocol; M.oval = m->sys0->dev.oval; M.oimag = m->sys0->dev.oimag;
+    M.no = no; M.wo = m->sys0->dev.wo; M.ocol = m->sys0->dev.ocol; M.oval = m->sys0->dev.oval; M.oimag = m->sys0->dev.oimag; M.owid = m->sys0->dev.owid;
     M.x = m->x.p; M.k = m->k.p; M.t = m->t.p; M.dtDid = m->dtDid.p; M.dtTry = m->dtTry.p;
     M.done = m->done.p; M.tss = m->tss.p; M.counters = m->cnt.p;
     M.epsAbs = m->p.epsRel; M.epsRel = m->p.epsAbs;  // ODE.h:205

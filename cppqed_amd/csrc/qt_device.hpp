@@ -46,6 +46,22 @@ __device__ __forceinline__ double readlane(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
+// Orders the LDS accesses of one wavefront around a hand-over between its lanes.  The LDS
+// executes one wave's instructions in issue order, so only the compiler has to be kept from
+// moving accesses across (wavefront-scope fences emit no hardware wait).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// barrier of a team of NW wavefronts: a whole work group for NW > 1, one wavefront otherwise
+template <int NW>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (NW == 1) wave_sync();
+  else __syncthreads();
+}
+
 // Sum / max over the 64 lanes of a wavefront; every lane gets the bitwise-identical result
 // (each butterfly step adds two equal-order partial sums, addition being commutative).
 // All 64 lanes must be active.

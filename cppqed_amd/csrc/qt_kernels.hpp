@@ -34,6 +34,7 @@ struct SysDev {
   const int* ocol;         // [no][wo][dim]
   const double2* oval;
   const int* oimag;        // [no]
+  const int* owid;         // [no]: entries of observable q per row (<= wo)
 };
 
 // Device view of an MCWF ensemble and the per-launch work description.
@@ -63,6 +64,22 @@ struct EnsDev {
   const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals ordinal[traj] + i (may be null)
   int draw_cap;
 };
+
+// Mixed batch: up to kMaxMembers ensembles of different systems advanced by one launch.
+constexpr int kMaxMembers = 16;
+
+struct alignas(16) MixDev {
+  int n_members;
+  int block_items;              // trajectories run one per work group (members with team 0)
+  int wave_items;               // trajectories run one per wavefront (members with team 1)
+  int pad_;
+  int* queue;                   // [2] heads of the work-group and wavefront queues
+  int team[kMaxMembers];        // 0: one work group per trajectory, 1: one wavefront per trajectory
+  int first[kMaxMembers];       // first item of member m in its queue
+  SysDev S[kMaxMembers];
+  EnsDev E[kMaxMembers];
+};
+static_assert(sizeof(MixDev) % 16 == 0, "MixDev is copied to LDS in 16-byte words");
 
 // One RKCK try of one trajectory (the plain form): x -> xn, err
 struct PlainTry {

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       double part = 0.;
       const bool im = M.oimag[q] != 0;
       for (int i = tid; i < d; i += 64 * kMasterWaves) {
-        for (int a = 0; a < M.wo; ++a) {
+        for (int a = 0; a < M.owid[q]; ++a) {
           const double2 o = M.oval[((size_t)q * M.wo + a) * d + i];
           const int cidx = M.ocol[((size_t)q * M.wo + a) * d + i];
           // Tr(rho O) = sum_i sum_a O_{i,c} rho_{c,i}

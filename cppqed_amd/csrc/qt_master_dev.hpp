@@ -15,6 +15,7 @@ struct MasterDev {
   int no, wo;            // observables (shared by all points)
   const int* ocol;       // [no][wo][dim]
   const double2* oval;
+  const int* owid;       // [no]
   const int* oimag;
   double2* x;            // [point][DP*DP] tile order
   double2* k;            // [point][7][DP*DP]: six stage derivatives and the 5th-order proposal

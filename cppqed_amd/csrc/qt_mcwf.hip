@@ -31,6 +31,7 @@ namespace qt {
 
 template <int ROWS, int NW, int WH, bool F32>
 struct Engine {
+  static constexpr int kRows = ROWS, kNW = NW;
   static constexpr int G = 64 * NW;
   // row coefficients live in registers when they fit; otherwise they are read from
   // the (L2-resident, shared by every trajectory) column-major ELL table
@@ -43,7 +44,7 @@ struct Engine {
   T* xs;            // LDS stage vector [ROWS*G] in derivative precision
   double2* ps;      // LDS FP64 state copy [ROWS*G] (rates, jumps, observables)
   double* red;      // LDS reduction scratch
-  int lane;
+  const int lane;
   int hcol[RR][RW];
   T hval[RR][RW];
 
@@ -52,8 +53,9 @@ struct Engine {
     else return S.hval[w * S.dim + row];
   }
 
-  __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_) : S(s), xs(xs_), ps(ps_), red(red_) {
-    lane = threadIdx.x;
+  // lane_ = index of this thread within its team of NW wavefronts
+  __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_)
+      : S(s), xs(xs_), ps(ps_), red(red_), lane(lane_) {
     if constexpr (kRegs) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
@@ -73,12 +75,16 @@ struct Engine {
   }
 
   __device__ __forceinline__ bool valid(int r) const { return r * G + lane < S.dim; }
+  // row block r holds part of the state for some lane of the team (uniform over the team)
+  __device__ __forceinline__ bool live(int r) const { return r == 0 || r * G < S.dim; }
+  static __device__ __forceinline__ void sync() { team_sync<NW>(); }
 
   // k = (H_nH/i) v, with v already staged in xs
   __device__ __forceinline__ void gather(T k[ROWS]) const {
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       T acc = from_d<T>(make_double2(0., 0.));
+      if (!live(r)) { k[r] = acc; continue; }
       if constexpr (kRegs) {
 #pragma unroll
         for (int w = 0; w < WH; ++w) acc = cfma(hval[r][w], xs[hcol[r][w]], acc);
@@ -93,18 +99,20 @@ struct Engine {
     }
   }
 
+  // xs and ps may share storage: both hand-overs start with a barrier, so the readers of the
+  // other buffer are done before it is overwritten
   __device__ __forceinline__ void stage(const double2 v[ROWS]) const {
-    __syncthreads();  // previous readers of xs are done
+    sync();  // previous readers of xs / ps are done
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) xs[r * G + lane] = from_d<T>(v[r]);
-    __syncthreads();
+    for (int r = 0; r < ROWS; ++r) if (live(r)) xs[r * G + lane] = from_d<T>(v[r]);
+    sync();
   }
 
   __device__ __forceinline__ void stagePsi(const double2 v[ROWS]) const {
-    __syncthreads();
+    sync();
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) ps[r * G + lane] = v[r];
-    __syncthreads();
+    for (int r = 0; r < ROWS; ++r) if (live(r)) ps[r * G + lane] = v[r];
+    sync();
   }
 
   // One RKCK54 try: returns the odeint error norm; xn is the 5th-order proposal.
@@ -115,25 +123,25 @@ struct Engine {
     double2 v[ROWS];
     const double s21 = CK::a21 * dt;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) v[r] = axpy(s21, to_d(k1[r]), x[r]);
+    for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, to_d(k1[r]), x[r]);
     stage(v); gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) v[r] = axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])); }
     stage(v); gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) v[r] = axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))); }
     stage(v); gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        v[r] = axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])))); }
+        if (live(r)) v[r] = axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])))); }
     stage(v); gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        v[r] = axpy(q5, to_d(k5[r]), axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))))); }
+        if (live(r)) v[r] = axpy(q5, to_d(k5[r]), axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))))); }
     stage(v); gather(k6);
     derivs += 5;
     double e = 0.;
@@ -142,6 +150,7 @@ struct Engine {
       const double f1 = CK::e1 * dt, f3 = CK::e3 * dt, f4 = CK::e4 * dt, f5 = CK::e5 * dt, f6 = CK::e6 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
+        if (!live(r)) { xn[r] = make_double2(0., 0.); continue; }
         const double2 d1 = to_d(k1[r]), d3 = to_d(k3[r]), d4 = to_d(k4[r]), d5 = to_d(k5[r]), d6 = to_d(k6[r]);
         xn[r] = axpy(q6, d6, axpy(q4, d4, axpy(q3, d3, axpy(q1, d1, x[r]))));
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
@@ -190,7 +199,8 @@ struct Engine {
       const int row = r * G + lane;
       double2 acc = make_double2(0., 0.);
       const int base = q * S.wo * S.dim + row;
-      for (int w = 0; w < S.wo; ++w) acc = cfma(S.oval[base + w * S.dim], ps[S.ocol[base + w * S.dim]], acc);
+      const int wq = S.owid[q];
+      for (int w = 0; w < wq; ++w) acc = cfma(S.oval[base + w * S.dim], ps[S.ocol[base + w * S.dim]], acc);
       // conj(x) * acc
       s += im ? (x[r].x * acc.y - x[r].y * acc.x) : (x[r].x * acc.x + x[r].y * acc.y);
     }
@@ -199,7 +209,171 @@ struct Engine {
 };
 
 // ---------------------------------------------------------------------------
-// batched persistent MCWF kernel
+// one trajectory through every stream interval of a launch
+// ---------------------------------------------------------------------------
+
+// Advances trajectory `traj` of ensemble E through E.intervals and writes its stream outputs.
+// Called by every lane of a team (eng.lane); pc (LDS, when PC_LDS) keeps psi at the start of the
+// MCWF step for a step-back, rts / cnt (LDS) hold the step's jump rates and the trajectory's
+// counters.
+template <class Eng, bool PC_LDS>
+__device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, const Eng& eng, int traj, double2* pc,
+                                             double* rts, long long* cnt) {
+  constexpr int ROWS = Eng::kRows, G = Eng::G, NW = Eng::kNW;
+  using T = typename Eng::T;
+  const int lane = eng.lane;
+  double2 pcr[PC_LDS ? 1 : ROWS];
+  auto bump = [&](int c, long long v) { if (lane == 0) cnt[c] += v; };
+
+  double2 x[ROWS], xn[ROWS];
+  T k1[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? E.psi[(size_t)traj * S.dim + r * G + lane] : make_double2(0., 0.);
+  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
+  unsigned long long ord = E.ordinal[traj];
+  const unsigned long long ord0 = ord;
+  const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
+  if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
+  Eng::sync();
+  const unsigned long long key = E.traj0 + (unsigned long long)traj;
+  const int nj = S.nj;
+  const double epsAbs = E.epsAbs, epsRel = E.epsRel;
+
+  // coherent part of one MCWF step (MCWF_Trajectory.h:250-276 with ODE.h:234-262)
+  auto coherent = [&](double Dt) {
+    const double timeBefore = t;
+    const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
+    if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
+    if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
+    eng.derivative(x, k1);
+    long long derivs = 1, tries = 0, failed = 0;
+    for (;;) {
+      ++tries;
+      const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs);
+      if (err > 1.0) {
+        // default_step_adjuster::decrease_step: dt *= max(0.9 err^(-1/(error_order-1)), 1/5)
+        dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
+        ++failed;
+        continue;
+      }
+      t += dtTry;
+      // increase_step: if err < 0.5, dt *= 0.9 max(5^-stepper_order, err)^(-1/stepper_order)
+      if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
+      break;
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) x[r] = xn[r];
+    dtDid = t - timeBefore;
+    if (nextDtTry != 0.) dtTry = nextDtTry;
+    if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
+    // renorm (StateVector.h:148-153)
+    double s2 = 0.;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(x[r]);
+    const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s2, eng.red));
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
+  };
+
+  auto computeRates = [&]() {
+    eng.stagePsi(x);
+    for (int m = 0; m < nj; ++m) {
+      double sacc = 0.;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+      const double v = GroupReduce<NW>::sum(sacc, eng.red);
+      if (lane == 0) rts[m] = v;
+    }
+    Eng::sync();
+  };
+
+  for (int k = 0; k < E.n_intervals; ++k) {
+    const double endTime = t + E.intervals[k];
+    for (;;) {
+      const double Dt = endTime - t;
+      if (Dt == 0.) break;
+      // MCWF_Trajectory::step (MCWF_Trajectory.h:347-374)
+      // the step's jump draw, fetched early so the load completes behind the RK work
+      const unsigned long long di = ord - ord0;
+      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if constexpr (PC_LDS) pc[r * G + lane] = x[r];
+        else pcr[r] = x[r];
+      }
+      const double tCache = t, dtDidCache = dtDid;
+      coherent(Dt);
+      if (nj > 0) {
+        computeRates();
+        for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
+          double totalRate = 0.;
+          for (int m = 0; m < nj; ++m) totalRate += rts[m];
+          const double sugg = E.dpLimit / totalRate;
+          if (totalRate * dtDid > E.overshootTol * E.dpLimit) {
+            t = tCache; dtDid = dtDidCache; dtTry = sugg;
+            bump(kStepBacks, 1);
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+              if constexpr (PC_LDS) x[r] = pc[r * G + lane];
+              else x[r] = pcr[r];
+            }
+            coherent(Dt);
+            computeRates();
+            continue;
+          } else if (totalRate * dtTry > E.dpLimit) bump(kOvershoots, 1);
+          dtTry = fmin(dtTry, sugg);
+          break;
+        }
+        // performJump (MCWF_Trajectory.h:320-344): one draw per step
+        const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);
+        ++ord;
+        double random = u / dtDid;
+        int no = 0;
+        for (; random > 0. && no != nj; random -= rts[no++]) ;
+        if (random < 0.) {
+          --no;
+          const double nf = sqrt(rts[no]);
+          if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
+          const double inv = 1.0 / nf;
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) {
+            const double2 y = eng.valid(r) ? eng.jumpRow(no, r) : make_double2(0., 0.);
+            x[r] = make_double2(y.x * inv, y.y * inv);
+          }
+          if (lane == 0) {
+            const long long nJ = cnt[kJumps];
+            if (nJ < E.jcap) {
+              E.jt[(size_t)traj * E.jcap + nJ] = t;
+              E.jch[(size_t)traj * E.jcap + nJ] = no;
+            }
+            cnt[kJumps] = nJ + 1;
+          }
+        }
+      }
+      bump(kMcwfSteps, 1);
+    }
+    // stream point: raw linear averages of the normalised state
+    eng.stagePsi(x);
+    for (int q = 0; q < S.no; ++q) {
+      const double v = eng.observable(q, x);
+      if (lane == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = v;
+    }
+    if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+  }
+
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) E.psi[(size_t)traj * S.dim + r * G + lane] = x[r];
+  Eng::sync();
+  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
+  if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
+}
+
+// psi at the start of the MCWF step goes to LDS when the stage vector and it fit
+template <int ROWS, int NW>
+constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2) <= 150 * 1024; }
+
+// ---------------------------------------------------------------------------
+// batched persistent MCWF kernel: one system, one team = one work group
 // ---------------------------------------------------------------------------
 
 template <int ROWS, int NW, int WH, bool F32, int MINW>
@@ -207,15 +381,13 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   using Eng = Engine<ROWS, NW, WH, F32>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
+  constexpr bool kPcLds = pc_in_lds<ROWS, NW>();
   // kernel arguments live in LDS: read where used instead of pinning ~80 scalar registers
   __shared__ SysDev S;
   __shared__ EnsDev E;
-  __shared__ T xs[ROWS * G];
+  // the RK stage vector (derivative precision) and the FP64 state copy share storage
   __shared__ double2 ps[ROWS * G];
-  // psi at the start of the MCWF step (step-back restore): LDS when it fits, else registers
-  static constexpr bool kPcLds = (size_t)ROWS * G * (sizeof(T) + 2 * sizeof(double2)) <= 150 * 1024;
   __shared__ double2 pc[kPcLds ? ROWS * G : 1];
-  double2 pcr[kPcLds ? 1 : ROWS];
   __shared__ double red[NW > 1 ? NW : 1];
   __shared__ double rts[kMaxJumps];        // jump rates of the current step
   __shared__ long long cnt[kCounters];     // this trajectory's counters (lane 0 updates)
@@ -224,8 +396,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   const int lane = threadIdx.x;
   if (lane == 0) { S = Sarg; E = Earg; }
   __syncthreads();
-  Eng eng(S, xs, ps, red);
-  auto bump = [&](int c, long long v) { if (lane == 0) cnt[c] += v; };
+  Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane);
 
   for (;;) {
     __syncthreads();
@@ -233,148 +404,83 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     __syncthreads();
     const int traj = shTraj;
     if (traj >= E.n_traj) return;
+    advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
+  }
+}
 
-    double2 x[ROWS], xn[ROWS];
-    T k1[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? E.psi[(size_t)traj * S.dim + r * G + lane] : make_double2(0., 0.);
-    double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
-    unsigned long long ord = E.ordinal[traj];
-    const unsigned long long ord0 = ord;
-    const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
-    if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
-    __syncthreads();
-    const unsigned long long key = E.traj0 + (unsigned long long)traj;
-    const int nj = S.nj;
-    const double epsAbs = E.epsAbs, epsRel = E.epsRel;
+// ---------------------------------------------------------------------------
+// mixed batch: trajectories of up to kMaxMembers ensembles (different systems) in one
+// persistent launch.  Members of team kind 0 run one trajectory per work group (NWB
+// wavefronts, RB rows per lane), members of kind 1 one trajectory per wavefront (RW rows per
+// lane).  Work groups first drain the group queue together, then every wavefront drains the
+// wave queue on its own (wave-level hand-overs only, no work-group barrier).
+// ---------------------------------------------------------------------------
 
-    // coherent part of one MCWF step (MCWF_Trajectory.h:250-276 with ODE.h:234-262)
-    auto coherent = [&](double Dt) {
-      const double timeBefore = t;
-      const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
-      if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
-      if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-      eng.derivative(x, k1);
-      long long derivs = 1, tries = 0, failed = 0;
-      for (;;) {
-        ++tries;
-        const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs);
-        if (err > 1.0) {
-          // default_step_adjuster::decrease_step: dt *= max(0.9 err^(-1/(error_order-1)), 1/5)
-          dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
-          ++failed;
-          continue;
-        }
-        t += dtTry;
-        // increase_step: if err < 0.5, dt *= 0.9 max(5^-stepper_order, err)^(-1/stepper_order)
-        if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
-        break;
-      }
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) x[r] = xn[r];
-      dtDid = t - timeBefore;
-      if (nextDtTry != 0.) dtTry = nextDtTry;
-      if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
-      // renorm (StateVector.h:148-153)
-      double s = 0.;
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s += cabs2(x[r]);
-      const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s, red));
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
-    };
+template <int NWB, int RB, int WHB, int RW, int WHW, bool F32>
+__global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
+  using EngB = Engine<RB, NWB, WHB, F32>;
+  using EngW = Engine<RW, 1, WHW, F32>;
+  using TB = typename EngB::T;
+  using TW = typename EngW::T;
+  constexpr int GB = 64 * NWB;
+  constexpr bool kPcB = pc_in_lds<RB, NWB>();
+  constexpr size_t kBlockBytes = (size_t)RB * GB * sizeof(double2) * (kPcB ? 2 : 1);
+  constexpr size_t kWaveEach = (size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8;
+  constexpr size_t kWaveBytes = kWaveEach * NWB;
+  constexpr size_t kBytes = kBlockBytes > kWaveBytes ? kBlockBytes : kWaveBytes;
+  __shared__ MixDev M;
+  __shared__ __align__(16) unsigned char pool[kBytes];
+  __shared__ double red[NWB];
+  __shared__ double rtsB[kMaxJumps];
+  __shared__ long long cntB[kCounters];
+  __shared__ int shItem;
 
-    auto computeRates = [&]() {
-      eng.stagePsi(x);
-      for (int m = 0; m < nj; ++m) {
-        double sacc = 0.;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
-        const double v = GroupReduce<NW>::sum(sacc, red);
-        if (lane == 0) rts[m] = v;
-      }
+  {  // copy the member descriptors to LDS (16-byte words)
+    const int4* src = reinterpret_cast<const int4*>(Mg);
+    int4* dst = reinterpret_cast<int4*>(&M);
+    for (int i = threadIdx.x; i < (int)(sizeof(MixDev) / sizeof(int4)); i += 64 * NWB) dst[i] = src[i];
+  }
+  __syncthreads();
+
+  // phase 1: one trajectory per work group
+  {
+    double2* ps = reinterpret_cast<double2*>(pool);
+    double2* pc = kPcB ? ps + RB * GB : nullptr;
+    for (;;) {
       __syncthreads();
-    };
-
-    for (int k = 0; k < E.n_intervals; ++k) {
-      const double endTime = t + E.intervals[k];
-      for (;;) {
-        const double Dt = endTime - t;
-        if (Dt == 0.) break;
-        // MCWF_Trajectory::step (MCWF_Trajectory.h:347-374)
-        // the step's jump draw, fetched early so the load completes behind the RK work
-        const unsigned long long di = ord - ord0;
-        const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-          if constexpr (kPcLds) pc[r * G + lane] = x[r];
-          else pcr[r] = x[r];
-        }
-        const double tCache = t, dtDidCache = dtDid;
-        coherent(Dt);
-        if (nj > 0) {
-          computeRates();
-          for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
-            double totalRate = 0.;
-            for (int m = 0; m < nj; ++m) totalRate += rts[m];
-            const double sugg = E.dpLimit / totalRate;
-            if (totalRate * dtDid > E.overshootTol * E.dpLimit) {
-              t = tCache; dtDid = dtDidCache; dtTry = sugg;
-              bump(kStepBacks, 1);
-#pragma unroll
-              for (int r = 0; r < ROWS; ++r) {
-                if constexpr (kPcLds) x[r] = pc[r * G + lane];
-                else x[r] = pcr[r];
-              }
-              coherent(Dt);
-              computeRates();
-              continue;
-            } else if (totalRate * dtTry > E.dpLimit) bump(kOvershoots, 1);
-            dtTry = fmin(dtTry, sugg);
-            break;
-          }
-          // performJump (MCWF_Trajectory.h:320-344): one draw per step
-          const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);
-          ++ord;
-          double random = u / dtDid;
-          int no = 0;
-          for (; random > 0. && no != nj; random -= rts[no++]) ;
-          if (random < 0.) {
-            --no;
-            const double nf = sqrt(rts[no]);
-            if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
-            const double inv = 1.0 / nf;
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-              const double2 y = eng.jumpRow(no, r);
-              x[r] = make_double2(y.x * inv, y.y * inv);
-            }
-            if (lane == 0) {
-              const long long nJ = cnt[kJumps];
-              if (nJ < E.jcap) {
-                E.jt[(size_t)traj * E.jcap + nJ] = t;
-                E.jch[(size_t)traj * E.jcap + nJ] = no;
-              }
-              cnt[kJumps] = nJ + 1;
-            }
-          }
-        }
-        bump(kMcwfSteps, 1);
-      }
-      // stream point: raw linear averages of the normalised state
-      eng.stagePsi(x);
-      for (int q = 0; q < S.no; ++q) {
-        const double v = eng.observable(q, x);
-        if (lane == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = v;
-      }
-      if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+      if (threadIdx.x == 0) shItem = atomicAdd(M.queue, 1);
+      __syncthreads();
+      const int item = shItem;
+      if (item >= M.block_items) break;
+      // members of one kind are laid out in member order in their queue
+      int m = -1;
+      for (int k = 0; k < M.n_members; ++k)
+        if (M.team[k] == 0 && M.first[k] <= item) m = k;
+      EngB eng(M.S[m], reinterpret_cast<TB*>(ps), ps, red, (int)threadIdx.x);
+      advance_traj<EngB, kPcB>(M.S[m], M.E[m], eng, item - M.first[m], pc, rtsB, cntB);
     }
+  }
 
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) E.psi[(size_t)traj * S.dim + r * G + lane] = x[r];
-    __syncthreads();
-    if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
-    if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
+  // phase 2: one trajectory per wavefront
+  {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned char* mine = pool + kWaveEach * w;
+    double2* ps = reinterpret_cast<double2*>(mine);
+    double2* pc = ps + RW * 64;
+    double* rts = reinterpret_cast<double*>(pc + RW * 64);
+    long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
+    for (;;) {
+      int item = 0;
+      if (lane == 0) item = atomicAdd(M.queue + 1, 1);
+      item = __builtin_amdgcn_readfirstlane(item);
+      if (item >= M.wave_items) break;
+      int m = -1;
+      for (int k = 0; k < M.n_members; ++k)
+        if (M.team[k] == 1 && M.first[k] <= item) m = k;
+      EngW eng(M.S[m], reinterpret_cast<TW*>(ps), ps, nullptr, lane);
+      advance_traj<EngW, true>(M.S[m], M.E[m], eng, item - M.first[m], pc, rts, cnt);
+      wave_sync();
+    }
   }
 }
 
@@ -430,7 +536,7 @@ __global__ void __launch_bounds__(64 * NW) plain_try_kernel(SysDev S, PlainTry P
   __shared__ double2 xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red);
+  Eng eng(S, xs, ps, red, (int)threadIdx.x);
   double2 x[ROWS], xn[ROWS], k1[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? P.x[r * G + threadIdx.x] : make_double2(0., 0.);
@@ -450,7 +556,7 @@ __global__ void __launch_bounds__(64 * NW) plain_post_kernel(SysDev S, double2* 
   __shared__ double2 xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red);
+  Eng eng(S, xs, ps, red, (int)threadIdx.x);
   double2 x[ROWS];
   double s = 0.;
 #pragma unroll
@@ -472,7 +578,7 @@ __global__ void __launch_bounds__(64 * NW) plain_jump_kernel(SysDev S, double2* 
   __shared__ double2 xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red);
+  Eng eng(S, xs, ps, red, (int)threadIdx.x);
   double2 x[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
@@ -489,7 +595,7 @@ __global__ void __launch_bounds__(64 * NW) plain_obs_kernel(SysDev S, const doub
   __shared__ double2 xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red);
+  Eng eng(S, xs, ps, red, (int)threadIdx.x);
   double2 x[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
@@ -506,7 +612,7 @@ __global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const dou
   __shared__ T xs[ROWS * G];
   __shared__ double2 ps[1];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red);
+  Eng eng(S, xs, ps, red, (int)threadIdx.x);
   for (int v = blockIdx.x; v < n; v += gridDim.x) {
     double2 x[ROWS];
     T k[ROWS];
@@ -587,6 +693,63 @@ hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, 
   QT_BATCH(QT_CASE)
 #undef QT_CASE
   return hipErrorInvalidValue;
+}
+
+// Mixed-batch variants: (waves per work group, rows per lane of the work-group team, its ELL
+// register width, rows per lane of the wavefront team, its ELL register width).
+#define QT_MIXED(X) \
+  X(16, 4, 16, 1, 8) X(16, 4, 16, 2, 8) X(16, 2, 16, 1, 8) X(16, 1, 16, 1, 8)
+
+bool mixed_supported(int nwb, int rb, int whb, int rw, int whw) {
+#define QT_CASE(A, B, C, D, F) if (nwb == A && rb == B && whb == C && rw == D && whw == F) return true;
+  QT_MIXED(QT_CASE)
+#undef QT_CASE
+  return false;
+}
+
+// compiled mixed variants as rows of 5 ints; returns their number
+int mixed_variants(int* table, int cap) {
+  int n = 0;
+#define QT_CASE(A, B, C, D, F) \
+  if (n < cap && table) { int* r = table + 5 * n; r[0] = A; r[1] = B; r[2] = C; r[3] = D; r[4] = F; } ++n;
+  QT_MIXED(QT_CASE)
+#undef QT_CASE
+  return n;
+}
+
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32) {
+  int n = 0;
+#define QT_CASE(A, B, C, D, F)                                                                                      \
+  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                     \
+    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, true>, 64 * A, 0); \
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, false>, 64 * A, 0);   \
+    return n;                                                                                                       \
+  }
+  QT_MIXED(QT_CASE)
+#undef QT_CASE
+  return 0;
+}
+
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st) {
+#define QT_CASE(A, B, C, D, F)                                                                                          \
+  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                         \
+    if (f32) hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, true>), dim3(grid), dim3(64 * A), 0, st, M);          \
+    else hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, false>), dim3(grid), dim3(64 * A), 0, st, M);             \
+    return hipGetLastError();                                                                                           \
+  }
+  QT_MIXED(QT_CASE)
+#undef QT_CASE
+  return hipErrorInvalidValue;
+}
+
+// p[0..n) = v  (a kernel rather than a memset node, so that the launch sequence can be captured)
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st) {
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, p, n, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,

@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(64) mcwf_sub_kernel(SysDev S, EnsDev E) {
           if (row >= dim) continue;
           double2 acc = make_double2(0., 0.);
           const int base = q * S.wo * dim + row;
-          for (int w = 0; w < S.wo; ++w) acc = cfma(S.oval[base + w * dim], pg[S.ocol[base + w * dim]], acc);
+          const int wq = S.owid[q];
+          for (int w = 0; w < wq; ++w) acc = cfma(S.oval[base + w * dim], pg[S.ocol[base + w * dim]], acc);
           s += im ? (x[r].x * acc.y - x[r].y * acc.x) : (x[r].x * acc.x + x[r].y * acc.y);
         }
         s = gsum<G>(s);

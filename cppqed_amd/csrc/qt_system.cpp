@@ -112,6 +112,7 @@ struct Builder {
     aim = are; aim.imagPart = true;
     freeObs.back() = {on, on2, are, aim};
     freeVar.back() = {-1, 0, -1, -1};  // VAR(n) = <n^2> - <n>^2  (Mode.cc:198-201)
+    freeSqrt.back() = {0, 0, 0, 0};
   }
   void avQbit(int ax) {
     Observable p0, p1, sre, sim;
@@ -122,6 +123,7 @@ struct Builder {
     sim = sre; sim.imagPart = true;
     freeObs.back() = {p0, p1, sre, sim};
     freeVar.back() = {-1, -1, -1, -1};
+    freeSqrt.back() = {0, 0, 0, 0};
   }
 
   // mode::make(ParsPumpedLossy, picture) for nTh = 0 (Mode.tcc:40-60, 160-197)
@@ -152,7 +154,7 @@ struct Builder {
     f.hasHa = true;  // every mode::Hamiltonian carries at least the (zero) Kerr term (Mode.cc:92-107)
     f.highestFrequency = hf;
     s.hTerms.push_back({{ax}, h});
-    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back(); freeSqrt.emplace_back();
     if (kappa) {  // sqrt(2 kappa) a  (Mode.cc:118-125)
       LocalOp j; j.dims = {dim};
       double fact = std::sqrt(2. * kappa);
@@ -190,7 +192,7 @@ struct Builder {
     f.hasHa = true;
     f.highestFrequency = hf;
     s.hTerms.push_back({{ax}, h});
-    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back(); freeSqrt.emplace_back();
     if (gamma) {
       LocalOp j; j.dims = {2};
       j.entries.push_back({0, 1, cd(std::sqrt(2. * gamma) * std::sqrt(1.), 0.), 0.});
@@ -215,7 +217,7 @@ struct Builder {
     h.entries.push_back({1, 0, -eta, 0.});
     h.entries.push_back({1, 1, za, 0.});
     s.hTerms.push_back({{ax}, h});
-    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back();
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back(); freeSqrt.emplace_back();
     LocalOp j; j.dims = {2};
     j.entries.push_back({0, 1, cd(std::sqrt(-2. * za.real()), 0.), 0.});
     freeJumps.back().push_back({{ax}, j});
@@ -263,6 +265,92 @@ struct Builder {
     iaHF = std::max(iaHF, std::fabs(J));
   }
 
+  // ParticleSch (Particle.cc:79-85, 247-252): H/i = -i omrec k^2 on the momentum grid
+  // k_n = n - 2^(fin-1) (particle::Spatial with deltaK = 1, Particle.cc:276-290); no jumps.
+  void particle(const KV& kv) {
+    int ax = (int)s.frees.size();
+    int fin = (int)num(kv, "fin", 6);
+    if (fin < 1 || fin > 12) throw std::invalid_argument("particle finesse out of range");
+    if (picture(kv) != "Sch") throw std::invalid_argument("the particle element is built in the Schroedinger picture");
+    double omrec = num(kv, "omrec", 1.);
+    int dim = 1 << fin;
+    std::vector<cd> d(dim);
+    for (int n = 0; n < dim; ++n) { double k = n - dim / 2; d[n] = cd(0., -omrec * k * k); }
+    FreeInfo f; f.kind = "particle"; f.dim = dim; f.avKind = "particle"; f.hasHa = true;
+    f.highestFrequency = std::fabs(omrec) * (double)dim * dim;  // RF{"omrec", omrec, sqr(1<<fin)}
+    s.hTerms.push_back({{ax}, diagonal(dim, d)});
+    freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back(); freeSqrt.emplace_back();
+    avParticle(ax, dim);
+    s.frees.push_back(f);
+  }
+
+  // particle::Averaged (Particle.cc:138-167): <P>, VAR(P), <X>, DEV(X).  The position
+  // populations are those of the k -> x transform of quantumdata::ffTransform
+  // (LazyDensityOperatorFFT.cc:16-40: GSL backward DFT, halves swapped, N^-1/2), so
+  // <f(X)> = <psi| F^dagger f(x) F |psi> with F[j][n] = N^-1/2 exp(2 pi i (j + N/2) n / N):
+  // a dense (circulant) operator in the momentum basis.
+  void avParticle(int ax, int dim) {
+    std::vector<cd> k(dim), k2(dim);
+    for (int n = 0; n < dim; ++n) { double kk = n - dim / 2; k[n] = kk; k2[n] = kk * kk; }
+    const double pi = 3.14159265358979323846, dx = 2. * pi / dim, xMax = pi;
+    LocalOp X, X2; X.dims = {dim}; X2.dims = {dim};
+    for (int n = 0; n < dim; ++n)
+      for (int m = 0; m < dim; ++m) {
+        cd a = 0., b = 0.;
+        for (int j = 0; j < dim; ++j) {
+          const double x = j * dx - xMax;
+          const long ph = ((long)(j + dim / 2) * (m - n)) % dim;
+          const cd e = std::polar(1.0 / dim, 2. * pi * (double)ph / dim);
+          a += x * e; b += x * x * e;
+        }
+        X.entries.push_back({n, m, a, 0.});
+        X2.entries.push_back({n, m, b, 0.});
+      }
+    Observable op, op2, ox, ox2;
+    op.terms.push_back({{ax}, diagonal(dim, k)});
+    op2.terms.push_back({{ax}, diagonal(dim, k2)});
+    ox.terms.push_back({{ax}, X});
+    ox2.terms.push_back({{ax}, X2});
+    freeObs.back() = {op, op2, ox, ox2};
+    freeVar.back() = {-1, 0, -1, 2};       // VAR(P) = <P^2> - <P>^2, DEV(X) = sqrt(<X^2> - <X>^2)
+    freeSqrt.back() = {0, 0, 0, 1};
+  }
+
+  const FreeInfo& particleAt(int i) const {
+    const FreeInfo& f = freeAt(i);
+    if (f.kind != "particle") throw std::invalid_argument("pcav needs a particle as its second free");
+    return f;
+  }
+
+  // ParticleAlongCavity with an unpumped particle (ParticleCavity.cc:26-32, 90-99): the dispersive
+  // term u0 n (x) (+-) cos(2 kcav x) / (2i); cos(K x) = (e^{iKx} + e^{-iKx}) / 2 shifts the momentum
+  // index by +-K (Particle.cc:60-80).  The constant u0 n / 2 of cos^2 belongs to the mode detuning.
+  void pcav(int m, int p, const KV& kv) {
+    const FreeInfo& fm = freeAt(m); const FreeInfo& fp = particleAt(p);
+    if (fm.kind != "mode") throw std::invalid_argument("pcav needs a mode as its first free");
+    double u0 = num(kv, "u0", 0);
+    int kcav = (int)num(kv, "kcav", 1);
+    auto mf = kv.find("mf");
+    std::string mfs = mf == kv.end() ? "cos" : mf->second;
+    if (mfs != "cos" && mfs != "sin") throw std::invalid_argument("pcav mode function must be cos or sin");
+    const double sgn = mfs == "sin" ? -1. : 1.;
+    const int K = 2 * kcav;
+    LocalOp t; t.dims = {fm.dim, fp.dim};
+    if (u0 != 0. && K < fp.dim) {
+      LocalOp n; n.dims = {fm.dim};
+      for (int a = 0; a < fm.dim; ++a) n.entries.push_back({a, a, (double)a, 0.});
+      LocalOp c; c.dims = {fp.dim};
+      for (int q = 0; q + K < fp.dim; ++q) {
+        c.entries.push_back({q + K, q, 0.5, 0.});
+        c.entries.push_back({q, q + K, 0.5, 0.});
+      }
+      t = scale(kron(n, c), cd(u0 * sgn, 0.) / cd(0., 2.));
+    }
+    s.hTerms.push_back({{m, p}, t});
+    iaHF = std::max(iaHF, std::fabs(u0) * fm.dim);  // RF{"Unot", uNot, mode dim}
+  }
+
+  std::vector<std::vector<char>> freeSqrt;
   double iaHF = 0.;
 };
 
@@ -296,6 +384,7 @@ System* buildFromSpec(const std::string& spec) {
       if (kind == "mode") b.mode(kv);
       else if (kind == "qbit") b.qbit(kv);
       else if (kind == "ptla") b.ptla(kv);
+      else if (kind == "particle") b.particle(kv);
       else throw std::invalid_argument("unknown free element " + kind);
     } else if (what == "ia") {
       if (axes.size() != 2) throw std::invalid_argument("interaction needs two axes");
@@ -303,6 +392,7 @@ System* buildFromSpec(const std::string& spec) {
       if (kind == "jc") b.jc(axes[0], axes[1], kv);
       else if (kind == "bs") b.bs(axes[0], axes[1], kv);
       else if (kind == "zz") b.zz(axes[0], axes[1], kv);
+      else if (kind == "pcav") b.pcav(axes[0], axes[1], kv);
       else throw std::invalid_argument("unknown interaction " + kind);
       ++nIa;
     } else if (what == "system") {
@@ -323,6 +413,7 @@ System* buildFromSpec(const std::string& spec) {
     for (size_t k = 0; k < b.freeObs[i].size(); ++k) {
       s.obs.push_back(b.freeObs[i][k]);
       s.processVarOf.push_back(b.freeVar[i][k] < 0 ? -1 : base + b.freeVar[i][k]);
+      s.processSqrt.push_back(b.freeSqrt[i][k]);
     }
   }
   // highest frequency (DynamicsBase.cc:49-66); binary systems include the interaction's (Q11)

@@ -57,7 +57,7 @@ struct FreeInfo {
   cd zI = 0.;
   double highestFrequency = 0.;
   int nJumps = 0;
-  std::string avKind;  // "mode" | "qbit" | ""
+  std::string avKind;  // "mode" | "qbit" | "particle" | ""
 };
 
 // ELL matrix over the full space
@@ -78,6 +78,7 @@ struct System {
   std::vector<std::vector<Term>> jumps;   // per channel
   std::vector<Observable> obs;            // raw linear averages, 4 per free with averages
   std::vector<int> processVarOf;          // for each obs: index of <n> to subtract squared, or -1
+  std::vector<char> processSqrt;          // for each obs: take the square root after the subtraction
   double highestFrequency = 0.;
   bool isBinary = false;
   bool hasEx = false;
@@ -94,9 +95,11 @@ struct System {
 //   free mode  cutoff=N delta=.. kappa=.. eta=.. [etaI=..] [picture=Sch|IP|UIP]
 //   free qbit  deltaA=.. gamma=.. eta=.. [etaI=..] [picture=..]
 //   free ptla  gamma=.. deltaA=.. eta=..                   (PumpedTwoLevelAtomSch)
+//   free particle fin=.. omrec=..                          (ParticleSch, momentum basis)
 //   ia jc i j g=.. [gI=..]                                 (JaynesCummings)
 //   ia bs i j g=..                                         (CoupledModes, RWA beam splitter)
 //   ia zz i j J=..                                         (qubit sigma_z sigma_z coupling)
+//   ia pcav m p u0=.. kcav=.. mf=cos|sin                   (ParticleAlongCavity, dispersive term)
 //   system binary|composite
 System* buildFromSpec(const std::string& spec);
 

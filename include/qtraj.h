@@ -119,6 +119,39 @@ int qt_mcwf_counters(qt_ens* ens, long long* per_traj, long long* totals);
 /* Average duration (ms) of the batched kernel launches so far, from HIP events on the launch stream. */
 int qt_mcwf_kernel_ms(qt_ens* ens, double* total_ms, int* launches);
 
+/* ---------------------------------------------------------------- mixed batch */
+
+/* A mixed batch advances several ensembles (each its own system, trajectory count, state and
+ * parameters) in ONE persistent launch: the reference runs each ensemble's trajectories
+ * serially (trajectory::Ensemble::advance, StochasticTrajectory.h:220) and separate ensembles
+ * one after the other.  Each member runs either one trajectory per work group (team 0) or one
+ * trajectory per wavefront (team 1); the layout variant sets the work-group width and the rows
+ * per lane of both teams.  All members share one precision. */
+typedef struct qt_batch_s qt_batch;
+
+typedef struct {
+  int variant;                 /* index into qt_batch_variants' table; -1: first that fits */
+  int team[16];                /* per member: 0 work group, 1 wavefront, -1 chosen by dimension */
+} qt_batch_params;
+
+/* Compiled layout variants, 5 ints each: waves per work group, rows per lane (team 0), ELL
+ * register width (team 0), rows per lane (team 1), ELL register width (team 1).  Returns the count. */
+int qt_batch_variants(int* table, int cap);
+/* members: up to 16 ensembles; p may be NULL (defaults) */
+int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params* p, qt_batch** out);
+int qt_batch_destroy(qt_batch* b);
+/* Chosen layout: *variant and team[n_members]. */
+int qt_batch_layout(const qt_batch* b, int* variant, int* team);
+/* Advances member m through intervals[m][0..n_intervals[m]) (each member as qt_mcwf_advance),
+ * all members in one launch; rows[m] (may be NULL) receives member m's ensemble rows. */
+int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_intervals, double* const* rows,
+                     qt_stream stream);
+/* Times every layout that fits the members on a probe (each member advanced by probe_span on a
+ * copy of at most probe_traj trajectories) and keeps the fastest; *variant_out and
+ * ms_out[n_candidates] (may be NULL) report the outcome.  Members' states are left unchanged. */
+int qt_batch_autotune(qt_batch* b, double probe_span, int probe_traj, int* variant_out, double* ms_out, int ms_cap);
+int qt_batch_kernel_ms(const qt_batch* b, double* total_ms, int* launches);
+
 /* ------------------------------------------------------- Master-equation sweep */
 
 typedef struct {

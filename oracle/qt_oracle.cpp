@@ -482,6 +482,45 @@ Free makePTLA(const std::map<std::string, std::string>& kv) {
   return f;
 }
 
+// particle::Spatial with deltaK = 1 (Particle.cc:276-290): k_n = n - N/2, x_j = j dx - pi, dx = 2 pi / N
+double particleK(int n, int N) { return (double)n - N / 2; }
+double particleX(int j, int N) { const double pi = 3.14159265358979323846; return j * (2. * pi / N) - pi; }
+
+// ParticleSch (Particle.cc:79-85, 247-252): Tridiagonal(mainDiagonal(space, -omrec)),
+// mainDiagonal = i omrec k^2 (Particle.cc:92-95); frequency RF{"omrec", omrec, sqr(1<<fin)}
+Free makeParticle(const std::map<std::string, std::string>& kv) {
+  Free f; f.kind = "particle"; f.avKind = "particle";
+  int fin = (int)num(kv, "fin", 6);
+  double omrec = num(kv, "omrec", 1.);
+  if (picture(kv) != Picture::Sch) throw std::invalid_argument("oracle: particle is Sch only");
+  f.dim = 1 << fin;
+  CV md(f.dim);
+  for (int n = 0; n < f.dim; ++n) md[n] = cd(0, 1) * (-omrec) * (particleK(n, f.dim) * particleK(n, f.dim));
+  f.hOverIs.push_back(unary(md, 0, CV(), CV()));
+  f.hasHa = true;
+  f.realFreqs.push_back({omrec, (double)f.dim * f.dim});
+  return f;
+}
+
+// ffTransformCV with DIR_KX (LazyDensityOperatorFFT.cc:16-40): GSL backward transform
+// y_m = sum_n psi_n exp(+2 pi i m n / N), halves swapped with factor N^-1/2, odd entries negated
+CV ffTransformKX(const CV& psi) {
+  const int N = (int)psi.size();
+  const double pi = 3.14159265358979323846;
+  CV y(N);
+  for (int m = 0; m < N; ++m) {
+    cd acc = 0.;
+    for (int n = 0; n < N; ++n) acc += psi[n] * std::polar(1.0, 2. * pi * (double)(((long)m * n) % N) / N);
+    y[m] = acc;
+  }
+  CV out(N);
+  const int h = N / 2;
+  const double nrm = std::pow((double)N, -.5);
+  for (int j = 0; j < h; ++j) { out[j] = nrm * y[j + h]; out[j + h] = nrm * y[j]; }
+  for (int j = 1; j < N; j += 2) out[j] *= -1.;
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // System (BinarySystem / Composite) built from a spec string
 // ---------------------------------------------------------------------------
@@ -521,10 +560,70 @@ Interaction makeJC(const System& s, int q, int m, const std::map<std::string, st
   return ia;
 }
 
+// CoupledModes<CM_XX_RWA> (CoupledModes.cc:36-41): tridiagMinusHC(conj(u) aop(m1) aop(m2)^dagger),
+// frequency CF{"u", u, dim1 sqrt(dim2)} (CoupledModes.cc:13-17)
+Interaction makeBS(const System& s, int m1, int m2, const std::map<std::string, std::string>& kv) {
+  Interaction ia; ia.kind = "bs"; ia.axes = {m1, m2};
+  cd u(num(kv, "g", 0), num(kv, "gI", 0));
+  const Free& f1 = s.frees[m1]; const Free& f2 = s.frees[m2];
+  Tridiag a1 = aop(f1.dim), a2 = aop(f2.dim);
+  if (f1.hasEx) furnishWithFreqs(a1, mainDiagonal(f1.zI, 0., f1.dim));
+  if (f2.hasEx) furnishWithFreqs(a2, mainDiagonal(f2.zI, 0., f2.dim));
+  ia.hOverIs.push_back(minusHC(product(scaled(a1, std::conj(u)), hc(a2))));
+  ia.hasHa = true; ia.timeDep = true;
+  ia.complexFreqs.push_back({u, f1.dim * std::sqrt((double)f2.dim)});
+  return ia;
+}
+
+// sigma_z (x) sigma_z with sigma_z = diag(-1, +1): H/i = -i J sz sz.  Not a reference element
+// (qbit::sigmazop is declared but never defined, Qbit_.h:68-70); frequency J.
+Interaction makeZZ(const System& s, int a, int b, const std::map<std::string, std::string>& kv) {
+  Interaction ia; ia.kind = "zz"; ia.axes = {a, b};
+  if (s.frees[a].dim != 2 || s.frees[b].dim != 2) throw std::invalid_argument("oracle: zz couples qubits");
+  double J = num(kv, "J", 0);
+  Tridiag sz = unary({-1., 1.}, 0, CV(), CV());
+  ia.hOverIs.push_back(scaled(product(sz, sz), cd(0., -J)));
+  ia.hasHa = true;
+  ia.realFreqs.push_back({J, 1.});
+  return ia;
+}
+
+// expINKX / cosNKX (Particle.cc:60-82): e^{iKx} = Tridiagonal(Diag(), K, ones) (minus diagonal),
+// e^{-iKx} = Tridiagonal(Diag(), K, Diag(), ones) (plus diagonal)
+Tridiag cosNKX(int dim, int K) {
+  int D = dim - K;
+  CV ones(std::max(D, 0), 1.);
+  Tridiag r = sum(unary(CV(), K, ones, CV()), unary(CV(), K, CV(), ones));
+  return scaled(r, 0.5);
+}
+
+// ParticleAlongCavity with an unpumped particle (ParticleCavity.cc:26-32, 47-53, 90-99):
+// dispersive u0 nop(mode) (+-) cosNKX(particle, 2 kcav) / (2i); etaeff = 0.
+// Frequencies RF{"Unot", u0, mode dim}, RF{"etaeff", 0, sqrt(mode dim)} (ParticleCavity.cc:62-66)
+Interaction makePcav(const System& s, int m, int p, const std::map<std::string, std::string>& kv) {
+  Interaction ia; ia.kind = "pcav"; ia.axes = {m, p};
+  const Free& fm = s.frees[m]; const Free& fp = s.frees[p];
+  if (fm.kind != "mode" || fp.kind != "particle") throw std::invalid_argument("oracle: pcav couples a mode and a particle");
+  double u0 = num(kv, "u0", 0);
+  int kcav = (int)num(kv, "kcav", 1);
+  auto it = kv.find("mf");
+  double sgn = (it != kv.end() && it->second == "sin") ? -1. : 1.;
+  if (u0 != 0. && 2 * kcav < fp.dim) {
+    CV nd(fm.dim); for (int n = 0; n < fm.dim; ++n) nd[n] = (double)n;   // nop = Tridiagonal(mainDiagonal(1,0,dim))
+    Tridiag t = product(scaled(unary(nd, 0, CV(), CV()), u0 * sgn), cosNKX(fp.dim, 2 * kcav));
+    ia.hOverIs.push_back(scaled(t, 1. / cd(0., 2.)));
+    ia.hasHa = true; ia.timeDep = true;
+  }
+  ia.realFreqs.push_back({u0, (double)fm.dim});
+  ia.realFreqs.push_back({0., std::sqrt((double)fm.dim)});
+  return ia;
+}
+
 System* buildSystem(const std::string& spec) {
   std::unique_ptr<System> s(new System);
   std::string line;
   std::vector<std::string> lines;
+  int forced = -1;
   { std::string cur; for (char c : spec) { if (c == ';' || c == '\n') { lines.push_back(cur); cur.clear(); } else cur += c; } lines.push_back(cur); }
   for (auto& ln : lines) {
     std::istringstream is(ln); std::string what, kind;
@@ -535,19 +634,23 @@ System* buildSystem(const std::string& spec) {
       if (kind == "mode") s->frees.push_back(makeMode(kv));
       else if (kind == "qbit") s->frees.push_back(makeQbit(kv));
       else if (kind == "ptla") s->frees.push_back(makePTLA(kv));
+      else if (kind == "particle") s->frees.push_back(makeParticle(kv));
       else throw std::invalid_argument("oracle: unknown free " + kind);
     } else if (what == "ia") {
       std::vector<int> axes; std::string tok; std::map<std::string, std::string> kv;
       while (is >> tok) { auto p = tok.find('='); if (p == std::string::npos) axes.push_back(std::stoi(tok)); else kv[tok.substr(0, p)] = tok.substr(p + 1); }
       if (kind == "jc") s->ias.push_back(makeJC(*s, axes.at(0), axes.at(1), kv));
+      else if (kind == "bs") s->ias.push_back(makeBS(*s, axes.at(0), axes.at(1), kv));
+      else if (kind == "zz") s->ias.push_back(makeZZ(*s, axes.at(0), axes.at(1), kv));
+      else if (kind == "pcav") s->ias.push_back(makePcav(*s, axes.at(0), axes.at(1), kv));
       else throw std::invalid_argument("oracle: unknown interaction " + kind);
     } else if (what == "system") {
-      s->isBinary = (kind == "binary");
+      forced = kind == "binary" ? 1 : 0;  // binary::make vs composite::make chosen explicitly
     } else throw std::invalid_argument("oracle: bad statement " + what);
   }
   std::vector<int> dims; for (auto& f : s->frees) dims.push_back(f.dim);
   s->L = Layout(dims);
-  if (s->frees.size() == 2 && s->ias.size() <= 1) s->isBinary = true;
+  s->isBinary = forced >= 0 ? forced == 1 : (s->frees.size() == 2 && s->ias.size() <= 1);
   return s.release();
 }
 
@@ -631,6 +734,15 @@ void averagesRaw(System& s, const cd* psi, double* out) {
           cd od = std::sqrt((double)n) * (el(n) * std::conj(el(n - 1)));
           a[2] += od.real(); a[3] += od.imag();
         }
+      } else if (f.avKind == "particle") {  // particle::Averaged::average_v, Particle.cc:138-160
+        CV sl(f.dim);
+        for (int n = 0; n < f.dim; ++n) {
+          sl[n] = el(n);
+          double dg = std::norm(sl[n]), k = particleK(n, f.dim);
+          a[0] += k * dg; a[1] += k * k * dg;
+        }
+        CV px = ffTransformKX(sl);
+        for (int j = 0; j < f.dim; ++j) { double dg = std::norm(px[j]), x = particleX(j, f.dim); a[2] += x * dg; a[3] += x * x * dg; }
       } else {
         cd r10 = el(1) * std::conj(el(0));
         a[0] += std::norm(el(0)); a[1] += std::norm(el(1)); a[2] += r10.real(); a[3] += r10.imag();
@@ -657,6 +769,19 @@ void averagesRawRho(System& s, const cd* rho, double* out) {
           cd od = std::sqrt((double)n) * el(n, n - 1);
           a[2] += od.real(); a[3] += od.imag();
         }
+      } else if (f.avKind == "particle") {
+        // x populations of F rho F^dagger, F the transform of ffTransformKX (the density-operator
+        // ffTransform of LazyDensityOperatorFFT.cc:53-58 transforms rows and columns)
+        const int N = f.dim;
+        for (int n = 0; n < N; ++n) { double dg = el(n, n).real(), k = particleK(n, N); a[0] += k * dg; a[1] += k * k * dg; }
+        std::vector<CV> F(N);  // F[n] = transform of the unit vector |n>: column n of F
+        for (int n = 0; n < N; ++n) { CV u(N, 0.); u[n] = 1.; F[n] = ffTransformKX(u); }
+        for (int j = 0; j < N; ++j) {
+          cd acc = 0.;
+          for (int n = 0; n < N; ++n) for (int m = 0; m < N; ++m) acc += F[n][j] * el(n, m) * std::conj(F[m][j]);
+          const double x = particleX(j, N);
+          a[2] += x * acc.real(); a[3] += x * x * acc.real();
+        }
       } else {
         a[0] += el(0, 0).real(); a[1] += el(1, 1).real(); a[2] += el(1, 0).real(); a[3] += el(1, 0).imag();
       }
@@ -668,7 +793,12 @@ void averagesRawRho(System& s, const cd* rho, double* out) {
 
 void processAverages(System& s, double* av) {  // Mode.cc:198-201
   int k = 0;
-  for (auto& f : s.frees) { if (f.avKind.empty()) continue; if (f.avKind == "mode") av[k + 1] -= av[k] * av[k]; k += 4; }
+  for (auto& f : s.frees) {
+    if (f.avKind.empty()) continue;
+    if (f.avKind == "mode") av[k + 1] -= av[k] * av[k];
+    if (f.avKind == "particle") { av[k + 1] -= av[k] * av[k]; av[k + 3] = std::sqrt(av[k + 3] - av[k + 2] * av[k + 2]); }  // Particle.cc:163-167
+    k += 4;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -16,6 +16,50 @@ JC60 = ("free qbit deltaA=0.5 gamma=0.2 eta=0\n"
         "ia jc 0 1 g=3")
 
 
+# Mixed batch of composite systems (all FP32 derivatives).  Parameters the workload leaves open
+# (losses, pumps, detunings, initial states) are fixed here.
+# (a) mode cutoff 16 (x) particle with 64 momentum states, cos^2 coupling U0 = -1: the dispersive
+#     term U0 n cos(2kx)/2 of ParticleAlongCavity plus U0 n / 2 folded into the mode detuning
+#     (deltaC = -1 -> -1 - U0/2, as CPPQEDscripts/1particle1mode.cc does), recoil frequency 0.1
+PCAV1024 = ("free mode cutoff=16 delta=-0.5 kappa=1 eta=1\n"
+            "free particle fin=6 omrec=0.1\n"
+            "ia pcav 0 1 u0=-1 kcav=1 mf=cos")
+# (b) four modes, cutoff 8 each (dim 4096), RWA beam splitters g = 0.5 between neighbours,
+#     kappa = 1 and deltaC = -1 on every mode, the first one pumped with eta = 1
+MODES4096 = "\n".join([f"free mode cutoff=8 delta=-1 kappa=1 eta={1 if i == 0 else 0}" for i in range(4)]
+                      + [f"ia bs {i} {i + 1} g=0.5" for i in range(3)] + ["system composite"])
+
+
+def qubits64_spec(seed=20240611):
+    """(c) six qubits (dim 64): ZZ couplings J_ij ~ U(-1, 1) on all 15 pairs (numpy PCG64 seed),
+    transverse drive eta = 0.7 and decay gamma = 0.2 on every qubit, deltaA = 0."""
+    rng = np.random.default_rng(seed)
+    lines = ["free qbit deltaA=0 gamma=0.2 eta=0.7"] * 6
+    for i in range(6):
+        for j in range(i + 1, 6):
+            lines.append(f"ia zz {i} {j} J={rng.uniform(-1, 1)!r}")
+    return "\n".join(lines + ["system composite"])
+
+
+QUBITS64 = qubits64_spec()
+
+# the four builds with their initial basis states (flat index): vacuum (x) particle at k = 0, vacuum,
+# all qubits in the ground state, ground (x) vacuum
+MIXED = [("pcav1024", PCAV1024, 32), ("modes4096", MODES4096, 0), ("qubits64", QUBITS64, 0), ("jc60", JC60, 0)]
+
+
+def mixed_counts(n_total=8192, seed=7):
+    """Trajectories per build: each of n_total drawn uniformly from the four builds."""
+    rng = np.random.default_rng(seed)
+    return np.bincount(rng.integers(0, len(MIXED), n_total), minlength=len(MIXED))
+
+
+def basis(dim, index):
+    psi = np.zeros(dim, np.complex128)
+    psi[index] = 1
+    return psi
+
+
 def jc_master_spec(cutoff=40, eta=1.0, g=10.0):
     return (f"free qbit deltaA=0 gamma=0 eta=0\n"
             f"free mode cutoff={cutoff} delta=0 kappa=1 eta={eta!r}\n"

@@ -1,0 +1,84 @@
+"""GPU parity of the mixed batch: ensembles of four different systems (dims 1024, 4096, 64, 60)
+advanced by one persistent launch, against the CPU oracle (FP64) and against each member run
+alone (FP32 derivatives)."""
+import numpy as np
+import pytest
+
+import cppqed_amd as Q
+import oracle_lib as O
+from sysdefs import MIXED, basis
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20240611
+
+
+def _members(counts, prec, jcap=256):
+    out = []
+    for i, ((name, spec, idx), n) in enumerate(zip(MIXED, counts)):
+        s = Q.System(spec)
+        out.append(Q.Ensemble(s, int(n), basis(s.dim, idx), Q.mcwf_pars(seed=SEED + i, precision=prec, jump_log_cap=jcap)))
+    return out
+
+
+def test_mixed_fp64_trajectories_match_oracle():
+    counts = [5, 3, 12, 12]
+    print(f"[gpu] mixed batch fp64 parity counts={counts}", flush=True)
+    iv = np.full(10, 0.1)
+    mem = _members(counts, Q.QT_FP64)
+    b = Q.Batch(mem)
+    v, team = b.layout()
+    assert team == [0, 0, 1, 1], team
+    rows = b.advance(iv)
+    for i, ((name, spec, idx), e) in enumerate(zip(MIXED, mem)):
+        so = O.System(spec)
+        psi0 = basis(so.dim, idx)
+        per, _ = e.counters()
+        for k in range(e.n):
+            res = O.mcwf_run(so, psi0, O.params(rng="philox", seed=SEED + i, stream=k), O.run_spec("NDt", Dt=0.1, NDt=10))
+            psi_g, _ = e.state(k)
+            jt, jch, nj = e.jumplog(k)
+            assert nj == len(res["jump_t"]), (name, k)
+            np.testing.assert_array_equal(jch, res["jump_ch"])
+            np.testing.assert_allclose(jt, res["jump_t"], rtol=1e-10, atol=1e-12)
+            err = np.linalg.norm(psi_g - res["psi"]) / np.linalg.norm(res["psi"])
+            assert err <= 1e-9, (name, k, err)
+            for c in (0, 1, 2, 4, 5, 7):
+                assert per[k][c] == res["counters"][c], (name, k, c)
+        ro, _ = O.ensemble_run(so, psi0, O.params(rng="philox", seed=SEED + i, stream=0), O.run_spec("NDt", Dt=0.1, NDt=10), e.n)
+        np.testing.assert_allclose(rows[i], ro[1:], rtol=1e-9, atol=1e-9, err_msg=name)
+
+
+def test_mixed_equals_members_alone():
+    """Each member's rows from the mixed launch equal the member run alone with the same engine."""
+    counts = [64, 32, 256, 256]
+    print(f"[gpu] mixed batch vs alone counts={counts}", flush=True)
+    iv = np.full(10, 0.1)
+    mem = _members(counts, Q.QT_FP32_DERIV)
+    rows = Q.Batch(mem).advance(iv)
+    layouts = [(4, 16), (4, 16), (1, 1), (1, 1)]
+    for i, ((name, spec, idx), n) in enumerate(zip(MIXED, counts)):
+        s = Q.System(spec)
+        alone = Q.Ensemble(s, n, basis(s.dim, idx), Q.mcwf_pars(seed=SEED + i, precision=Q.QT_FP32_DERIV, layout=layouts[i]))
+        r = alone.advance(iv)
+        diff = np.abs(r - rows[i]).max()
+        print(f"  {name}: max |alone - mixed| = {diff:.3g}", flush=True)
+        np.testing.assert_allclose(rows[i], r, rtol=1e-13, atol=1e-14, err_msg=name)
+        _, t1 = alone.counters()
+        _, t2 = mem[i].counters()
+        np.testing.assert_array_equal(t1, t2)
+
+
+def test_mixed_autotune_keeps_results():
+    counts = [16, 8, 64, 64]
+    print(f"[gpu] mixed batch autotune counts={counts}", flush=True)
+    iv = np.full(5, 0.1)
+    ref = Q.Batch(_members(counts, Q.QT_FP64)).advance(iv)
+    mem = _members(counts, Q.QT_FP64)
+    b = Q.Batch(mem)
+    v, ms = b.autotune(probe_span=0.2, probe_traj=256)
+    print(f"  autotune picked variant {v} of {Q.batch_variants()}, probe ms {ms}", flush=True)
+    assert 0 <= v < len(Q.batch_variants()) and len(ms) >= 1
+    rows = b.advance(iv)
+    for a, r in zip(rows, ref):
+        np.testing.assert_allclose(a, r, rtol=1e-9, atol=1e-12)

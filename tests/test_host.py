@@ -20,7 +20,27 @@ SPECS = {
     "jc_ip_flavour_sch_qbit": "free qbit deltaA=-2 gamma=0.5 eta=0.3\nfree mode cutoff=5 delta=1 kappa=0.2 eta=0.7\nia jc 0 1 g=0.4 gI=0.2",
     "lossy_mode": "free mode cutoff=7 delta=-2 kappa=0.5 eta=0",
     "pumped_mode": "free mode cutoff=7 delta=-2 kappa=0 eta=1.5",
+    "pcav_small": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1\nfree particle fin=3 omrec=0.3\nia pcav 0 1 u0=-1 kcav=1 mf=cos",
+    "pcav_sin": "free mode cutoff=3 delta=1 kappa=0.5 eta=0.5\nfree particle fin=4 omrec=1\nia pcav 0 1 u0=2 kcav=1 mf=sin",
+    "bs_chain": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0\n"
+                "free mode cutoff=2 delta=0 kappa=1 eta=0\nia bs 0 1 g=0.5\nia bs 1 2 g=0.25 gI=0.5\nsystem composite",
+    "zz_triangle": "free qbit deltaA=0 gamma=0.2 eta=0.7\nfree qbit deltaA=0.3 gamma=0.1 eta=0.7\n"
+                   "free qbit deltaA=0 gamma=0.2 eta=0\nia zz 0 1 J=-0.5\nia zz 1 2 J=0.75\nia zz 0 2 J=0.25\nsystem composite",
 }
+
+
+def process(spec, raw):
+    """Averaged::process of the frees, in spec order: VAR of n (Mode.cc:198-201), VAR(P) and
+    DEV(X) of the particle (Particle.cc:163-167)."""
+    out = raw.copy()
+    frees = [l.split()[1] for l in spec.splitlines() if l.startswith("free")]
+    for f, kind in enumerate(frees):
+        b = 4 * f
+        if kind in ("mode", "particle"):
+            out[b + 1] -= out[b] ** 2
+        if kind == "particle":
+            out[b + 3] = np.sqrt(out[b + 3] - out[b + 2] ** 2)
+    return out
 
 
 @pytest.mark.parametrize("name", sorted(SPECS))
@@ -36,13 +56,45 @@ def test_tables_match_oracle(name):
     for m in range(sq.n_jumps):
         np.testing.assert_allclose(sq.dense(1 + m) @ psi, so.act_jump(psi, m), rtol=0, atol=1e-15)
     raw = np.array([np.vdot(psi, sq.dense(1 + sq.n_jumps + q) @ psi) for q in range(sq.n_avr)])
-    # observable q streams Re or Im of <psi|O_q|psi>; every 4th block is (a, a, Re, Im) per free
-    vals = np.array([raw[q].imag if q % 4 == 3 else raw[q].real for q in range(sq.n_avr)])
-    proc = vals.copy()
-    for f in range(sq.n_avr // 4):
-        if "mode" in spec.splitlines()[f]:
-            proc[4 * f + 1] -= proc[4 * f] ** 2
-    np.testing.assert_allclose(proc, so.averages(psi), rtol=1e-12, atol=1e-12)
+    # observable q streams Re or Im of <psi|O_q|psi>: the 4th of a mode's or qubit's block is Im<a>
+    frees = [l.split()[1] for l in spec.splitlines() if l.startswith("free")]
+    vals = np.array([raw[q].imag if q % 4 == 3 and frees[q // 4] != "particle" else raw[q].real
+                     for q in range(sq.n_avr)])
+    np.testing.assert_allclose(process(spec, vals), so.averages(psi), rtol=1e-12, atol=1e-12)
+
+
+def test_particle_position_operator_is_the_fft_population():
+    """<X> of the dense momentum-basis operator equals sum_j x_j |psi_x(j)|^2 with psi_x from numpy's
+    inverse FFT (the GSL backward transform up to 1/N), halves swapped (LazyDensityOperatorFFT.cc:16-40);
+    a particle with fin=3 has x_j = j*pi/4 - pi."""
+    spec = "free particle fin=3 omrec=0.3"
+    sq = Q.System(spec)
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=8) + 1j * rng.normal(size=8)
+    psi /= np.linalg.norm(psi)
+    px = np.fft.ifft(psi) * np.sqrt(8)
+    px = np.concatenate([px[4:], px[:4]])
+    x = np.arange(8) * np.pi / 4 - np.pi
+    pop = np.abs(px) ** 2
+    X, X2 = sq.dense(3), sq.dense(4)
+    assert np.vdot(psi, X @ psi).real == pytest.approx((x * pop).sum(), abs=1e-13)
+    assert np.vdot(psi, X2 @ psi).real == pytest.approx((x * x * pop).sum(), abs=1e-13)
+    np.testing.assert_allclose(X, X.conj().T, atol=1e-15)
+    # kinetic term: H/i = -i omrec k^2, k = n - 4
+    np.testing.assert_allclose(np.diag(sq.dense(0)), -1j * 0.3 * (np.arange(8) - 4.0) ** 2, atol=0)
+
+
+def test_mixed_builds_lower():
+    """The four builds of the mixed batch lower to the expected dimensions, widths and channels."""
+    from sysdefs import MIXED, mixed_counts
+    want = {"pcav1024": (1024, 1, 5), "modes4096": (4096, 4, 9), "qubits64": (64, 6, 7), "jc60": (60, 2, 4)}
+    for name, spec, idx in MIXED:
+        s = Q.System(spec)
+        d, nj, w = want[name]
+        assert (s.dim, s.n_jumps, s.info.h_width) == (d, nj, w), name
+        assert 0 <= idx < s.dim
+    c = mixed_counts()
+    assert c.sum() == 8192 and c.min() > 1900
 
 
 def test_jc_ell_width_and_nnz():
