@@ -46,6 +46,16 @@ class BatchParams(C.Structure):
     _fields_ = [("variant", C.c_int), ("team", C.c_int * 16)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk_intervals", C.c_int),
+                ("mcwf", McwfParams)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("finished", C.c_longlong), ("tries", C.c_longlong),
+                ("kernel_ms", C.c_double), ("active", C.c_int), ("waiting", C.c_int), ("bytes_in_use", C.c_size_t)]
+
+
 def lib():
     """Loads libqtraj.so; raises if it has not been built."""
     global _lib
@@ -78,6 +88,14 @@ def lib():
         l.qt_batch_advance.argtypes = [vp, vp, vp, vp, vp]
         l.qt_batch_autotune.argtypes = [vp, dp, ip, C.POINTER(C.c_int), vp, ip]
         l.qt_batch_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        l.qt_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(vp)]
+        l.qt_server_destroy.argtypes = [vp]
+        l.qt_server_submit.argtypes = [vp, vp, ip, vp, dp, dp, C.c_ulonglong, C.POINTER(C.c_longlong)]
+        l.qt_server_step.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        l.qt_server_result.argtypes = [vp, C.c_longlong, vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        l.qt_server_release.argtypes = [vp, C.c_longlong]
+        l.qt_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         l.qt_master_create.argtypes = [vp, ip, C.POINTER(MasterParams), vp, C.POINTER(vp)]
         l.qt_master_destroy.argtypes = [vp]
         l.qt_master_advance.argtypes = [vp, vp, ip, vp, vp, vp]
@@ -382,6 +400,57 @@ class Batch:
         n = C.c_int()
         check(lib().qt_batch_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Server:
+    """Continuous-batching request loop over the mixed batch (qt_server_*): MCWF ensemble requests
+    of different systems are admitted by member slots and device memory, advanced together one
+    chunk of stream intervals per launch, and retired when done."""
+
+    def __init__(self, hbm_budget=8 << 30, max_active=16, chunk_intervals=10, pars=None):
+        self.p = ServerParams(int(hbm_budget), max_active, chunk_intervals, pars or mcwf_pars())
+        h = C.c_void_p()
+        check(lib().qt_server_create(C.byref(self.p), C.byref(h)))
+        self.h = h
+        self._systems = {}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.qt_server_destroy(self.h)
+            self.h = None
+
+    def submit(self, system, n_traj, psi0, T, Dt=0.1, seed=1001):
+        psi0, p = _cbuf(psi0)
+        rid = C.c_longlong()
+        check(lib().qt_server_submit(self.h, system.h, int(n_traj), p, float(T), float(Dt), int(seed), C.byref(rid)))
+        self._systems[rid.value] = system
+        return rid.value
+
+    def step(self):
+        a, w = C.c_int(), C.c_int()
+        check(lib().qt_server_step(self.h, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def result(self, rid):
+        """(rows, latency_s, queue_wait_s, tries) of a finished request, or None if not finished."""
+        system = self._systems[rid]
+        n = C.c_int()
+        lat, wait, tries = C.c_double(), C.c_double(), C.c_longlong()
+        if lib().qt_server_result(self.h, rid, None, 0, C.byref(n), C.byref(lat), C.byref(wait), C.byref(tries)):
+            return None
+        rows = np.zeros((n.value, 2 + system.n_avr))
+        check(lib().qt_server_result(self.h, rid, rows.ctypes.data, n.value, C.byref(n), C.byref(lat), C.byref(wait),
+                                     C.byref(tries)))
+        return rows, lat.value, wait.value, tries.value
+
+    def release(self, rid):
+        check(lib().qt_server_release(self.h, rid))
+        self._systems.pop(rid, None)
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().qt_server_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
 
 COUNTER_NAMES = ("mcwf_steps", "ode_steps", "failed", "derivs", "tries", "step_backs", "overshoots", "jumps")

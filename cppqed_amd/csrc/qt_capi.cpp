@@ -5,7 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <deque>
+#include <map>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -652,6 +655,7 @@ struct qt_batch_s {
   DevBuf<int> queue;
   qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
   int grid = 0;
+  int cus = 0;                  // compute units of the device (cached)
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double kernelMs = 0.;
   int launches = 0;
@@ -673,12 +677,15 @@ struct qt_batch_s {
       variant = found;
     }
     const MixVariant& L = V[variant];
-    int dev = 0;
-    hipCheck(hipGetDevice(&dev), "hipGetDevice");
-    hipDeviceProp_t prop;
-    hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (!cus) {
+      int dev = 0;
+      hipCheck(hipGetDevice(&dev), "hipGetDevice");
+      hipDeviceProp_t prop;
+      hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+      cus = prop.multiProcessorCount;
+    }
     int per = qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, f32);
-    grid = prop.multiProcessorCount * std::max(1, per);
+    grid = cus * std::max(1, per);
   }
 };
 
@@ -1106,6 +1113,225 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     if (psi_out) hipCheck(hipMemcpy(psi_out, x.p, d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
     if (counters) std::memcpy(counters, c, sizeof(c));
   });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving loop
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// stream intervals of trajectory::run in Dt mode (Trajectory.h:560-595): advance(min(Dt, T - t))
+std::vector<double> dt_intervals(double T, double Dt) {
+  std::vector<double> v;
+  double t = 0.;
+  while (t < T) { const double d = std::min(Dt, T - t); v.push_back(d); t = t + d; }
+  return v;
+}
+
+struct Request {
+  long long id = 0;
+  qt_sys* sys = nullptr;
+  int n = 0;
+  std::vector<double> psi0;
+  std::vector<double> iv;
+  size_t next = 0;
+  qt_mcwf_params p{};
+  qt_ens* ens = nullptr;
+  size_t bytes = 0;
+  std::vector<double> rows;
+  double tSubmit = 0., tAdmit = 0., tDone = 0.;
+  long long tries = 0;
+  ~Request() { if (ens) qt_mcwf_destroy(ens); }
+};
+
+// device bytes a request holds while admitted (qt_ens buffers for one chunk of `chunk` intervals)
+size_t request_bytes(const qt::System& s, int n, int chunk, int jcap, double span) {
+  const size_t d = s.dim, no = std::max<size_t>(1, s.O.size());
+  const size_t cap = (size_t)std::min(2048.0, std::max(64.0, std::ceil(span / 0.02 * 1.25) + 8));
+  size_t b = 2 * (size_t)n * d * 16;                  // psi, psiInit
+  b += (size_t)n * (5 * 8 + qt::kCounters * 8);        // t, dtDid, dtTry, dtTryInit, ordinal, counters
+  b += (size_t)n * jcap * 12;                          // jump log
+  b += (size_t)n * chunk * (no + 1) * 8;               // stream outputs
+  b += s.J.empty() ? 0 : (size_t)n * cap * 8;          // jump draws
+  return b;
+}
+
+}  // namespace
+
+struct qt_server_s {
+  qt_server_params P{};
+  long long nextId = 1;
+  std::deque<Request*> waiting;
+  std::vector<Request*> active;
+  std::map<long long, std::unique_ptr<Request>> all;
+  size_t used = 0;
+  qt_batch_s batch;
+  long long steps = 0, tries = 0, finished = 0;
+  double kernelMs = 0.;
+};
+
+extern "C" {
+
+int qt_server_create(const qt_server_params* p, qt_server** out) {
+  return guard([&] {
+    if (!p || !out) throw Fail(QT_EINVAL, "null argument");
+    if (p->max_active < 1 || p->max_active > qt::kMaxMembers) throw Fail(QT_EINVAL, "max_active must be 1..16");
+    if (p->chunk_intervals < 1) throw Fail(QT_EINVAL, "chunk_intervals must be positive");
+    if (p->hbm_budget == 0) throw Fail(QT_EINVAL, "zero memory budget");
+    check_params(&p->mcwf);
+    std::unique_ptr<qt_server> s(new qt_server);
+    s->P = *p;
+    s->batch.f32 = p->mcwf.precision == QT_FP32_DERIV;
+    s->batch.mix.alloc(1);
+    s->batch.queue.alloc(2);
+    hipCheck(hipHostMalloc((void**)&s->batch.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
+    hipCheck(hipEventCreate(&s->batch.e0), "hipEventCreate");
+    hipCheck(hipEventCreate(&s->batch.e1), "hipEventCreate");
+    *out = s.release();
+  });
+}
+
+int qt_server_destroy(qt_server* s) { delete s; return QT_OK; }
+
+int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0, double T, double Dt,
+                     unsigned long long seed, long long* id) {
+  return guard([&] {
+    if (!s || !sys || !psi0 || n_traj <= 0 || !(T > 0) || !(Dt > 0)) throw Fail(QT_EINVAL, "bad arguments");
+    const qt::System& h = *sys->h;
+    if (h.H.timeDependent() || h.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    std::unique_ptr<Request> r(new Request);
+    r->id = s->nextId++;
+    r->sys = sys; r->n = n_traj;
+    r->psi0.assign(psi0, psi0 + 2 * h.dim);
+    r->iv = dt_intervals(T, Dt);
+    r->p = s->P.mcwf;
+    r->p.seed = seed; r->p.traj0 = 0;
+    r->bytes = request_bytes(h, n_traj, s->P.chunk_intervals, std::max(1, r->p.jump_log_cap), Dt * s->P.chunk_intervals);
+    if (r->bytes > s->P.hbm_budget) throw Fail(QT_ENOMEM, "request larger than the server's memory budget");
+    // every compiled mixed layout must be able to take the system
+    {
+      auto V = mix_variants();
+      bool ok = false;
+      for (auto& v : V) ok = ok || team_fits(v, 1, h.dim, h.H.width) || team_fits(v, 0, h.dim, h.H.width);
+      if (!ok) throw Fail(QT_ENOTSUP, "system does not fit the mixed-batch layouts");
+    }
+    r->tSubmit = now_s();
+    if (id) *id = r->id;
+    s->waiting.push_back(r.get());
+    s->all[r->id] = std::move(r);
+  });
+}
+
+// One continuous-batching iteration: admit waiting requests in arrival order while slots and
+// the memory budget allow (a request that does not fit lets later, smaller ones pass), advance
+// every admitted request by its next chunk of stream intervals in ONE mixed launch, append the
+// streamed rows, retire finished requests and free their memory.
+int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
+  return guard([&] {
+    if (!s) throw Fail(QT_EINVAL, "null argument");
+    for (auto it = s->waiting.begin(); it != s->waiting.end();) {
+      Request* r = *it;
+      if ((int)s->active.size() >= s->P.max_active) break;
+      if (s->used + r->bytes > s->P.hbm_budget) { ++it; continue; }
+      int rc = qt_mcwf_create(r->sys, r->n, &r->p, r->psi0.data(), &r->ens);
+      if (rc) throw Fail(rc, g_err);
+      r->tAdmit = now_s();
+      s->used += r->bytes;
+      s->active.push_back(r);
+      it = s->waiting.erase(it);
+    }
+    if (!s->active.empty()) {
+      const size_t na = s->active.size();
+      std::vector<const double*> ivp(na);
+      std::vector<int> ni(na);
+      std::vector<double> t0(na);
+      for (size_t m = 0; m < na; ++m) {
+        Request* r = s->active[m];
+        ivp[m] = r->iv.data() + r->next;
+        ni[m] = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
+        t0[m] = r->ens->time;
+      }
+      s->batch.mem.clear();
+      for (auto* r : s->active) s->batch.mem.push_back(r->ens);
+      s->batch.choose(-1, nullptr);
+      batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
+      for (size_t m = 0; m < na; ++m) {
+        Request* r = s->active[m];
+        const int no = (int)r->sys->h->O.size();
+        std::vector<double> rows((size_t)ni[m] * (2 + no));
+        for (int k = 0; k < ni[m]; ++k) r->ens->time += ivp[m][k];
+        ens_rows(r->ens, t0[m], ivp[m], ni[m], rows.data(), nullptr);
+        r->rows.insert(r->rows.end(), rows.begin(), rows.end());
+        r->next += ni[m];
+      }
+      float ms = 0.f;
+      hipCheck(hipEventElapsedTime(&ms, s->batch.e0, s->batch.e1), "hipEventElapsedTime");
+      s->kernelMs += ms;
+      ++s->steps;
+      // retire
+      std::vector<Request*> keep;
+      for (auto* r : s->active) {
+        if (r->next < r->iv.size()) { keep.push_back(r); continue; }
+        long long tot[qt::kCounters];
+        int rc = qt_mcwf_counters(r->ens, nullptr, tot);
+        if (rc) throw Fail(rc, g_err);
+        r->tries = tot[qt::kTries];
+        s->tries += r->tries;
+        qt_mcwf_destroy(r->ens);
+        r->ens = nullptr;
+        s->used -= r->bytes;
+        r->tDone = now_s();
+        ++s->finished;
+      }
+      s->active.swap(keep);
+    }
+    if (n_active) *n_active = (int)s->active.size();
+    if (n_waiting) *n_waiting = (int)s->waiting.size();
+  });
+}
+
+int qt_server_result(qt_server* s, long long id, double* rows, int cap_rows, int* n_rows, double* latency,
+                     double* queue_wait, long long* tries) {
+  return guard([&] {
+    if (!s) throw Fail(QT_EINVAL, "null argument");
+    auto it = s->all.find(id);
+    if (it == s->all.end()) throw Fail(QT_EINVAL, "unknown request id");
+    Request* r = it->second.get();
+    if (r->tDone == 0.) throw Fail(QT_EINVAL, "request not finished");
+    const int w = 2 + (int)r->sys->h->O.size();
+    const int n = (int)(r->rows.size() / w);
+    if (n_rows) *n_rows = n;
+    if (rows) std::memcpy(rows, r->rows.data(), (size_t)std::min(n, cap_rows) * w * sizeof(double));
+    if (latency) *latency = r->tDone - r->tSubmit;
+    if (queue_wait) *queue_wait = r->tAdmit - r->tSubmit;
+    if (tries) *tries = r->tries;
+  });
+}
+
+int qt_server_release(qt_server* s, long long id) {
+  return guard([&] {
+    auto it = s->all.find(id);
+    if (it == s->all.end()) throw Fail(QT_EINVAL, "unknown request id");
+    if (it->second->tDone == 0.) throw Fail(QT_EINVAL, "request not finished");
+    s->all.erase(it);
+  });
+}
+
+int qt_server_stats(const qt_server* s, qt_server_stats_t* st) {
+  if (!s || !st) return QT_EINVAL;
+  st->steps = s->steps;
+  st->finished = s->finished;
+  st->tries = s->tries;
+  st->kernel_ms = s->kernelMs;
+  st->active = (int)s->active.size();
+  st->waiting = (int)s->waiting.size();
+  st->bytes_in_use = s->used;
+  return QT_OK;
 }
 
 }  // extern "C"

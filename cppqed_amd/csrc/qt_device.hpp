@@ -29,6 +29,14 @@ __device__ __forceinline__ double2 to_d(float2 a) { return make_double2((double)
 template <typename T> __device__ __forceinline__ T from_d(double2 a);
 template <> __device__ __forceinline__ double2 from_d<double2>(double2 a) { return a; }
 template <> __device__ __forceinline__ float2 from_d<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+// FP64 copy of a stage derivative made at its point of use.  For FP32 derivatives the empty asm
+// hides the operand from common-subexpression elimination, so the FP32 values stay live across
+// the RK stages instead of hoisted FP64 copies (half the registers).
+__device__ __forceinline__ double2 to_d_use(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d_use(float2 a) {
+  asm volatile("" : "+v"(a.x), "+v"(a.y));
+  return make_double2((double)a.x, (double)a.y);
+}
 __device__ __forceinline__ double2 axpy(double s, double2 k, double2 acc) {  // acc + s*k
   acc.x = fma(s, k.x, acc.x); acc.y = fma(s, k.y, acc.y); return acc;
 }

@@ -123,25 +123,25 @@ struct Engine {
     double2 v[ROWS];
     const double s21 = CK::a21 * dt;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, to_d(k1[r]), x[r]);
+    for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, to_d_use(k1[r]), x[r]);
     stage(v); gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r])); }
     stage(v); gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r]))); }
     stage(v); gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        if (live(r)) v[r] = axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r])))); }
+        if (live(r)) v[r] = axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r])))); }
     stage(v); gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        if (live(r)) v[r] = axpy(q5, to_d(k5[r]), axpy(q4, to_d(k4[r]), axpy(q3, to_d(k3[r]), axpy(q2, to_d(k2[r]), axpy(q1, to_d(k1[r]), x[r]))))); }
+        if (live(r)) v[r] = axpy(q5, to_d_use(k5[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r]))))); }
     stage(v); gather(k6);
     derivs += 5;
     double e = 0.;
@@ -151,7 +151,7 @@ struct Engine {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         if (!live(r)) { xn[r] = make_double2(0., 0.); continue; }
-        const double2 d1 = to_d(k1[r]), d3 = to_d(k3[r]), d4 = to_d(k4[r]), d5 = to_d(k5[r]), d6 = to_d(k6[r]);
+        const double2 d1 = to_d_use(k1[r]), d3 = to_d_use(k3[r]), d4 = to_d_use(k4[r]), d5 = to_d_use(k5[r]), d6 = to_d_use(k6[r]);
         xn[r] = axpy(q6, d6, axpy(q4, d4, axpy(q3, d3, axpy(q1, d1, x[r]))));
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
         xe = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, xe))));

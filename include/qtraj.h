@@ -152,6 +152,48 @@ int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_i
 int qt_batch_autotune(qt_batch* b, double probe_span, int probe_traj, int* variant_out, double* ms_out, int ms_cap);
 int qt_batch_kernel_ms(const qt_batch* b, double* total_ms, int* launches);
 
+/* ---------------------------------------------------------------- serving loop */
+
+/* A request loop over the mixed batch (continuous batching): requests -- MCWF ensembles of
+ * different systems -- are submitted at any time; each qt_server_step admits waiting requests
+ * (arrival order, smaller ones may pass one that does not fit) while member slots and the
+ * device-memory budget allow, advances every admitted request by its next chunk of stream
+ * intervals in ONE launch, and retires finished requests, freeing their memory.  Request state
+ * stays resident on the device between steps.  The reference runs one trajectory driver at a
+ * time (cppqedutils::run, Trajectory.h:462-610); it has no counterpart of this loop. */
+typedef struct qt_server_s qt_server;
+
+typedef struct {
+  size_t hbm_budget;       /* device bytes the admitted requests may hold */
+  int max_active;          /* requests advanced per launch, 1..16 */
+  int chunk_intervals;     /* stream intervals each request advances per step */
+  qt_mcwf_params mcwf;     /* ODE / MCWF parameters of every request (seed set per request) */
+} qt_server_params;
+
+typedef struct {
+  long long steps, finished, tries;
+  double kernel_ms;
+  int active, waiting;
+  size_t bytes_in_use;
+} qt_server_stats_t;
+
+int qt_server_create(const qt_server_params* p, qt_server** out);
+int qt_server_destroy(qt_server* s);
+/* Queues an ensemble of n_traj trajectories of sys from psi0_host, run to T with stream
+ * interval Dt (trajectory::run Dt mode) and RNG key (seed, trajectory index).  sys must outlive
+ * the request.  Returns the request id. */
+int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0_host, double T, double Dt,
+                     unsigned long long seed, long long* id);
+/* One iteration of the loop (admit, one launch, retire); reports active and waiting counts. */
+int qt_server_step(qt_server* s, int* n_active, int* n_waiting);
+/* Rows (time, mean dtDid, processed averages) of a finished request, its latency (submit to
+ * retirement, seconds), its wait before admission and its RKCK tries. */
+int qt_server_result(qt_server* s, long long id, double* rows, int cap_rows, int* n_rows, double* latency,
+                     double* queue_wait, long long* tries);
+/* Forgets a finished request. */
+int qt_server_release(qt_server* s, long long id);
+int qt_server_stats(const qt_server* s, qt_server_stats_t* stats);
+
 /* ------------------------------------------------------- Master-equation sweep */
 
 typedef struct {

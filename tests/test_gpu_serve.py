@@ -1,0 +1,88 @@
+"""GPU tests of the serving loop: a request's result is the same alone and in a mix of
+requests arriving while others run; admission respects the memory budget; results match the
+oracle."""
+import numpy as np
+import pytest
+
+import cppqed_amd as Q
+import oracle_lib as O
+from sysdefs import JC60, MIXED, PLM20, basis, coherent, ground
+
+pytestmark = pytest.mark.gpu
+
+
+def _drain(srv, rids, max_steps=10000):
+    for _ in range(max_steps):
+        a, w = srv.step()
+        if a == 0 and w == 0:
+            break
+    return {r: srv.result(r) for r in rids}
+
+
+@pytest.mark.parametrize("prec", [Q.QT_FP64, Q.QT_FP32_DERIV])
+def test_request_same_alone_and_in_mix(prec):
+    print(f"[gpu] serve alone vs mix precision={prec}", flush=True)
+    pars = Q.mcwf_pars(precision=prec)
+    jc = Q.System(JC60)
+    others = [(Q.System(spec), idx) for _, spec, idx in MIXED]
+    plm = Q.System(PLM20)
+    # alone
+    srv = Q.Server(chunk_intervals=5, pars=pars)
+    rid = srv.submit(jc, 256, ground(60), T=2.0, seed=11)
+    alone = _drain(srv, [rid])[rid][0]
+    # in a mix: requests arrive before, with and after it, between steps
+    srv = Q.Server(chunk_intervals=5, pars=pars)
+    ids = [srv.submit(plm, 512, coherent(0.5, 20), T=1.5, seed=3)]
+    ids.append(srv.submit(others[1][0], 48, basis(4096, others[1][1]), T=1.0, seed=4))
+    srv.step()
+    rid = srv.submit(jc, 256, ground(60), T=2.0, seed=11)
+    srv.step()
+    ids.append(srv.submit(others[0][0], 128, basis(1024, others[0][1]), T=1.2, seed=5))
+    ids.append(srv.submit(others[2][0], 300, basis(64, others[2][1]), T=0.7, seed=6))
+    res = _drain(srv, ids + [rid])
+    mixed = res[rid][0]
+    assert mixed.shape == alone.shape == (20, 2 + jc.n_avr)
+    np.testing.assert_array_equal(mixed, alone)
+    # and the same as the ensemble driven directly
+    ens = Q.Ensemble(jc, 256, ground(60), Q.mcwf_pars(precision=prec, seed=11))
+    direct = np.concatenate([ens.advance(np.full(5, 0.1)) for _ in range(4)])
+    np.testing.assert_allclose(mixed, direct, rtol=1e-13, atol=1e-15)
+    for r in ids:
+        assert res[r] is not None and res[r][1] > 0
+
+
+def test_admission_by_memory():
+    print("[gpu] serve admission by memory", flush=True)
+    jc = Q.System(JC60)
+    probe = Q.Server(chunk_intervals=4)
+    probe.submit(jc, 2048, ground(60), T=0.8, seed=0)
+    probe.step()
+    one = probe.stats()["bytes_in_use"]
+    assert one > 2048 * 60 * 16
+    budget = int(2.5 * one)            # two such requests fit at a time
+    srv = Q.Server(hbm_budget=budget, chunk_intervals=4)
+    ids = [srv.submit(jc, 2048, ground(60), T=0.8, seed=s) for s in range(5)]
+    peak = 0
+    for _ in range(1000):
+        a, w = srv.step()
+        assert srv.stats()["bytes_in_use"] <= budget
+        peak = max(peak, a)
+        if a == 0 and w == 0:
+            break
+    assert peak == 2, peak
+    waits = sorted(srv.result(r)[2] for r in ids)
+    assert waits[-1] > waits[0]
+    with pytest.raises(Q.QtError):
+        srv.submit(jc, 1 << 16, ground(60), T=1.0)   # larger than the whole budget
+
+
+def test_served_results_match_oracle():
+    print("[gpu] serve fp64 vs oracle", flush=True)
+    srv = Q.Server(chunk_intervals=3, pars=Q.mcwf_pars(precision=Q.QT_FP64))
+    cases = [(PLM20, coherent(0.5, 20), 40, 1.0, 21), (JC60, ground(60), 24, 0.9, 22)]
+    ids = [srv.submit(Q.System(spec), n, psi, T=T, seed=seed) for spec, psi, n, T, seed in cases]
+    res = _drain(srv, ids)
+    for (spec, psi, n, T, seed), r in zip(cases, ids):
+        ro, _ = O.ensemble_run(O.System(spec), psi, O.params(rng="philox", seed=seed, stream=0),
+                               O.run_spec("Dt", Dt=0.1, T=T), n)
+        np.testing.assert_allclose(res[r][0], ro[1:], rtol=1e-9, atol=1e-9)
