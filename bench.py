@@ -25,13 +25,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-# algorithmic work per RKCK try for JC-60 (Schroedinger picture):
-# M = 354 complex multiply-adds per H_nH/i application (stored diagonals of the mode (176),
-# qubit (120) and JC (58) terms over their slices), F = 6*8*M + 124*d
-JC_M, JC_D = 354, 60
-F_DERIV = 6 * 8 * JC_M          # 16992 flop per try in the derivative evaluations
-F_CTRL = 124 * JC_D             # 7440 flop per try of RK algebra + error norm (FP64)
+# Algorithmic work per RKCK try: F = 6*8*M + 124*d flop, M = complex multiply-adds of one H_nH/i
+# application counted the reference's way (stored entries of every term times its slices, the
+# all-zero Kerr diagonal excluded; SysInfo.m_terms).  JC-60: mode 88*2 + qubit 2*30 + JC 58 = 294,
+# so 14112 flop in the derivative evaluations (FP32 in the FP32-derivative run) + 7440 FP64.
 P_FP32, P_FP64 = 157.3e12, 78.6e12   # MI355X vector peaks (FLOP/s)
+
+
+def try_flops(system):
+    """(derivative flops, control flops) of one RKCK try of `system`."""
+    return 6 * 8 * system.info.m_terms, 124 * system.dim
+
+
+def valu_roofline(f32_flops, f64_flops, seconds):
+    """Achieved vs the time-weighted FP32/FP64 vector peak for this flop mix (TFLOP/s)."""
+    achieved = (f32_flops + f64_flops) / seconds / 1e12
+    peak = (f32_flops + f64_flops) / (f32_flops / P_FP32 + f64_flops / P_FP64) / 1e12
+    return achieved, peak
 
 
 def main():
@@ -48,6 +58,12 @@ def main():
     ap.add_argument("--no-master", action="store_true")
     ap.add_argument("--master-points", type=int, default=512)
     ap.add_argument("--master-T", type=int, default=60)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-T", type=float, default=5.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -96,11 +112,8 @@ def main():
                      sample=f"{args.plain_traj} trajectories to T={args.T}")
 
     # roofline of the batched FP32-derivative kernel: algorithmic flops / kernel time
-    f32_flops = F_DERIV * b32["tries"]
-    f64_flops = F_CTRL * b32["tries"]
-    achieved = (f32_flops + f64_flops) / b32["kernel_s"] / 1e12
-    t_min = f32_flops / P_FP32 + f64_flops / P_FP64
-    peak = (f32_flops + f64_flops) / t_min / 1e12
+    fd, fc = try_flops(sysq)
+    achieved, peak = valu_roofline(fd * b32["tries"], fc * b32["tries"], b32["kernel_s"])
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "mcwf_batch_jc60_16k_traffic.json")
@@ -153,6 +166,9 @@ def main():
                                "note": "8 d^3 flop of dense complex (H_nH/i) rho per derivative, 6 per try; "
                                        f"{f_try / 1e6:.1f} MFLOP per try in all, sparse-equivalent 1.82 MFLOP"}}
 
+    mixed = None if args.no_mixed else mixed_batch(Q, args)
+    serving = None if args.no_serve else serve(Q, args)
+
     out = {
         "metric": "MCWF RK steps/s (JC dim 60, 16k traj) batched vs plain; p99 latency",
         "value": value,
@@ -171,7 +187,8 @@ def main():
                    "n_traj": args.ntraj, "T": args.T, "dim": 60, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "note": "F = 6*8*354 FP32 + 124*60 FP64 flop per RK try; peak = mix of 157.3 FP32 / 78.6 FP64 vector"},
+                     "note": f"F = 6*8*{sysq.info.m_terms} FP32 + 124*60 FP64 flop per RK try; "
+                             "peak = time-weighted mix of 157.3 FP32 / 78.6 FP64 vector"},
         "batched_fp64": {"value": v64, "seconds": b64["seconds"], "tries": b64["tries"]},
         "plain_fp64": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
@@ -179,8 +196,110 @@ def main():
         "kernel": {"launches": b32["launches"], "avg_ms": 1e3 * b32["kernel_s"] / max(1, b32["launches"])},
         "cpu_baseline": cpu,
         "master_sweep": master,
+        "mixed_batch": mixed,
+        "serving": serving,
     }
     print(json.dumps(out))
+
+
+def mixed_batch(Q, args):
+    """Uneven batch: 8192 trajectories drawn uniformly from four builds (dims 1024, 4096, 64, 60),
+    FP32 derivatives, one persistent launch, layout chosen by the autotuner."""
+    from sysdefs import MIXED, basis, mixed_counts
+    counts = mixed_counts()
+    iv = Q.run_intervals(T=args.mixed_T, Dt=0.1)
+    systems = [Q.System(spec) for _, spec, _ in MIXED]
+
+    def members():
+        return [Q.Ensemble(s, int(n), basis(s.dim, idx), Q.mcwf_pars(seed=1000 + i, precision=Q.QT_FP32_DERIV))
+                for i, (s, n, (_, _, idx)) in enumerate(zip(systems, counts, MIXED))]
+
+    mem = members()
+    b = Q.Batch(mem)
+    t0 = time.perf_counter()
+    variant, probe_ms = b.autotune(probe_span=0.3, probe_traj=2048)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.advance(iv)
+    el = time.perf_counter() - t0
+    kms, _ = b.kernel_ms()
+    tries = [int(m.counters()[1][4]) for m in mem]
+    f32 = sum(try_flops(s)[0] * t for s, t in zip(systems, tries))
+    f64 = sum(try_flops(s)[1] * t for s, t in zip(systems, tries))
+    ach, peak = valu_roofline(f32, f64, kms / 1e3)
+    # the same members one after another, each in its own launch with its default layout
+    seq = 0.0
+    for i, (s, n, (_, _, idx)) in enumerate(zip(systems, counts, MIXED)):
+        e = Q.Ensemble(s, int(n), basis(s.dim, idx), Q.mcwf_pars(seed=1000 + i, precision=Q.QT_FP32_DERIV))
+        t1 = time.perf_counter()
+        e.advance(iv)
+        seq += time.perf_counter() - t1
+    return {"workload": f"{int(counts.sum())} trajectories drawn uniformly from 4 builds "
+                        f"({', '.join(f'{n} {c}' for (n, _, _), c in zip(MIXED, counts))}), T={args.mixed_T}, "
+                        "Dt=0.1, FP32 derivatives, one persistent launch",
+            "tries": sum(tries), "tries_per_member": tries, "seconds": el, "tries_per_s": sum(tries) / el,
+            "kernel_ms": kms, "variant": Q.batch_variants()[variant], "teams": b.layout()[1],
+            "autotune_s": tune_s, "autotune_probe_ms": [float(x) for x in probe_ms],
+            "sequential_seconds": seq, "sequential_tries_per_s": sum(tries) / seq,
+            "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                         "note": "sum over members of (6*8*M FP32 + 124*d FP64) flop per try"}}
+
+
+def serve(Q, args):
+    """Serving load: Poisson arrivals at a fixed rate, each an MCWF ensemble (256-4096 trajectories,
+    log-uniform) of a system drawn from the PLM-20, JC-60, mixed-batch and JC-80 builds, T=2, Dt=0.1,
+    FP32 derivatives; continuous batching with memory-based admission."""
+    from sysdefs import JC60, MIXED, PLM20, basis, coherent, ground, jc_master_spec
+    rng = np.random.default_rng(4242)
+    pool = [("plm20", Q.System(PLM20), coherent(0.5, 20)), ("jc60", Q.System(JC60), ground(60))]
+    pool += [(n, Q.System(spec), None) for n, spec, _ in MIXED[:3]]
+    idx = {n: i for n, _, i in MIXED}
+    jc80 = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in np.linspace(0.5, 15, 8)]
+    srv = Q.Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_active=16, chunk_intervals=10,
+                   pars=Q.mcwf_pars(precision=Q.QT_FP32_DERIV))
+    t_start = time.perf_counter()
+    t_next = t_start + rng.exponential(1.0 / args.serve_rate)
+    pending, lat, waits, tries, kinds = {}, [], [], 0, {}
+    submitted = 0
+    while True:
+        now = time.perf_counter()
+        while now >= t_next and now - t_start < args.serve_seconds:
+            k = int(rng.integers(0, len(pool) + 1))
+            if k == len(pool):
+                name, s = "jc80", jc80[int(rng.integers(0, len(jc80)))]
+                psi = ground(80)
+            else:
+                name, s, psi = pool[k]
+                if psi is None:
+                    psi = basis(s.dim, idx[name])
+            n = int(round(np.exp(rng.uniform(np.log(256), np.log(4096)))))
+            rid = srv.submit(s, n, psi, T=2.0, Dt=0.1, seed=int(rng.integers(1, 1 << 40)))
+            pending[rid] = name
+            submitted += 1
+            t_next += rng.exponential(1.0 / args.serve_rate)
+        a, w = srv.step()
+        for rid in list(pending):
+            r = srv.result(rid)
+            if r is not None:
+                lat.append(r[1]); waits.append(r[2]); tries += r[3]
+                kinds[pending[rid]] = kinds.get(pending[rid], 0) + 1
+                srv.release(rid)
+                del pending[rid]
+        if a == 0 and w == 0:
+            if time.perf_counter() - t_start >= args.serve_seconds:
+                break
+            time.sleep(max(0.0, min(t_next - time.perf_counter(), 0.05)))
+    el = time.perf_counter() - t_start
+    st = srv.stats()
+    lat = np.array(lat)
+    return {"workload": f"Poisson arrivals at {args.serve_rate}/s for {args.serve_seconds} s; ensembles of 256-4096 "
+                        "trajectories (log-uniform) of PLM-20, JC-60, particle-cavity 1024, four modes 4096, six qubits 64 "
+                        "or JC-80 (eta in [0.5, 15]); T=2, Dt=0.1; FP32 derivatives; up to 16 requests per launch, "
+                        f"{args.serve_budget_gib} GiB admission budget",
+            "requests": int(len(lat)), "submitted": submitted, "seconds": el, "tries": tries, "tries_per_s": tries / el,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_max_s": float(lat.max()), "queue_wait_mean_s": float(np.mean(waits)),
+            "launches": st["steps"], "kernel_s": st["kernel_ms"] / 1e3, "by_system": kinds}
 
 
 if __name__ == "__main__":

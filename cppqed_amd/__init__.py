@@ -27,7 +27,8 @@ class QtError(RuntimeError):
 class SysInfo(C.Structure):
     _fields_ = [("dim", C.c_long), ("rank", C.c_int), ("dims", C.c_int * 16), ("n_jumps", C.c_int),
                 ("n_avr", C.c_int), ("h_width", C.c_int), ("j_width", C.c_int), ("o_width", C.c_int),
-                ("time_dependent", C.c_int), ("highest_frequency", C.c_double), ("h_nnz", C.c_longlong)]
+                ("time_dependent", C.c_int), ("highest_frequency", C.c_double), ("h_nnz", C.c_longlong),
+                ("m_terms", C.c_longlong)]
 
 
 class McwfParams(C.Structure):

@@ -351,6 +351,9 @@ int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
     long long nnz = 0;
     for (auto& z : s.H.vals) if (z != qt::cd(0.)) ++nnz;
     info->h_nnz = nnz;
+    long long mt = 0;
+    for (auto& t : s.hTerms) mt += (long long)t.op.entries.size() * (s.dim / std::max(1, t.op.size()));
+    info->m_terms = mt;
   });
 }
 

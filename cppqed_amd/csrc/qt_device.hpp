@@ -54,6 +54,17 @@ __device__ __forceinline__ double readlane(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
+// A wavefront-uniform value read from LDS lands in vector registers; these put it back in scalar
+// registers so that addresses built from it stay scalar base + vector offset.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename P>
+__device__ __forceinline__ P* uniform(P* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
+}
+
 // Orders the LDS accesses of one wavefront around a hand-over between its lanes.  The LDS
 // executes one wave's instructions in issue order, so only the compiler has to be kept from
 // moving accesses across (wavefront-scope fences emit no hardware wait).

@@ -47,23 +47,29 @@ struct Engine {
   const int lane;
   int hcol[RR][RW];
   T hval[RR][RW];
+  // scalar copies of the H table's base addresses, width and dimension
+  const int* hcolU;
+  const T* hvalU;
+  int whU, dimU;
 
-  __device__ __forceinline__ T coef(int w, int row) const {
-    if constexpr (F32) return S.hval32[w * S.dim + row];
-    else return S.hval[w * S.dim + row];
-  }
+  __device__ __forceinline__ T coef(int w, int row) const { return hvalU[w * dimU + row]; }
 
   // lane_ = index of this thread within its team of NW wavefronts
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_)
       : S(s), xs(xs_), ps(ps_), red(red_), lane(lane_) {
+    hcolU = uniform(s.hcol);
+    if constexpr (F32) hvalU = uniform(s.hval32);
+    else hvalU = uniform(s.hval);
+    whU = uniform(s.wh);
+    dimU = uniform(s.dim);
     if constexpr (kRegs) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         const int row = r * G + lane;
 #pragma unroll
         for (int w = 0; w < WH; ++w) {
-          if (row < S.dim && w < S.wh) {
-            hcol[r][w] = S.hcol[w * S.dim + row];
+          if (row < dimU && w < whU) {
+            hcol[r][w] = hcolU[w * dimU + row];
             hval[r][w] = coef(w, row);
           } else {
             hcol[r][w] = 0;
@@ -74,9 +80,9 @@ struct Engine {
     }
   }
 
-  __device__ __forceinline__ bool valid(int r) const { return r * G + lane < S.dim; }
+  __device__ __forceinline__ bool valid(int r) const { return r * G + lane < dimU; }
   // row block r holds part of the state for some lane of the team (uniform over the team)
-  __device__ __forceinline__ bool live(int r) const { return r == 0 || r * G < S.dim; }
+  __device__ __forceinline__ bool live(int r) const { return r == 0 || r * G < dimU; }
   static __device__ __forceinline__ void sync() { team_sync<NW>(); }
 
   // k = (H_nH/i) v, with v already staged in xs
@@ -90,9 +96,9 @@ struct Engine {
         for (int w = 0; w < WH; ++w) acc = cfma(hval[r][w], xs[hcol[r][w]], acc);
       } else {
         const int row = r * G + lane;
-        if (row < S.dim) {
+        if (row < dimU) {
 #pragma unroll 4
-          for (int w = 0; w < S.wh; ++w) acc = cfma(coef(w, row), xs[S.hcol[w * S.dim + row]], acc);
+          for (int w = 0; w < whU; ++w) acc = cfma(coef(w, row), xs[hcolU[w * dimU + row]], acc);
         }
       }
       k[r] = acc;
@@ -207,6 +213,257 @@ struct Engine {
     return GroupReduce<NW>::sum(s, red);
   }
 };
+
+// ---------------------------------------------------------------------------
+// LDS-state engine for multi-wavefront teams: psi lives in LDS (FP64, read by every lane for
+// rates, jumps and observables without a copy), only the six stage derivatives stay in
+// registers, the step-start copy for a step-back goes to the trajectory's slot in HBM.  This
+// halves the register footprint of a lane's rows and keeps large states (dim up to 4096 in a
+// 16-wavefront team) free of register spills.
+// ---------------------------------------------------------------------------
+
+template <int ROWS, int NW, int WH, bool F32>
+struct LdsEngine : Engine<ROWS, NW, WH, F32> {
+  using B = Engine<ROWS, NW, WH, F32>;
+  using T = typename B::T;
+  using B::G; using B::S; using B::xs; using B::red; using B::lane;
+  double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
+
+  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_)
+      : B(s, xs_, X_, red_, lane_), X(X_) {}
+
+  // k = (H/i) X
+  __device__ __forceinline__ void derivativeX(T k[ROWS]) const {
+    B::sync();
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) if (B::live(r)) xs[r * G + lane] = from_d<T>(X[r * G + lane]);
+    B::sync();
+    B::gather(k);
+  }
+
+  __device__ __forceinline__ void stageX(const double2 v[ROWS]) const { B::stage(v); }
+
+  // One RKCK54 try from X with k1 = (H/i) X; returns the odeint error norm and, when the try is
+  // accepted (norm <= 1), replaces the lane's rows of X by the 5th-order solution.
+  __device__ double tryStepX(const T k1[ROWS], double dt, double epsAbs, double epsRel, long long& derivs) const {
+    T k2[ROWS], k3[ROWS], k4[ROWS], k5[ROWS], k6[ROWS];
+    double2 v[ROWS];
+    auto xr = [&](int r) { return X[r * G + lane]; };
+    const double s21 = CK::a21 * dt;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(s21, to_d_use(k1[r]), xr(r));
+    B::stage(v); B::gather(k2);
+    { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r))); }
+    B::stage(v); B::gather(k3);
+    { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (B::live(r)) v[r] = axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r)))); }
+    B::stage(v); B::gather(k4);
+    { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (B::live(r))
+          v[r] = axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r))))); }
+    B::stage(v); B::gather(k5);
+    { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (B::live(r))
+          v[r] = axpy(q5, to_d_use(k5[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]),
+                 axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r)))))); }
+    B::stage(v); B::gather(k6);
+    derivs += 5;
+    const double q1 = CK::b1 * dt, q3 = CK::b3 * dt, q4 = CK::b4 * dt, q6 = CK::b6 * dt;
+    double e = 0.;
+    {
+      const double f1 = CK::e1 * dt, f3 = CK::e3 * dt, f4 = CK::e4 * dt, f5 = CK::e5 * dt, f6 = CK::e6 * dt;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (!B::valid(r)) continue;
+        const double2 d1 = to_d_use(k1[r]);
+        double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
+        xe = axpy(f6, to_d_use(k6[r]), axpy(f5, to_d_use(k5[r]), axpy(f4, to_d_use(k4[r]), axpy(f3, to_d_use(k3[r]), xe))));
+        const double ax = sqrt(cabs2(xr(r))), ak = sqrt(cabs2(d1));
+        const double D = epsAbs + epsRel * (ax + dt * ak);
+        double rc = __builtin_amdgcn_rcp(D);
+        rc = fma(rc, fma(-D, rc, 1.0), rc);
+        rc = fma(rc, fma(-D, rc, 1.0), rc);
+        e = fmax(e, cabs2(xe) * (rc * rc));
+      }
+    }
+    const double err = sqrt(GroupReduce<NW>::max(e, red));
+    if (err <= 1.0) {
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (B::live(r))
+          X[r * G + lane] = axpy(q6, to_d_use(k6[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q1, to_d_use(k1[r]), xr(r)))));
+    }
+    return err;
+  }
+
+  // <X|O_q|X>, real or imaginary part as the observable asks
+  __device__ double observableX(int q) const {
+    double s = 0.;
+    const bool im = S.oimag[q] != 0;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      if (!B::valid(r)) continue;
+      const int row = r * G + lane;
+      double2 acc = make_double2(0., 0.);
+      const int base = q * S.wo * S.dim + row;
+      const int wq = S.owid[q];
+      for (int w = 0; w < wq; ++w) acc = cfma(S.oval[base + w * S.dim], X[S.ocol[base + w * S.dim]], acc);
+      const double2 x = X[row];
+      s += im ? (x.x * acc.y - x.y * acc.x) : (x.x * acc.x + x.y * acc.y);
+    }
+    return GroupReduce<NW>::sum(s, red);
+  }
+};
+
+// advance_traj for the LDS-state engine: the same MCWF / ODE control flow, with psi in X.
+template <class Eng>
+__device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& E, const Eng& eng, int traj,
+                                                 double* rts, long long* cnt) {
+  constexpr int ROWS = Eng::kRows, G = Eng::G, NW = Eng::kNW;
+  using T = typename Eng::T;
+  const int lane = eng.lane;
+  double2* X = eng.X;
+  double2* slot = E.psi + (size_t)traj * S.dim;  // step-start copy (the launch's final state goes here too)
+  auto bump = [&](int c, long long v) { if (lane == 0) cnt[c] += v; };
+
+  T k1[ROWS];
+  Eng::sync();  // previous users of X / cnt are done
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = eng.valid(r) ? slot[r * G + lane] : make_double2(0., 0.);
+  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
+  unsigned long long ord = E.ordinal[traj];
+  const unsigned long long ord0 = ord;
+  const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
+  if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
+  Eng::sync();
+  const unsigned long long key = E.traj0 + (unsigned long long)traj;
+  const int nj = S.nj;
+  const double epsAbs = E.epsAbs, epsRel = E.epsRel;
+
+  auto coherent = [&](double Dt) {  // MCWF_Trajectory.h:250-276 with ODE.h:234-262
+    const double timeBefore = t;
+    const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
+    if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
+    if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
+    eng.derivativeX(k1);
+    long long derivs = 1, tries = 0, failed = 0;
+    for (;;) {
+      ++tries;
+      const double err = eng.tryStepX(k1, dtTry, epsAbs, epsRel, derivs);
+      if (err > 1.0) {
+        dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
+        ++failed;
+        continue;
+      }
+      t += dtTry;
+      if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
+      break;
+    }
+    dtDid = t - timeBefore;
+    if (nextDtTry != 0.) dtTry = nextDtTry;
+    if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
+    double s2 = 0.;  // renorm (StateVector.h:148-153)
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(X[r * G + lane]);
+    const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s2, eng.red));
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) if (eng.live(r)) { double2& x = X[r * G + lane]; x.x *= inv; x.y *= inv; }
+    Eng::sync();  // X complete before other lanes gather from it
+  };
+
+  auto computeRates = [&]() {
+    for (int m = 0; m < nj; ++m) {
+      double sacc = 0.;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+      const double v = GroupReduce<NW>::sum(sacc, eng.red);
+      if (lane == 0) rts[m] = v;
+    }
+    Eng::sync();
+  };
+
+  for (int k = 0; k < E.n_intervals; ++k) {
+    const double endTime = t + E.intervals[k];
+    for (;;) {
+      const double Dt = endTime - t;
+      if (Dt == 0.) break;
+      const unsigned long long di = ord - ord0;
+      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) slot[r * G + lane] = X[r * G + lane];
+      const double tCache = t, dtDidCache = dtDid;
+      coherent(Dt);
+      if (nj > 0) {
+        computeRates();
+        for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
+          double totalRate = 0.;
+          for (int m = 0; m < nj; ++m) totalRate += rts[m];
+          const double sugg = E.dpLimit / totalRate;
+          if (totalRate * dtDid > E.overshootTol * E.dpLimit) {
+            t = tCache; dtDid = dtDidCache; dtTry = sugg;
+            bump(kStepBacks, 1);
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = eng.valid(r) ? slot[r * G + lane] : make_double2(0., 0.);
+            coherent(Dt);
+            computeRates();
+            continue;
+          } else if (totalRate * dtTry > E.dpLimit) bump(kOvershoots, 1);
+          dtTry = fmin(dtTry, sugg);
+          break;
+        }
+        const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);  // performJump (:320-344)
+        ++ord;
+        double random = u / dtDid;
+        int no = 0;
+        for (; random > 0. && no != nj; random -= rts[no++]) ;
+        if (random < 0.) {
+          --no;
+          const double nf = sqrt(rts[no]);
+          if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
+          const double inv = 1.0 / nf;
+          double2 y[ROWS];
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) {
+            const double2 a = eng.valid(r) ? eng.jumpRow(no, r) : make_double2(0., 0.);
+            y[r] = make_double2(a.x * inv, a.y * inv);
+          }
+          Eng::sync();  // every lane has read X
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = y[r];
+          Eng::sync();
+          if (lane == 0) {
+            const long long nJ = cnt[kJumps];
+            if (nJ < E.jcap) {
+              E.jt[(size_t)traj * E.jcap + nJ] = t;
+              E.jch[(size_t)traj * E.jcap + nJ] = no;
+            }
+            cnt[kJumps] = nJ + 1;
+          }
+        }
+      }
+      bump(kMcwfSteps, 1);
+    }
+    for (int q = 0; q < S.no; ++q) {
+      const double v = eng.observableX(q);
+      if (lane == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = v;
+    }
+    if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+  }
+
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) slot[r * G + lane] = X[r * G + lane];
+  Eng::sync();
+  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
+  if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
+}
 
 // ---------------------------------------------------------------------------
 // one trajectory through every stream interval of a launch
@@ -385,9 +642,6 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   // kernel arguments live in LDS: read where used instead of pinning ~80 scalar registers
   __shared__ SysDev S;
   __shared__ EnsDev E;
-  // the RK stage vector (derivative precision) and the FP64 state copy share storage
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double2 pc[kPcLds ? ROWS * G : 1];
   __shared__ double red[NW > 1 ? NW : 1];
   __shared__ double rts[kMaxJumps];        // jump rates of the current step
   __shared__ long long cnt[kCounters];     // this trajectory's counters (lane 0 updates)
@@ -396,15 +650,25 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   const int lane = threadIdx.x;
   if (lane == 0) { S = Sarg; E = Earg; }
   __syncthreads();
-  Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane);
-
-  for (;;) {
+  auto claim = [&]() {
     __syncthreads();
     if (lane == 0) shTraj = atomicAdd(E.work, 1);
     __syncthreads();
-    const int traj = shTraj;
-    if (traj >= E.n_traj) return;
-    advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
+    return shTraj;
+  };
+  if constexpr (NW > 1) {
+    // multi-wavefront team: psi in LDS
+    __shared__ double2 Xs[ROWS * G];
+    __shared__ T xs[ROWS * G];
+    LdsEngine<ROWS, NW, WH, F32> eng(S, xs, Xs, red, lane);
+    for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds(S, E, eng, traj, rts, cnt);
+  } else {
+    // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
+    // state copy share storage
+    __shared__ double2 ps[ROWS * G];
+    __shared__ double2 pc[kPcLds ? ROWS * G : 1];
+    Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane);
+    for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
   }
 }
 
@@ -418,13 +682,12 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
 
 template <int NWB, int RB, int WHB, int RW, int WHW, bool F32>
 __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
-  using EngB = Engine<RB, NWB, WHB, F32>;
+  using EngB = LdsEngine<RB, NWB, WHB, F32>;
   using EngW = Engine<RW, 1, WHW, F32>;
   using TB = typename EngB::T;
   using TW = typename EngW::T;
   constexpr int GB = 64 * NWB;
-  constexpr bool kPcB = pc_in_lds<RB, NWB>();
-  constexpr size_t kBlockBytes = (size_t)RB * GB * sizeof(double2) * (kPcB ? 2 : 1);
+  constexpr size_t kBlockBytes = (size_t)RB * GB * (sizeof(double2) + sizeof(TB));
   constexpr size_t kWaveEach = (size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8;
   constexpr size_t kWaveBytes = kWaveEach * NWB;
   constexpr size_t kBytes = kBlockBytes > kWaveBytes ? kBlockBytes : kWaveBytes;
@@ -442,10 +705,10 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   }
   __syncthreads();
 
-  // phase 1: one trajectory per work group
+  // phase 1: one trajectory per work group (psi in LDS)
   {
-    double2* ps = reinterpret_cast<double2*>(pool);
-    double2* pc = kPcB ? ps + RB * GB : nullptr;
+    double2* X = reinterpret_cast<double2*>(pool);
+    TB* xs = reinterpret_cast<TB*>(X + RB * GB);
     for (;;) {
       __syncthreads();
       if (threadIdx.x == 0) shItem = atomicAdd(M.queue, 1);
@@ -456,8 +719,8 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       int m = -1;
       for (int k = 0; k < M.n_members; ++k)
         if (M.team[k] == 0 && M.first[k] <= item) m = k;
-      EngB eng(M.S[m], reinterpret_cast<TB*>(ps), ps, red, (int)threadIdx.x);
-      advance_traj<EngB, kPcB>(M.S[m], M.E[m], eng, item - M.first[m], pc, rtsB, cntB);
+      EngB eng(M.S[m], xs, X, red, (int)threadIdx.x);
+      advance_traj_lds(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
     }
   }
 

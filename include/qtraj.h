@@ -54,6 +54,9 @@ typedef struct {
   int time_dependent;       /* 1 in interaction picture */
   double highest_frequency; /* QuantumSystem::highestFrequency, QuantumSystem.h:21-41 */
   long long h_nnz;          /* stored nonzeros of H_nH/i (merged) */
+  long long m_terms;        /* complex multiply-adds of one H_nH/i application counted the reference's
+                               way: stored entries of every term's operator times its slice count
+                               (Tridiagonal<R>::apply over fullRange, Tridiagonal.tcc:220-263) */
 } qt_sys_info_t;
 int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info);
 
