@@ -76,6 +76,7 @@ def lib():
         l.qt_mcwf_advance.argtypes = [vp, vp, ip, vp, vp]
         l.qt_mcwf_advance_async.argtypes = [vp, vp, ip, vp]
         l.qt_mcwf_reset.argtypes = [vp, vp]
+        l.qt_mcwf_set_graph.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         l.qt_mcwf_traj_averages.argtypes = [vp, vp]
         l.qt_mcwf_state.argtypes = [vp, ip, vp, vp]
         l.qt_mcwf_jumplog.argtypes = [vp, ip, vp, vp, ip, C.POINTER(C.c_int)]
@@ -311,6 +312,13 @@ class Ensemble:
     def advance_async(self, intervals, stream=None):
         iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
         check(lib().qt_mcwf_advance_async(self.h, iv.ctypes.data, len(iv), stream))
+
+    def set_graph(self, enable=True):
+        """Replay the advance's launch sequence from a captured hipGraph (qt_mcwf_set_graph);
+        returns (captures, replays) so far."""
+        c, r = C.c_longlong(), C.c_longlong()
+        check(lib().qt_mcwf_set_graph(self.h, -1 if enable is None else int(bool(enable)), C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def traj_averages(self):
         out = np.zeros((self.n, self.system.n_avr))

@@ -246,7 +246,22 @@ struct qt_ens {
   double kernelMs = 0.;
   int launches = 0;
   double dtMinSeen = 0.02;  // typical accepted step, sizes the jump-draw budget
+  // captured launch sequence (jump draws, queue reset, batched kernel), replayed while the
+  // launch description is unchanged
+  bool useGraph = false;
+  hipStream_t capStream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  qt::EnsDev gKey{};
+  long long graphLaunches = 0, graphCaptures = 0;
+  void dropGraph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr; graph = nullptr;
+  }
   ~qt_ens() {
+    dropGraph();
+    if (capStream) (void)hipStreamDestroy(capStream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
   }
@@ -450,8 +465,10 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
 int qt_mcwf_destroy(qt_ens* ens) { delete ens; return QT_OK; }
 
 // Per-launch preparation of one ensemble: uploads the intervals, sizes the stream outputs,
-// generates the launch's jump draws; returns the kernel's view of the ensemble.
-static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool bulkDraws) {
+// generates the launch's jump draws (unless launchDraws is false: then only sized); returns the
+// kernel's view of the ensemble.
+static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool bulkDraws,
+                              bool launchDraws = true) {
   const qt::System& s = *e->sys->h;
   if (n_int <= 0) throw Fail(QT_EINVAL, "no intervals");
   for (int k = 0; k < n_int; ++k) if (!(intervals[k] >= 0) || !std::isfinite(intervals[k])) throw Fail(QT_EINVAL, "intervals must be finite and non-negative");
@@ -469,16 +486,14 @@ static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hip
     double span = 0.; for (int k = 0; k < n_int; ++k) span += intervals[k];
     const int cap = (int)std::min(2048.0, std::max(64.0, std::ceil(span / std::max(e->dtMinSeen, 1e-6) * 1.25) + 8));
     e->draws.alloc((size_t)e->n * cap);
-    hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st), "jump draws");
+    if (launchDraws)
+      hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st), "jump draws");
     E.draws = e->draws.p; E.draw_cap = cap;
   }
   return E;
 }
 
-static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool timed) {
-  qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0);
-  hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
-  if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
+static void ens_kernel(qt_ens* e, const qt::EnsDev& E, hipStream_t st) {
   if (e->lay.glanes > 0)
     hipCheck(qt::launch_sub(e->lay.glanes, e->lay.rows, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
              "mcwf sub-wave launch");
@@ -486,7 +501,37 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
     hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV, e->grid,
                               e->sys->dev, E, st),
              "mcwf batch launch");
-  if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
+}
+
+static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_t st, bool timed) {
+  if (e->useGraph) {
+    // the launch description (buffers, interval count, draw budget) decides whether the
+    // captured graph still applies; the interval values are re-uploaded before every replay
+    qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0, false);
+    if (!e->gexec || std::memcmp(&E, &e->gKey, sizeof(E)) != 0) {
+      e->dropGraph();
+      if (!e->capStream) hipCheck(hipStreamCreate(&e->capStream), "hipStreamCreate");
+      hipCheck(hipStreamBeginCapture(e->capStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      if (E.draws)
+        hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, E.draw_cap, e->draws.p, e->capStream), "jump draws");
+      hipCheck(qt::launch_fill_int(e->work.p, 1, 0, e->capStream), "reset work queue");
+      ens_kernel(e, E, e->capStream);
+      hipCheck(hipStreamEndCapture(e->capStream, &e->graph), "hipStreamEndCapture");
+      hipCheck(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      e->gKey = E;
+      ++e->graphCaptures;
+    }
+    if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
+    hipCheck(hipGraphLaunch(e->gexec, st), "hipGraphLaunch");
+    if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
+    ++e->graphLaunches;
+  } else {
+    qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0);
+    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
+    ens_kernel(e, E, st);
+    if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
+  }
   e->lastIntervals = n_int;
   for (int k = 0; k < n_int; ++k) e->time += intervals[k];
 }
@@ -550,6 +595,18 @@ int qt_mcwf_advance_async(qt_ens* e, const double* intervals, int n_int, qt_stre
   return guard([&] {
     if (!e || !intervals) throw Fail(QT_EINVAL, "null argument");
     ens_launch(e, intervals, n_int, (hipStream_t)stream, false);
+  });
+}
+
+int qt_mcwf_set_graph(qt_ens* e, int enable, long long* captures, long long* replays) {
+  return guard([&] {
+    if (!e) throw Fail(QT_EINVAL, "null argument");
+    if (enable >= 0) {
+      e->useGraph = enable != 0;
+      if (!e->useGraph) e->dropGraph();
+    }
+    if (captures) *captures = e->graphCaptures;
+    if (replays) *replays = e->graphLaunches;
   });
 }
 

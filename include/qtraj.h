@@ -109,6 +109,13 @@ int qt_mcwf_advance_async(qt_ens* ens, const double* intervals, int n_intervals,
  * RNG ordinal 0, counters 0), device-side, asynchronously on `stream`. */
 int qt_mcwf_reset(qt_ens* ens, qt_stream stream);
 
+/* Graph replay: with enable = 1 the launch sequence of an advance (jump draws, queue reset,
+ * batched kernel) is captured once into a hipGraph and replayed by later advances whose launch
+ * description (interval count, buffers, draw budget) is unchanged -- interval values are
+ * uploaded before each replay.  enable = 0 returns to direct launches, -1 only queries.
+ * captures / replays (may be NULL) report how often each happened. */
+int qt_mcwf_set_graph(qt_ens* ens, int enable, long long* captures, long long* replays);
+
 /* Per-trajectory raw (unprocessed) averages at the last streamed point: out[n_traj*n_avr]. */
 int qt_mcwf_traj_averages(qt_ens* ens, double* out);
 /* State of trajectory i (host copy) and its time. */
