@@ -165,7 +165,7 @@ def main():
                   "residual_median": float(np.median(dn[:, -1])), "residual_max": float(dn[:, -1].max()),
                   "t_steady_median": float(np.median(tss[tss > 0])) if (tss > 0).any() else None,
                   "roofline": {"bound": "mfma", "achieved": m_ach, "peak": 78.6, "unit": "TFLOP/s", "frac": m_ach / 78.6,
-                               "traffic": None,
+                               "traffic": master_traffic(args, mtries, chunks),
                                "note": "8 d^3 flop of dense complex (H_nH/i) rho per derivative, 6 per try; "
                                        f"{f_try / 1e6:.1f} MFLOP per try in all, sparse-equivalent 1.82 MFLOP"}}
 
@@ -206,6 +206,17 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out))
+
+
+def master_traffic(args, tries, launches):
+    """HBM bytes per Master launch from the committed FETCH_SIZE / WRITE_SIZE runs of this workload
+    (bytes per RKCK try x this run's tries per launch), or None for another workload."""
+    path = os.path.join(ROOT, "profiles", "master_sweep_jc80_512_traffic.json")
+    if args.master_points != 512 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        per_try = json.load(f)["traffic_bytes_per_try"]
+    return per_try * tries / max(1, launches)
 
 
 def mixed_batch(Q, args):
