@@ -52,7 +52,9 @@ struct Engine {
   const T* hvalU;
   int whU, dimU;
 
-  __device__ __forceinline__ T coef(int w, int row) const { return hvalU[w * dimU + row]; }
+  // unsigned element offsets from a scalar base: loads address as scalar base + 32-bit vector offset
+  __device__ __forceinline__ T coef(int w, int row) const { return hvalU[(unsigned)(w * dimU + row)]; }
+  __device__ __forceinline__ int col(int w, int row) const { return hcolU[(unsigned)(w * dimU + row)]; }
 
   // lane_ = index of this thread within its team of NW wavefronts
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_)
@@ -69,7 +71,7 @@ struct Engine {
 #pragma unroll
         for (int w = 0; w < WH; ++w) {
           if (row < dimU && w < whU) {
-            hcol[r][w] = hcolU[w * dimU + row];
+            hcol[r][w] = col(w, row);
             hval[r][w] = coef(w, row);
           } else {
             hcol[r][w] = 0;
@@ -78,6 +80,14 @@ struct Engine {
         }
       }
     }
+  }
+
+  // FP64 value of a stage derivative at its point of use.  With several rows per lane the FP32
+  // values stay live and are converted at each use (half the registers); with one or two rows
+  // the compiler may keep converted copies (fewer instructions).
+  static __device__ __forceinline__ double2 kd(const T& k) {
+    if constexpr (ROWS >= 4) return to_d_use(k);
+    else return to_d(k);
   }
 
   __device__ __forceinline__ bool valid(int r) const { return r * G + lane < dimU; }
@@ -98,7 +108,7 @@ struct Engine {
         const int row = r * G + lane;
         if (row < dimU) {
 #pragma unroll 4
-          for (int w = 0; w < whU; ++w) acc = cfma(coef(w, row), xs[hcolU[w * dimU + row]], acc);
+          for (int w = 0; w < whU; ++w) acc = cfma(coef(w, row), xs[col(w, row)], acc);
         }
       }
       k[r] = acc;
@@ -129,25 +139,25 @@ struct Engine {
     double2 v[ROWS];
     const double s21 = CK::a21 * dt;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, to_d_use(k1[r]), x[r]);
+    for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, kd(k1[r]), x[r]);
     stage(v); gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r])); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r])); }
     stage(v); gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r]))); }
+      for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r]))); }
     stage(v); gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        if (live(r)) v[r] = axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r])))); }
+        if (live(r)) v[r] = axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r])))); }
     stage(v); gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        if (live(r)) v[r] = axpy(q5, to_d_use(k5[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), x[r]))))); }
+        if (live(r)) v[r] = axpy(q5, kd(k5[r]), axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r]))))); }
     stage(v); gather(k6);
     derivs += 5;
     double e = 0.;
@@ -157,7 +167,7 @@ struct Engine {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         if (!live(r)) { xn[r] = make_double2(0., 0.); continue; }
-        const double2 d1 = to_d_use(k1[r]), d3 = to_d_use(k3[r]), d4 = to_d_use(k4[r]), d5 = to_d_use(k5[r]), d6 = to_d_use(k6[r]);
+        const double2 d1 = kd(k1[r]), d3 = kd(k3[r]), d4 = kd(k4[r]), d5 = kd(k5[r]), d6 = kd(k6[r]);
         xn[r] = axpy(q6, d6, axpy(q4, d4, axpy(q3, d3, axpy(q1, d1, x[r]))));
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
         xe = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, xe))));
@@ -226,7 +236,7 @@ template <int ROWS, int NW, int WH, bool F32>
 struct LdsEngine : Engine<ROWS, NW, WH, F32> {
   using B = Engine<ROWS, NW, WH, F32>;
   using T = typename B::T;
-  using B::G; using B::S; using B::xs; using B::red; using B::lane;
+  using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
 
   __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_)
@@ -251,29 +261,29 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
     auto xr = [&](int r) { return X[r * G + lane]; };
     const double s21 = CK::a21 * dt;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(s21, to_d_use(k1[r]), xr(r));
+    for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(s21, kd(k1[r]), xr(r));
     B::stage(v); B::gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r))); }
+      for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r))); }
     B::stage(v); B::gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
-        if (B::live(r)) v[r] = axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r)))); }
+        if (B::live(r)) v[r] = axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r)))); }
     B::stage(v); B::gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r))
-          v[r] = axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r))))); }
+          v[r] = axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r))))); }
     B::stage(v); B::gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r))
-          v[r] = axpy(q5, to_d_use(k5[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]),
-                 axpy(q2, to_d_use(k2[r]), axpy(q1, to_d_use(k1[r]), xr(r)))))); }
+          v[r] = axpy(q5, kd(k5[r]), axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]),
+                 axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r)))))); }
     B::stage(v); B::gather(k6);
     derivs += 5;
     const double q1 = CK::b1 * dt, q3 = CK::b3 * dt, q4 = CK::b4 * dt, q6 = CK::b6 * dt;
@@ -283,9 +293,9 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         if (!B::valid(r)) continue;
-        const double2 d1 = to_d_use(k1[r]);
+        const double2 d1 = kd(k1[r]);
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
-        xe = axpy(f6, to_d_use(k6[r]), axpy(f5, to_d_use(k5[r]), axpy(f4, to_d_use(k4[r]), axpy(f3, to_d_use(k3[r]), xe))));
+        xe = axpy(f6, kd(k6[r]), axpy(f5, kd(k5[r]), axpy(f4, kd(k4[r]), axpy(f3, kd(k3[r]), xe))));
         const double ax = sqrt(cabs2(xr(r))), ak = sqrt(cabs2(d1));
         const double D = epsAbs + epsRel * (ax + dt * ak);
         double rc = __builtin_amdgcn_rcp(D);
@@ -299,7 +309,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r))
-          X[r * G + lane] = axpy(q6, to_d_use(k6[r]), axpy(q4, to_d_use(k4[r]), axpy(q3, to_d_use(k3[r]), axpy(q1, to_d_use(k1[r]), xr(r)))));
+          X[r * G + lane] = axpy(q6, kd(k6[r]), axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q1, kd(k1[r]), xr(r)))));
     }
     return err;
   }
