@@ -681,25 +681,34 @@ std::vector<MixVariant> mix_variants() {
   return v;
 }
 
-// can member (dim, width) run as team `team` under variant V?  Register-cached ELL rows need
-// width <= the variant's width; uncached ones read any width from the table.
+// Team kinds of the mixed launch: 0 = a whole work group, 1 = one wavefront, 2 = a quarter
+// of the work group.  Can member (dim, width) run as `team` under variant V?  Register-cached
+// ELL rows need width <= the variant's width; uncached ones read any width from the table.
 bool team_fits(const MixVariant& V, int team, long dim, int width) {
   if (team == 1) return dim <= 64L * V.rw && (V.rw * V.whw > 24 || width <= V.whw);
-  return dim <= 64L * V.nwb * V.rb && (V.rb * V.whb > 24 || width <= V.whb);
+  const int nw = team == 2 ? V.nwb / 4 : V.nwb;
+  return nw >= 1 && dim <= 64L * nw * V.rb && (V.rb * V.whb > 24 || width <= V.whb);
 }
 
-// default team: one wavefront per trajectory when the state fits a wavefront's rows
-bool assign_teams(const MixVariant& V, const std::vector<qt_ens*>& mem, const int* req, std::vector<int>& team) {
+// Default team: the smallest that holds the state (wavefront, then quarter, then work group);
+// policy 1 leaves quarters out, policy 2 puts every member on whole work groups.
+bool assign_teams(const MixVariant& V, const std::vector<qt_ens*>& mem, const int* req, std::vector<int>& team,
+                  int policy = 0) {
   team.assign(mem.size(), 0);
+  static const int order[3][3] = {{1, 2, 0}, {1, 0, -1}, {0, -1, -1}};
   for (size_t m = 0; m < mem.size(); ++m) {
     const qt::System& s = *mem[m]->sys->h;
     const int want = req ? req[m] : -1;
-    if (want == 0 || want == 1) {
-      if (!team_fits(V, want, s.dim, s.H.width)) return false;
+    if (want >= 0) {
+      if (want > 2 || !team_fits(V, want, s.dim, s.H.width)) return false;
       team[m] = want;
-    } else if (team_fits(V, 1, s.dim, s.H.width)) team[m] = 1;
-    else if (team_fits(V, 0, s.dim, s.H.width)) team[m] = 0;
-    else return false;
+      continue;
+    }
+    int got = -1;
+    for (int k = 0; k < 3 && got < 0; ++k)
+      if (order[policy][k] >= 0 && team_fits(V, order[policy][k], s.dim, s.H.width)) got = order[policy][k];
+    if (got < 0) return false;
+    team[m] = got;
   }
   return true;
 }
@@ -712,7 +721,7 @@ struct qt_batch_s {
   std::vector<int> team;
   bool f32 = false;
   DevBuf<qt::MixDev> mix;
-  DevBuf<int> queue;
+  DevBuf<int> queue;            // [3] queue heads + [1] barrier error flag
   qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
   int grid = 0;
   int cus = 0;                  // compute units of the device (cached)
@@ -724,15 +733,15 @@ struct qt_batch_s {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
   }
-  void choose(int v, const int* req) {
+  void choose(int v, const int* req, int policy = 0) {
     auto V = mix_variants();
     if (v >= (int)V.size()) throw Fail(QT_EINVAL, "unknown batch layout variant");
     if (v >= 0) {
-      if (!assign_teams(V[v], mem, req, team)) throw Fail(QT_ENOTSUP, "the requested layout does not fit every member");
+      if (!assign_teams(V[v], mem, req, team, policy)) throw Fail(QT_ENOTSUP, "the requested layout does not fit every member");
       variant = v;
     } else {
       int found = -1;
-      for (size_t i = 0; i < V.size() && found < 0; ++i) if (assign_teams(V[i], mem, req, team)) found = (int)i;
+      for (size_t i = 0; i < V.size() && found < 0; ++i) if (assign_teams(V[i], mem, req, team, policy)) found = (int)i;
       if (found < 0) throw Fail(QT_ENOTSUP, "no compiled mixed-batch layout fits every member");
       variant = found;
     }
@@ -767,7 +776,7 @@ int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params
       if ((e->p.precision == QT_FP32_DERIV) != b->f32) throw Fail(QT_EINVAL, "members must share one precision");
     b->choose(p ? p->variant : -1, p ? p->team : nullptr);
     b->mix.alloc(1);
-    b->queue.alloc(2);
+    b->queue.alloc(4);
     hipCheck(hipHostMalloc((void**)&b->host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
     hipCheck(hipEventCreate(&b->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&b->e1), "hipEventCreate");
@@ -803,7 +812,8 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   std::memset(&M, 0, sizeof(M));
   M.n_members = (int)b->mem.size();
   M.queue = b->queue.p;
-  int items[2] = {0, 0};
+  M.err = b->queue.p + 3;
+  int items[3] = {0, 0, 0};
   for (size_t m = 0; m < b->mem.size(); ++m) {
     qt_ens* e = b->mem[m];
     if (!intervals[m]) throw Fail(QT_EINVAL, "null interval list");
@@ -813,18 +823,26 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
     M.first[m] = items[b->team[m]];
     items[b->team[m]] += e->n;
   }
-  M.block_items = items[0];
-  M.wave_items = items[1];
+  for (int k = 0; k < 3; ++k) M.items[k] = items[k];
   hipCheck(hipMemcpyAsync(b->mix.p, &M, sizeof(M), hipMemcpyHostToDevice, st), "batch descriptor H2D");
-  hipCheck(qt::launch_fill_int(b->queue.p, 2, 0, st), "reset queues");
-  const int groupsForWaves = (items[1] + L.nwb - 1) / L.nwb;
-  const int grid = std::max(1, std::min(b->grid, items[0] + groupsForWaves));
+  hipCheck(qt::launch_fill_int(b->queue.p, 4, 0, st), "reset queues");
+  // every resident work group, also when the work would fit fewer: wavefront and quarter
+  // items then spread over all compute units instead of filling a few
+  const int grid = std::max(1, b->grid);
   hipCheck(hipEventRecord(b->e0, st), "hipEventRecord");
   hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, b->f32, grid, b->mix.p, st), "mixed launch");
   hipCheck(hipEventRecord(b->e1, st), "hipEventRecord");
   for (size_t m = 0; m < b->mem.size(); ++m) {
     b->mem[m]->lastIntervals = n_int[m];
   }
+}
+
+// a quarter team's counter barrier that timed out leaves 2 in the batch's error flag
+static void batch_check(qt_batch* b, hipStream_t st) {
+  int flag = 0;
+  hipCheck(hipMemcpyAsync(&flag, b->queue.p + 3, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (flag) throw Fail(QT_EDEVICE, "mixed launch: a team barrier timed out");
 }
 
 extern "C" {
@@ -837,6 +855,7 @@ int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_i
     std::vector<double> t0(b->mem.size());
     for (size_t m = 0; m < b->mem.size(); ++m) t0[m] = b->mem[m]->time;
     batch_launch(b, intervals, n_int, st);
+    batch_check(b, st);
     for (size_t m = 0; m < b->mem.size(); ++m) {
       qt_ens* e = b->mem[m];
       for (int k = 0; k < n_int[m]; ++k) e->time += intervals[m][k];
@@ -879,25 +898,24 @@ int qt_batch_autotune(qt_batch* b, double probe_span, int probe_traj, int* varia
     std::vector<const double*> ivs(pm.size(), &iv);
     std::vector<int> nis(pm.size(), 1);
     for (size_t v = 0; v < V.size(); ++v) {
-      // candidate team assignments: by dimension, and everything on work groups
-      for (int mode = 0; mode < 2; ++mode) {
-        std::vector<int> req(pm.size(), mode == 0 ? -1 : 0);
+      // candidate team assignments: smallest team that holds each state, no quarter teams,
+      // everything on whole work groups (duplicates skipped)
+      std::vector<std::vector<int>> seen;
+      for (int policy = 0; policy < 3; ++policy) {
         std::vector<int> tm;
-        if (!assign_teams(V[v], pm, req.data(), tm)) continue;
-        if (mode == 1) {
-          std::vector<int> t0;
-          if (assign_teams(V[v], pm, nullptr, t0) && t0 == tm) continue;  // same as mode 0
-        }
+        if (!assign_teams(V[v], pm, nullptr, tm, policy)) continue;
+        if (std::find(seen.begin(), seen.end(), tm) != seen.end()) continue;
+        seen.push_back(tm);
         qt_batch probe;
         probe.mem = pm; probe.f32 = b->f32;
-        probe.choose((int)v, req.data());
-        probe.mix.alloc(1); probe.queue.alloc(2);
+        probe.choose((int)v, tm.data());
+        probe.mix.alloc(1); probe.queue.alloc(4);
         hipCheck(hipHostMalloc((void**)&probe.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
         hipCheck(hipEventCreate(&probe.e0), "hipEventCreate");
         hipCheck(hipEventCreate(&probe.e1), "hipEventCreate");
         for (auto* e : pm) { int rc = qt_mcwf_reset(e, nullptr); if (rc) throw Fail(rc, g_err); }
         batch_launch(&probe, ivs.data(), nis.data(), nullptr);
-        hipCheck(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        batch_check(&probe, nullptr);
         float ms = 0.f;
         hipCheck(hipEventElapsedTime(&ms, probe.e0, probe.e1), "hipEventElapsedTime");
         if (ms_out && nc < ms_cap) ms_out[nc] = ms;
@@ -1248,7 +1266,7 @@ int qt_server_create(const qt_server_params* p, qt_server** out) {
     s->P = *p;
     s->batch.f32 = p->mcwf.precision == QT_FP32_DERIV;
     s->batch.mix.alloc(1);
-    s->batch.queue.alloc(2);
+    s->batch.queue.alloc(4);
     hipCheck(hipHostMalloc((void**)&s->batch.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
     hipCheck(hipEventCreate(&s->batch.e0), "hipEventCreate");
     hipCheck(hipEventCreate(&s->batch.e1), "hipEventCreate");
@@ -1320,6 +1338,7 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
       for (auto* r : s->active) s->batch.mem.push_back(r->ens);
       s->batch.choose(-1, nullptr);
       batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
+      batch_check(&s->batch, nullptr);
       for (size_t m = 0; m < na; ++m) {
         Request* r = s->active[m];
         const int no = (int)r->sys->h->O.size();

@@ -70,12 +70,11 @@ constexpr int kMaxMembers = 16;
 
 struct alignas(16) MixDev {
   int n_members;
-  int block_items;              // trajectories run one per work group (members with team 0)
-  int wave_items;               // trajectories run one per wavefront (members with team 1)
-  int pad_;
-  int* queue;                   // [2] heads of the work-group and wavefront queues
-  int team[kMaxMembers];        // 0: one work group per trajectory, 1: one wavefront per trajectory
-  int first[kMaxMembers];       // first item of member m in its queue
+  int items[3];                 // trajectories per team kind: 0 work group, 1 wavefront, 2 quarter work group
+  int* queue;                   // [3] queue heads, one per team kind
+  int* err;                     // set to 2 when a quarter's counter barrier timed out
+  int team[kMaxMembers];        // team kind of member m
+  int first[kMaxMembers];       // first item of member m in its kind's queue
   SysDev S[kMaxMembers];
   EnsDev E[kMaxMembers];
 };

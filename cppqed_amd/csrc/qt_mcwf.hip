@@ -29,7 +29,9 @@ namespace qt {
 // the per-group trajectory engine
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32>
+// SUB = the team is NW wavefronts of a larger work group: its hand-overs use a counter
+// barrier in LDS instead of the work-group barrier.
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false>
 struct Engine {
   static constexpr int kRows = ROWS, kNW = NW;
   static constexpr int G = 64 * NW;
@@ -51,6 +53,12 @@ struct Engine {
   const int* hcolU;
   const T* hvalU;
   int whU, dimU;
+  // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
+  // launch's error flag (set when a wait times out)
+  unsigned* bar = nullptr;
+  mutable unsigned barTarget = 0;
+  mutable bool barBroken = false;
+  int* barErr = nullptr;
 
   // unsigned element offsets from a scalar base: loads address as scalar base + 32-bit vector offset
   __device__ __forceinline__ T coef(int w, int row) const { return hvalU[(unsigned)(w * dimU + row)]; }
@@ -93,7 +101,60 @@ struct Engine {
   __device__ __forceinline__ bool valid(int r) const { return r * G + lane < dimU; }
   // row block r holds part of the state for some lane of the team (uniform over the team)
   __device__ __forceinline__ bool live(int r) const { return r == 0 || r * G < dimU; }
-  static __device__ __forceinline__ void sync() { team_sync<NW>(); }
+  __device__ __forceinline__ void sync() const {
+    if constexpr (NW == 1) wave_sync();
+    else if constexpr (SUB) sub_sync();
+    else __syncthreads();
+  }
+
+  // Counter barrier of a SUB team.  Every wavefront adds one arrival and waits until the count
+  // reaches NW per barrier passed.  The wait is bounded: after ~0.1 s the wavefront flags the
+  // launch as failed and stops waiting at later barriers, so the kernel always ends.
+  __device__ __forceinline__ void sub_sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    barTarget += NW;
+    if ((lane & 63) == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (!barBroken) {
+      unsigned spins = 0;
+      while ((int)(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) - barTarget) < 0) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 22)) {
+          barBroken = true;
+          if ((lane & 63) == 0) atomicExch(barErr, 2);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+
+  // sum / max over the team's lanes; every lane gets the bitwise-same result
+  __device__ __forceinline__ double gsum(double v) const {
+    v = wave_sum(v);
+    if constexpr (NW == 1) return v;
+    else {
+      sync();
+      if ((lane & 63) == 0) red[lane >> 6] = v;
+      sync();
+      double s = 0.;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) s += red[i];
+      return s;
+    }
+  }
+  __device__ __forceinline__ double gmax(double v) const {
+    v = wave_max(v);
+    if constexpr (NW == 1) return v;
+    else {
+      sync();
+      if ((lane & 63) == 0) red[lane >> 6] = v;
+      sync();
+      double s = red[0];
+#pragma unroll
+      for (int i = 1; i < NW; ++i) s = fmax(s, red[i]);
+      return s;
+    }
+  }
 
   // k = (H_nH/i) v, with v already staged in xs
   __device__ __forceinline__ void gather(T k[ROWS]) const {
@@ -183,7 +244,7 @@ struct Engine {
         }
       }
     }
-    return sqrt(GroupReduce<NW>::max(e, red));
+    return sqrt(gmax(e));
   }
 
   __device__ __forceinline__ void derivative(const double2 x[ROWS], T k[ROWS]) const { stage(x); gather(k); }
@@ -202,7 +263,7 @@ struct Engine {
       double s = 0.;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (valid(r)) s += cabs2(jumpRow(m, r));
-      rt[m] = GroupReduce<NW>::sum(s, red);
+      rt[m] = gsum(s);
     }
   }
 
@@ -220,7 +281,7 @@ struct Engine {
       // conj(x) * acc
       s += im ? (x[r].x * acc.y - x[r].y * acc.x) : (x[r].x * acc.x + x[r].y * acc.y);
     }
-    return GroupReduce<NW>::sum(s, red);
+    return gsum(s);
   }
 };
 
@@ -232,11 +293,11 @@ struct Engine {
 // 16-wavefront team) free of register spills.
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32>
-struct LdsEngine : Engine<ROWS, NW, WH, F32> {
-  using B = Engine<ROWS, NW, WH, F32>;
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false>
+struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB> {
+  using B = Engine<ROWS, NW, WH, F32, SUB>;
   using T = typename B::T;
-  using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd;
+  using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd; using B::gsum; using B::gmax;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
 
   __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_)
@@ -244,10 +305,10 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
 
   // k = (H/i) X
   __device__ __forceinline__ void derivativeX(T k[ROWS]) const {
-    B::sync();
+    this->sync();
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (B::live(r)) xs[r * G + lane] = from_d<T>(X[r * G + lane]);
-    B::sync();
+    this->sync();
     B::gather(k);
   }
 
@@ -304,7 +365,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
         e = fmax(e, cabs2(xe) * (rc * rc));
       }
     }
-    const double err = sqrt(GroupReduce<NW>::max(e, red));
+    const double err = sqrt(gmax(e));
     if (err <= 1.0) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
@@ -329,7 +390,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32> {
       const double2 x = X[row];
       s += im ? (x.x * acc.y - x.y * acc.x) : (x.x * acc.x + x.y * acc.y);
     }
-    return GroupReduce<NW>::sum(s, red);
+    return gsum(s);
   }
 };
 
@@ -345,7 +406,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   auto bump = [&](int c, long long v) { if (lane == 0) cnt[c] += v; };
 
   T k1[ROWS];
-  Eng::sync();  // previous users of X / cnt are done
+  eng.sync();  // previous users of X / cnt are done
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = eng.valid(r) ? slot[r * G + lane] : make_double2(0., 0.);
   double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
@@ -353,7 +414,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   const unsigned long long ord0 = ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
-  Eng::sync();
+  eng.sync();
   const unsigned long long key = E.traj0 + (unsigned long long)traj;
   const int nj = S.nj;
   const double epsAbs = E.epsAbs, epsRel = E.epsRel;
@@ -383,10 +444,10 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     double s2 = 0.;  // renorm (StateVector.h:148-153)
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(X[r * G + lane]);
-    const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s2, eng.red));
+    const double inv = 1.0 / sqrt(eng.gsum(s2));
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.live(r)) { double2& x = X[r * G + lane]; x.x *= inv; x.y *= inv; }
-    Eng::sync();  // X complete before other lanes gather from it
+    eng.sync();  // X complete before other lanes gather from it
   };
 
   auto computeRates = [&]() {
@@ -394,10 +455,10 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
       double sacc = 0.;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
-      const double v = GroupReduce<NW>::sum(sacc, eng.red);
+      const double v = eng.gsum(sacc);
       if (lane == 0) rts[m] = v;
     }
-    Eng::sync();
+    eng.sync();
   };
 
   for (int k = 0; k < E.n_intervals; ++k) {
@@ -445,10 +506,10 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
             const double2 a = eng.valid(r) ? eng.jumpRow(no, r) : make_double2(0., 0.);
             y[r] = make_double2(a.x * inv, a.y * inv);
           }
-          Eng::sync();  // every lane has read X
+          eng.sync();  // every lane has read X
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = y[r];
-          Eng::sync();
+          eng.sync();
           if (lane == 0) {
             const long long nJ = cnt[kJumps];
             if (nJ < E.jcap) {
@@ -470,7 +531,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) slot[r * G + lane] = X[r * G + lane];
-  Eng::sync();
+  eng.sync();
   if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
   if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
 }
@@ -501,7 +562,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   const unsigned long long ord0 = ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
-  Eng::sync();
+  eng.sync();
   const unsigned long long key = E.traj0 + (unsigned long long)traj;
   const int nj = S.nj;
   const double epsAbs = E.epsAbs, epsRel = E.epsRel;
@@ -537,7 +598,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     double s2 = 0.;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(x[r]);
-    const double inv = 1.0 / sqrt(GroupReduce<NW>::sum(s2, eng.red));
+    const double inv = 1.0 / sqrt(eng.gsum(s2));
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
   };
@@ -548,10 +609,10 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
       double sacc = 0.;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
-      const double v = GroupReduce<NW>::sum(sacc, eng.red);
+      const double v = eng.gsum(sacc);
       if (lane == 0) rts[m] = v;
     }
-    Eng::sync();
+    eng.sync();
   };
 
   for (int k = 0; k < E.n_intervals; ++k) {
@@ -630,7 +691,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
 
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) E.psi[(size_t)traj * S.dim + r * G + lane] = x[r];
-  Eng::sync();
+  eng.sync();
   if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
   if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
 }
@@ -682,31 +743,53 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   }
 }
 
+// One-wavefront trajectory runner of the mixed kernel, kept out of line: it gets its own
+// register allocation, so the spills of the large-state paths do not reach it.
+template <class Eng>
+__device__ __noinline__ void run_wave_traj(const SysDev* S, const EnsDev* E, double2* ps, double2* pc, double* rts,
+                                           long long* cnt, int lane, int traj) {
+  Eng eng(*S, reinterpret_cast<typename Eng::T*>(ps), ps, nullptr, lane);
+  advance_traj<Eng, true>(*S, *E, eng, traj, pc, rts, cnt);
+}
+
 // ---------------------------------------------------------------------------
 // mixed batch: trajectories of up to kMaxMembers ensembles (different systems) in one
-// persistent launch.  Members of team kind 0 run one trajectory per work group (NWB
-// wavefronts, RB rows per lane), members of kind 1 one trajectory per wavefront (RW rows per
-// lane).  Work groups first drain the group queue together, then every wavefront drains the
-// wave queue on its own (wave-level hand-overs only, no work-group barrier).
+// persistent launch.  Three team kinds share one work group of NWB wavefronts:
+//   kind 0: the whole work group owns a trajectory (RB rows per lane, psi in LDS);
+//   kind 2: each quarter (NWB/4 wavefronts) owns a trajectory (RB rows per lane, psi in LDS,
+//           counter barriers in LDS);
+//   kind 1: each wavefront owns a trajectory (RW rows per lane, psi in registers).
+// Work groups first drain the kind-0 queue together, then each quarter drains the kind-2
+// queue on its own, then each wavefront the kind-1 queue.  A quarter's LDS region holds its
+// four wavefronts' kind-1 regions, so a quarter that moves on never touches LDS another
+// quarter still uses.
 // ---------------------------------------------------------------------------
 
 template <int NWB, int RB, int WHB, int RW, int WHW, bool F32>
 __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
+  constexpr int NWQ = NWB / 4;
   using EngB = LdsEngine<RB, NWB, WHB, F32>;
+  using EngQ = LdsEngine<RB, NWQ, WHB, F32, true>;
   using EngW = Engine<RW, 1, WHW, F32>;
   using TB = typename EngB::T;
   using TW = typename EngW::T;
-  constexpr int GB = 64 * NWB;
+  constexpr int GB = 64 * NWB, GQ = 64 * NWQ;
   constexpr size_t kBlockBytes = (size_t)RB * GB * (sizeof(double2) + sizeof(TB));
   constexpr size_t kWaveEach = (size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8;
-  constexpr size_t kWaveBytes = kWaveEach * NWB;
-  constexpr size_t kBytes = kBlockBytes > kWaveBytes ? kBlockBytes : kWaveBytes;
+  constexpr size_t kQuarterOwn = (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ + kMaxJumps + kCounters) * 8;
+  constexpr size_t kQuarterRaw = kQuarterOwn > kWaveEach * NWQ ? kQuarterOwn : kWaveEach * NWQ;
+  constexpr size_t kQuarter = (kQuarterRaw + 15) / 16 * 16;
+  constexpr size_t kBytes = kBlockBytes > 4 * kQuarter ? kBlockBytes : 4 * kQuarter;
   __shared__ MixDev M;
   __shared__ __align__(16) unsigned char pool[kBytes];
   __shared__ double red[NWB];
   __shared__ double rtsB[kMaxJumps];
   __shared__ long long cntB[kCounters];
   __shared__ int shItem;
+  // quarter teams' claimed items and barrier counters, outside the pool: a wavefront that has
+  // passed its quarter's last barrier reuses the pool while the others may still read them
+  __shared__ int qslot[4];
+  __shared__ unsigned qbar[4];
 
   {  // copy the member descriptors to LDS (16-byte words)
     const int4* src = reinterpret_cast<const int4*>(Mg);
@@ -715,8 +798,15 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   }
   __syncthreads();
 
-  // phase 1: one trajectory per work group (psi in LDS)
-  {
+  auto member = [&](int kind, int item) {  // members of one kind are queued in member order
+    int m = -1;
+    for (int k = 0; k < M.n_members; ++k)
+      if (M.team[k] == kind && M.first[k] <= item) m = k;
+    return m;
+  };
+
+  // phase 1: one trajectory per work group
+  if (M.items[0] > 0) {
     double2* X = reinterpret_cast<double2*>(pool);
     TB* xs = reinterpret_cast<TB*>(X + RB * GB);
     for (;;) {
@@ -724,20 +814,51 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       if (threadIdx.x == 0) shItem = atomicAdd(M.queue, 1);
       __syncthreads();
       const int item = shItem;
-      if (item >= M.block_items) break;
-      // members of one kind are laid out in member order in their queue
-      int m = -1;
-      for (int k = 0; k < M.n_members; ++k)
-        if (M.team[k] == 0 && M.first[k] <= item) m = k;
+      if (item >= M.items[0]) break;
+      const int m = member(0, item);
       EngB eng(M.S[m], xs, X, red, (int)threadIdx.x);
       advance_traj_lds(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
     }
+    __syncthreads();
   }
 
-  // phase 2: one trajectory per wavefront
+  const int q = threadIdx.x / GQ;
+  unsigned char* qbase = pool + kQuarter * q;
+
+  // phase 2: one trajectory per quarter of the work group
+  if (M.items[2] > 0) {
+    const int qlane = threadIdx.x % GQ;
+    double2* X = reinterpret_cast<double2*>(qbase);
+    TB* xs = reinterpret_cast<TB*>(X + RB * GQ);
+    double* qred = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(xs) + RB * GQ * sizeof(TB));
+    double* rts = qred + NWQ;
+    long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
+    int* slot = qslot + q;
+    unsigned* bar = qbar + q;
+    if (qlane == 0) *bar = 0u;
+    __syncthreads();  // every quarter starts phase 2 together, with its counter at zero
+    EngQ eng(M.S[0], xs, X, qred, qlane);  // re-targeted below; holds the barrier state
+    eng.bar = bar;
+    eng.barErr = M.err;
+    for (;;) {
+      eng.sync();
+      if (qlane == 0) *slot = atomicAdd(M.queue + 2, 1);
+      eng.sync();
+      const int item = *slot;
+      if (item >= M.items[2]) break;
+      const int m = member(2, item);
+      EngQ e2(M.S[m], xs, X, qred, qlane);
+      e2.bar = bar; e2.barErr = M.err; e2.barTarget = eng.barTarget; e2.barBroken = eng.barBroken;
+      advance_traj_lds(M.S[m], M.E[m], e2, item - M.first[m], rts, cnt);
+      eng.barTarget = e2.barTarget; eng.barBroken = e2.barBroken;
+    }
+    eng.sync();  // the quarter's LDS is free only when all of its wavefronts are done with it
+  }
+
+  // phase 3: one trajectory per wavefront (inside its quarter's region)
   {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned char* mine = pool + kWaveEach * w;
+    unsigned char* mine = qbase + kWaveEach * (w % NWQ);
     double2* ps = reinterpret_cast<double2*>(mine);
     double2* pc = ps + RW * 64;
     double* rts = reinterpret_cast<double*>(pc + RW * 64);
@@ -746,12 +867,9 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       int item = 0;
       if (lane == 0) item = atomicAdd(M.queue + 1, 1);
       item = __builtin_amdgcn_readfirstlane(item);
-      if (item >= M.wave_items) break;
-      int m = -1;
-      for (int k = 0; k < M.n_members; ++k)
-        if (M.team[k] == 1 && M.first[k] <= item) m = k;
-      EngW eng(M.S[m], reinterpret_cast<TW*>(ps), ps, nullptr, lane);
-      advance_traj<EngW, true>(M.S[m], M.E[m], eng, item - M.first[m], pc, rts, cnt);
+      if (item >= M.items[1]) break;
+      const int m = member(1, item);
+      run_wave_traj<EngW>(&M.S[m], &M.E[m], ps, pc, rts, cnt, lane, item - M.first[m]);
       wave_sync();
     }
   }
@@ -834,7 +952,7 @@ __global__ void __launch_bounds__(64 * NW) plain_post_kernel(SysDev S, double2* 
   double s = 0.;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) { x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.); s += cabs2(x[r]); }
-  const double nrm = sqrt(GroupReduce<NW>::sum(s, red));
+  const double nrm = sqrt(eng.gsum(s));
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) { x[r].x /= nrm; x[r].y /= nrm; if (eng.valid(r)) psi[r * G + threadIdx.x] = x[r]; }
   eng.stagePsi(x);

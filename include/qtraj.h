@@ -134,14 +134,15 @@ int qt_mcwf_kernel_ms(qt_ens* ens, double* total_ms, int* launches);
 /* A mixed batch advances several ensembles (each its own system, trajectory count, state and
  * parameters) in ONE persistent launch: the reference runs each ensemble's trajectories
  * serially (trajectory::Ensemble::advance, StochasticTrajectory.h:220) and separate ensembles
- * one after the other.  Each member runs either one trajectory per work group (team 0) or one
- * trajectory per wavefront (team 1); the layout variant sets the work-group width and the rows
- * per lane of both teams.  All members share one precision. */
+ * one after the other.  Each member runs one trajectory per work group (team 0), per quarter
+ * of a work group (team 2) or per wavefront (team 1); the layout variant sets the work-group
+ * width and the rows per lane of the teams.  All members share one precision. */
 typedef struct qt_batch_s qt_batch;
 
 typedef struct {
   int variant;                 /* index into qt_batch_variants' table; -1: first that fits */
-  int team[16];                /* per member: 0 work group, 1 wavefront, -1 chosen by dimension */
+  int team[16];                /* per member: 0 work group, 1 wavefront, 2 quarter of a work group,
+                                  -1 chosen by dimension (smallest team that holds the state) */
 } qt_batch_params;
 
 /* Compiled layout variants, 5 ints each: waves per work group, rows per lane (team 0), ELL

@@ -28,7 +28,7 @@ def test_mixed_fp64_trajectories_match_oracle():
     mem = _members(counts, Q.QT_FP64)
     b = Q.Batch(mem)
     v, team = b.layout()
-    assert team == [0, 0, 1, 1], team
+    assert team == [2, 0, 1, 1], team
     rows = b.advance(iv)
     for i, ((name, spec, idx), e) in enumerate(zip(MIXED, mem)):
         so = O.System(spec)
@@ -56,7 +56,7 @@ def test_mixed_equals_members_alone():
     iv = np.full(10, 0.1)
     mem = _members(counts, Q.QT_FP32_DERIV)
     rows = Q.Batch(mem).advance(iv)
-    layouts = [(4, 16), (4, 16), (1, 1), (1, 1)]
+    layouts = [(4, 4), (4, 16), (1, 1), (1, 1)]   # the engines of team kinds 2, 0, 1, 1
     for i, ((name, spec, idx), n) in enumerate(zip(MIXED, counts)):
         s = Q.System(spec)
         alone = Q.Ensemble(s, n, basis(s.dim, idx), Q.mcwf_pars(seed=SEED + i, precision=Q.QT_FP32_DERIV, layout=layouts[i]))
