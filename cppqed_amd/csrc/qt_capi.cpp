@@ -21,13 +21,13 @@
 
 namespace qt {
 bool layout_supported(int rows, int nw, int wh);
-bool batch_supported(int rows, int nw, int wh, int minw);
-int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32);
+bool batch_supported(int rows, int nw, int wh, int minw, bool td);
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td);
 bool sub_supported(int g, int rows, int wh);
 int sub_blocks_per_cu(int g, int rows, int wh, bool f32);
 hipError_t launch_sub(int g, int rows, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st);
-hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E,
-                        hipStream_t st);
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, int grid, const SysDev& S,
+                        const EnsDev& E, hipStream_t st);
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
                              int n_traj, int cap, double* draws, hipStream_t st);
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw);
@@ -41,8 +41,8 @@ hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const Pla
 hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st);
 hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st);
 hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st);
-hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, const SysDev& S, const double2* psi, double2* dpsidt,
-                             int n, hipStream_t st);
+hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const SysDev& S, const double2* psi,
+                             double2* dpsidt, int n, double tau, hipStream_t st);
 struct MasterDev;
 bool master_supported(int nb);
 hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st);
@@ -163,8 +163,16 @@ void check_params(const qt_mcwf_params* p) {
 struct qt_sys {
   std::unique_ptr<qt::System> h;
   bool uploaded = false;
-  DevBuf<int> hcol, jcol, ocol, oimag, owid;
-  DevBuf<double2> hval, jval, oval;
+  bool interactionPicture() const { return h->H.timeDependent() || h->hasEx; }
+  // the device kernels take an interaction-picture system when its frequencies fit the table
+  void checkPictureSupported() {
+    upload();
+    if (h->H.timeDependent() && nfreq > qt::kMaxFreqs)
+      throw Fail(QT_ENOTSUP, "interaction picture with more than 32 distinct frequencies");
+  }
+  DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx;
+  DevBuf<double2> hval, jval, oval, freq, exz;
+  int nfreq = 0;          // distinct interaction-picture frequencies of H (0: Schroedinger picture)
   DevBuf<float2> hval32;
   qt::SysDev dev{};
 
@@ -221,6 +229,34 @@ struct qt_sys {
     }
     if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); im.push_back(0); ow.push_back(0); }
     ocol.upload(c); oval.upload(v); oimag.upload(im); owid.upload(ow);
+    // interaction picture: distinct entry frequencies (index 0 = 0) and per-entry indices,
+    // column-major like the values; the Exact diagonal's exponents per basis state
+    if (s.H.timeDependent()) {
+      std::vector<qt::cd> fl{qt::cd(0.)};
+      std::vector<int> fi((size_t)d * s.H.width, 0);
+      for (int w = 0; w < s.H.width; ++w)
+        for (int r = 0; r < d; ++r) {
+          const qt::cd f = s.H.freqs[(size_t)r * s.H.width + w];
+          size_t k = 0;
+          while (k < fl.size() && fl[k] != f) ++k;
+          if (k == fl.size()) fl.push_back(f);
+          fi[(size_t)w * d + r] = (int)k;
+        }
+      nfreq = (int)fl.size();
+      if (nfreq <= qt::kMaxFreqs) {
+        std::vector<double2> fv;
+        for (auto& f : fl) fv.push_back(make_double2(f.real(), f.imag()));
+        freq.upload(fv);
+        hfidx.upload(fi);
+      }
+    }
+    if (s.hasEx) {
+      std::vector<double2> z;
+      for (auto& c : s.exactZ) z.push_back(make_double2(c.real(), c.imag()));
+      exz.upload(z);
+    }
+    dev.nfreq = nfreq <= qt::kMaxFreqs ? nfreq : 0;
+    dev.freq = freq.p; dev.hfidx = hfidx.p; dev.exz = s.hasEx ? exz.p : nullptr;
     dev.dim = d; dev.wh = s.H.width;
     dev.hcol = hcol.p; dev.hval = hval.p; dev.hval32 = hval32.p;
     dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
@@ -236,7 +272,7 @@ struct qt_ens {
   LayoutChoice lay{};
   double time = 0.;           // common time of all trajectories (they all stop at each endTime)
   DevBuf<double2> psi, psiInit;
-  DevBuf<double> t, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt, draws;
+  DevBuf<double> t, t0, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt, draws;
   DevBuf<unsigned long long> ord;
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
@@ -268,7 +304,7 @@ struct qt_ens {
 
   qt::EnsDev view() {
     qt::EnsDev E{};
-    E.n_traj = n; E.psi = psi.p; E.t = t.p; E.dtDid = dtDid.p; E.dtTry = dtTry.p; E.ordinal = ord.p;
+    E.n_traj = n; E.psi = psi.p; E.t = t.p; E.t0 = t0.p; E.dtDid = dtDid.p; E.dtTry = dtTry.p; E.ordinal = ord.p;
     E.seed = p.seed; E.traj0 = p.traj0; E.counters = cnt.p; E.jt = jt.p; E.jch = jch.p; E.jcap = p.jump_log_cap;
     E.epsAbs = p.epsRel; E.epsRel = p.epsAbs;  // ODE.h:205 hands (epsRel, epsAbs) to make_controlled(abs, rel, ...)
     E.dpLimit = p.dpLimit; E.overshootTol = p.overshootTol;
@@ -312,12 +348,12 @@ std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& p
   return r;
 }
 
-int resident_groups(const LayoutChoice& L, bool f32) {
+int resident_groups(const LayoutChoice& L, bool f32, bool td) {
   int dev = 0;
   hipCheck(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
   hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  int per_cu = L.glanes > 0 ? qt::sub_blocks_per_cu(L.glanes, L.rows, L.wh, f32) : qt::batch_blocks_per_cu(L.rows, L.nw, L.wh, L.occ, f32);
+  int per_cu = L.glanes > 0 ? qt::sub_blocks_per_cu(L.glanes, L.rows, L.wh, f32) : qt::batch_blocks_per_cu(L.rows, L.nw, L.wh, L.occ, f32, td);
   if (per_cu < 1) per_cu = 1;
   return prop.multiProcessorCount * per_cu;
 }
@@ -395,14 +431,13 @@ int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* va
 
 int qt_apply_heff(qt_sys* sys, double t, double t0, const double* psi, double* dpsidt, int n, int precision,
                   qt_stream stream) {
-  (void)t; (void)t0;
   return guard([&] {
     if (!sys || !psi || !dpsidt || n <= 0) throw Fail(QT_EINVAL, "bad arguments");
-    if (sys->h->H.timeDependent() || sys->h->hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
-    sys->upload();
+    sys->checkPictureSupported();
     LayoutChoice L = pick_layout(sys->h->dim, sys->h->H.width, 0, 0);
-    hipCheck(qt::launch_apply_heff(L.rows, L.nw, L.wh, precision == QT_FP32_DERIV, sys->dev, (const double2*)psi,
-                                   (double2*)dpsidt, n, (hipStream_t)stream), "apply_heff launch");
+    hipCheck(qt::launch_apply_heff(L.rows, L.nw, L.wh, precision == QT_FP32_DERIV, sys->interactionPicture(), sys->dev,
+                                   (const double2*)psi,
+                                   (double2*)dpsidt, n, t - t0, (hipStream_t)stream), "apply_heff launch");
   });
 }
 
@@ -411,9 +446,10 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     if (!sys || !psi0 || !out || n_traj <= 0) throw Fail(QT_EINVAL, "bad arguments");
     check_params(p);
     const qt::System& s = *sys->h;
-    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
-    sys->upload();
+    sys->checkPictureSupported();
+    if (sys->interactionPicture() && p->layout_group_lanes > 0)
+      throw Fail(QT_ENOTSUP, "sub-wave layouts run the Schroedinger picture only");
     std::unique_ptr<qt_ens> e(new qt_ens);
     e->sys = sys; e->n = n_traj; e->p = *p;
     if (e->p.jump_log_cap < 1) e->p.jump_log_cap = 1;
@@ -423,9 +459,18 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
                         p->layout_rows);
     } else {
       e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
-      e->lay.occ = p->layout_occupancy > 0 ? p->layout_occupancy : default_occupancy(e->lay);
-      if (!qt::batch_supported(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ))
-        throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
+      if (sys->interactionPicture()) {
+        // interaction-picture variants: no occupancy hint, the ELL width may round up
+        e->lay.occ = 1;
+        int w = e->lay.wh;
+        while (w <= 16 && !qt::batch_supported(e->lay.rows, e->lay.nw, w, 1, true)) w *= 2;
+        if (w > 16) throw Fail(QT_ENOTSUP, "no interaction-picture variant for this layout");
+        e->lay.wh = w;
+      } else {
+        e->lay.occ = p->layout_occupancy > 0 ? p->layout_occupancy : default_occupancy(e->lay);
+        if (!qt::batch_supported(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, false))
+          throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
+      }
     }
     const long d = s.dim;
     std::vector<qt::cd> psi(d);
@@ -445,6 +490,7 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->psi.upload(rep);
     e->psiInit.upload(rep);
     e->t.upload(std::vector<double>(n_traj, 0.));
+    e->t0.upload(std::vector<double>(n_traj, 0.));
     e->dtDid.upload(std::vector<double>(n_traj, 0.));
     e->dtTry.upload(std::vector<double>(n_traj, dtTry));
     e->dtTryInit.upload(std::vector<double>(n_traj, dtTry));
@@ -455,7 +501,7 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->work.alloc(1);
     e->errflag.upload(std::vector<int>(1, 0));
     const int perGroup = e->lay.glanes > 0 ? 64 / e->lay.glanes : 1;  // trajectories per work group
-    e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV));
+    e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV, sys->interactionPicture()));
     hipCheck(hipEventCreate(&e->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&e->e1), "hipEventCreate");
     *out = e.release();
@@ -498,7 +544,8 @@ static void ens_kernel(qt_ens* e, const qt::EnsDev& E, hipStream_t st) {
     hipCheck(qt::launch_sub(e->lay.glanes, e->lay.rows, e->lay.wh, e->p.precision == QT_FP32_DERIV, e->grid, e->sys->dev, E, st),
              "mcwf sub-wave launch");
   else
-    hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV, e->grid,
+    hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV,
+                              e->sys->interactionPicture(), e->grid,
                               e->sys->dev, E, st),
              "mcwf batch launch");
 }
@@ -584,6 +631,7 @@ int qt_mcwf_reset(qt_ens* e, qt_stream stream) {
     hipCheck(hipMemcpyAsync(e->psi.p, e->psiInit.p, n * d * sizeof(double2), hipMemcpyDeviceToDevice, st), "reset psi");
     hipCheck(hipMemcpyAsync(e->dtTry.p, e->dtTryInit.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), "reset dtTry");
     hipCheck(hipMemsetAsync(e->t.p, 0, n * sizeof(double), st), "reset t");
+    hipCheck(hipMemsetAsync(e->t0.p, 0, n * sizeof(double), st), "reset t0");
     hipCheck(hipMemsetAsync(e->dtDid.p, 0, n * sizeof(double), st), "reset dtDid");
     hipCheck(hipMemsetAsync(e->ord.p, 0, n * sizeof(unsigned long long), st), "reset ordinal");
     hipCheck(hipMemsetAsync(e->cnt.p, 0, n * qt::kCounters * sizeof(long long), st), "reset counters");
@@ -769,6 +817,8 @@ int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params
     std::unique_ptr<qt_batch> b(new qt_batch);
     for (int m = 0; m < n_members; ++m) {
       if (!members[m]) throw Fail(QT_EINVAL, "null member");
+      if (members[m]->sys->interactionPicture())
+        throw Fail(QT_ENOTSUP, "mixed batches run the Schroedinger picture only");
       b->mem.push_back(members[m]);
     }
     b->f32 = members[0]->p.precision == QT_FP32_DERIV;
@@ -947,7 +997,7 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     for (int k = 0; k < n_points; ++k) {
       const qt::System& s = *systems[k]->h;
       if (s.dim != d || (int)s.J.size() != nj) throw Fail(QT_EINVAL, "sweep points must share dimension and jump channels");
-      if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+      if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the Master sweep runs the Schroedinger picture only");
       for (auto& j : s.J) wj = std::max(wj, j.width);
     }
     std::unique_ptr<qt_master> m(new qt_master);
@@ -1098,7 +1148,7 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     if (!sys || !psi0 || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
     check_params(p);
     const qt::System& s = *sys->h;
-    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the plain form runs the Schroedinger picture only");
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
     sys->upload();
     LayoutChoice L = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
@@ -1281,7 +1331,7 @@ int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0, 
   return guard([&] {
     if (!s || !sys || !psi0 || n_traj <= 0 || !(T > 0) || !(Dt > 0)) throw Fail(QT_EINVAL, "bad arguments");
     const qt::System& h = *sys->h;
-    if (h.H.timeDependent() || h.hasEx) throw Fail(QT_ENOTSUP, "interaction picture is not available on the device");
+    if (h.H.timeDependent() || h.hasEx) throw Fail(QT_ENOTSUP, "the request loop runs the Schroedinger picture only");
     std::unique_ptr<Request> r(new Request);
     r->id = s->nextId++;
     r->sys = sys; r->n = n_traj;

@@ -24,6 +24,12 @@ __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
   c.y = fmaf(a.x, b.y, c.y); c.y = fmaf(a.y, b.x, c.y);
   return c;
 }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
 __device__ __forceinline__ double2 to_d(double2 a) { return a; }
 __device__ __forceinline__ double2 to_d(float2 a) { return make_double2((double)a.x, (double)a.y); }
 template <typename T> __device__ __forceinline__ T from_d(double2 a);

@@ -8,6 +8,7 @@ namespace qt {
 
 constexpr int kMaxJumps = 8;      // jump channels handled by the batched kernel
 constexpr int kCounters = 8;      // per-trajectory counters, see Counter
+constexpr int kMaxFreqs = 32;     // distinct interaction-picture frequencies of one H_nH/i
 
 enum Counter {
   kMcwfSteps = 0,   // MCWF_Trajectory::step calls
@@ -35,6 +36,12 @@ struct SysDev {
   const double2* oval;
   const int* oimag;        // [no]
   const int* owid;         // [no]: entries of observable q per row (<= wo)
+  // interaction picture (nfreq = 0 in the Schroedinger picture): H entry (w, row) at time t is
+  // hval * exp(freq[hfidx] (t - t0)); after each ODE step psi_row *= exp(-exz[row] (t - t0))
+  int nfreq;
+  const double2* freq;     // [nfreq], freq[0] = 0
+  const int* hfidx;        // [wh][dim]
+  const double2* exz;      // [dim] (null without an Exact part)
 };
 
 // Device view of an MCWF ensemble and the per-launch work description.
@@ -44,6 +51,7 @@ struct EnsDev {
   double* t;                    // [n_traj]
   double* dtDid;
   double* dtTry;
+  double* t0;                   // time at which the pictures coincide (interaction picture)
   unsigned long long* ordinal;  // RNG counter (MCWF step ordinal)
   unsigned long long seed;
   unsigned long long traj0;     // global index of trajectory 0 (RNG key)

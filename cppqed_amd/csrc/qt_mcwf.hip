@@ -30,10 +30,13 @@ namespace qt {
 // ---------------------------------------------------------------------------
 
 // SUB = the team is NW wavefronts of a larger work group: its hand-overs use a counter
-// barrier in LDS instead of the work-group barrier.
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false>
+// barrier in LDS instead of the work-group barrier.  TD = interaction-picture system (H entries
+// rotate with the picture, psi gets the Exact U after each ODE step); the Schroedinger-picture
+// instantiations carry none of that code.
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false>
 struct Engine {
   static constexpr int kRows = ROWS, kNW = NW;
+  static constexpr bool kTD = TD;
   static constexpr int G = 64 * NW;
   // row coefficients live in registers when they fit; otherwise they are read from
   // the (L2-resident, shared by every trajectory) column-major ELL table
@@ -59,19 +62,35 @@ struct Engine {
   mutable unsigned barTarget = 0;
   mutable bool barBroken = false;
   int* barErr = nullptr;
+  // interaction picture: the H entry (w, row) at picture time tau is hval * ph[f] with
+  // ph[f] = exp(freq[f] tau), the phasor table in LDS rebuilt for each stage time; the
+  // register-cached column carries f in its top byte
+  bool td = false;
+  T* ph = nullptr;
+  const double2* freqU = nullptr;
+  const int* hfidxU = nullptr;
+  int nfU = 0;
 
   // unsigned element offsets from a scalar base: loads address as scalar base + 32-bit vector offset
   __device__ __forceinline__ T coef(int w, int row) const { return hvalU[(unsigned)(w * dimU + row)]; }
   __device__ __forceinline__ int col(int w, int row) const { return hcolU[(unsigned)(w * dimU + row)]; }
 
-  // lane_ = index of this thread within its team of NW wavefronts
-  __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_)
+  // lane_ = index of this thread within its team of NW wavefronts; ph_ = LDS phasor table
+  // [kMaxFreqs] (needed for interaction-picture systems only)
+  __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
       : S(s), xs(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
     if constexpr (F32) hvalU = uniform(s.hval32);
     else hvalU = uniform(s.hval);
     whU = uniform(s.wh);
     dimU = uniform(s.dim);
+    if constexpr (TD) {
+      nfU = uniform(s.nfreq);
+      td = nfU > 0 && ph_ != nullptr;
+      if (td) { ph = ph_; freqU = uniform(s.freq); hfidxU = uniform(s.hfidx); }
+    } else {
+      (void)ph_;
+    }
     if constexpr (kRegs) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
@@ -80,6 +99,7 @@ struct Engine {
         for (int w = 0; w < WH; ++w) {
           if (row < dimU && w < whU) {
             hcol[r][w] = col(w, row);
+            if constexpr (TD) if (td) hcol[r][w] |= hfidxU[(unsigned)(w * dimU + row)] << 24;
             hval[r][w] = coef(w, row);
           } else {
             hcol[r][w] = 0;
@@ -158,6 +178,7 @@ struct Engine {
 
   // k = (H_nH/i) v, with v already staged in xs
   __device__ __forceinline__ void gather(T k[ROWS]) const {
+    if constexpr (TD) if (td) { gatherTD(k); return; }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       T acc = from_d<T>(make_double2(0., 0.));
@@ -174,6 +195,46 @@ struct Engine {
       }
       k[r] = acc;
     }
+  }
+
+  // the same at the picture time of the last setTime: entry = hval * ph[f]
+  __device__ __forceinline__ void gatherTD(T k[ROWS]) const {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      T acc = from_d<T>(make_double2(0., 0.));
+      if (!live(r)) { k[r] = acc; continue; }
+      if constexpr (kRegs) {
+#pragma unroll
+        for (int w = 0; w < WH; ++w) {
+          const T c = cmul(hval[r][w], ph[(unsigned)hcol[r][w] >> 24]);
+          acc = cfma(c, xs[hcol[r][w] & 0xFFFFFF], acc);
+        }
+      } else {
+        const int row = r * G + lane;
+        if (row < dimU) {
+          for (int w = 0; w < whU; ++w) {
+            const unsigned o = (unsigned)(w * dimU + row);
+            acc = cfma(cmul(hvalU[o], ph[hfidxU[o]]), xs[hcolU[o]], acc);
+          }
+        }
+      }
+      k[r] = acc;
+    }
+  }
+
+  // rebuilds the phasor table for picture time tau = t_stage - t0 (interaction picture only)
+  __device__ __forceinline__ void setTime(double tau) const {
+    if constexpr (!TD) { (void)tau; return; }
+    if (!td) return;
+    sync();  // previous readers of the table are done
+    if (lane < nfU) {
+      const double2 f = freqU[lane];
+      const double e = exp(f.x * tau);
+      double sn, cs;
+      sincos(f.y * tau, &sn, &cs);
+      ph[lane] = from_d<T>(make_double2(e * cs, e * sn));
+    }
+    sync();
   }
 
   // xs and ps may share storage: both hand-overs start with a barrier, so the readers of the
@@ -194,32 +255,34 @@ struct Engine {
 
   // One RKCK54 try: returns the odeint error norm; xn is the 5th-order proposal.
   // k1 must hold (H/i) x on entry.
+  // t, t0: time at the start of the try and picture origin (stage times t + c_i dt - t0 set the
+  // interaction-picture phasors)
   __device__ double tryStep(const double2 x[ROWS], const T k1[ROWS], double dt, double epsAbs, double epsRel,
-                            double2 xn[ROWS], long long& derivs) const {
+                            double2 xn[ROWS], long long& derivs, double t = 0., double t0 = 0.) const {
     T k2[ROWS], k3[ROWS], k4[ROWS], k5[ROWS], k6[ROWS];
     double2 v[ROWS];
     const double s21 = CK::a21 * dt;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(s21, kd(k1[r]), x[r]);
-    stage(v); gather(k2);
+    stage(v); setTime((t + CK::c2 * dt) - t0); gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r])); }
-    stage(v); gather(k3);
+    stage(v); setTime((t + CK::c3 * dt) - t0); gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (live(r)) v[r] = axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r]))); }
-    stage(v); gather(k4);
+    stage(v); setTime((t + CK::c4 * dt) - t0); gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (live(r)) v[r] = axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r])))); }
-    stage(v); gather(k5);
+    stage(v); setTime((t + CK::c5 * dt) - t0); gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (live(r)) v[r] = axpy(q5, kd(k5[r]), axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), x[r]))))); }
-    stage(v); gather(k6);
+    stage(v); setTime((t + CK::c6 * dt) - t0); gather(k6);
     derivs += 5;
     double e = 0.;
     {
@@ -247,7 +310,11 @@ struct Engine {
     return sqrt(gmax(e));
   }
 
-  __device__ __forceinline__ void derivative(const double2 x[ROWS], T k[ROWS]) const { stage(x); gather(k); }
+  __device__ __forceinline__ void derivative(const double2 x[ROWS], T k[ROWS], double tau = 0.) const {
+    stage(x);
+    setTime(tau);
+    gather(k);
+  }
 
   // y = J_m psi for the rows of this lane (psi staged in ps)
   __device__ __forceinline__ double2 jumpRow(int m, int r) const {
@@ -293,22 +360,23 @@ struct Engine {
 // 16-wavefront team) free of register spills.
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false>
-struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB> {
-  using B = Engine<ROWS, NW, WH, F32, SUB>;
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false>
+struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
+  using B = Engine<ROWS, NW, WH, F32, SUB, TD>;
   using T = typename B::T;
   using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd; using B::gsum; using B::gmax;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
 
-  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_)
-      : B(s, xs_, X_, red_, lane_), X(X_) {}
+  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_, T* ph_ = nullptr)
+      : B(s, xs_, X_, red_, lane_, ph_), X(X_) {}
 
-  // k = (H/i) X
-  __device__ __forceinline__ void derivativeX(T k[ROWS]) const {
+  // k = (H/i) X at picture time tau
+  __device__ __forceinline__ void derivativeX(T k[ROWS], double tau = 0.) const {
     this->sync();
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (B::live(r)) xs[r * G + lane] = from_d<T>(X[r * G + lane]);
     this->sync();
+    this->setTime(tau);
     B::gather(k);
   }
 
@@ -316,36 +384,37 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB> {
 
   // One RKCK54 try from X with k1 = (H/i) X; returns the odeint error norm and, when the try is
   // accepted (norm <= 1), replaces the lane's rows of X by the 5th-order solution.
-  __device__ double tryStepX(const T k1[ROWS], double dt, double epsAbs, double epsRel, long long& derivs) const {
+  __device__ double tryStepX(const T k1[ROWS], double dt, double epsAbs, double epsRel, long long& derivs,
+                             double t = 0., double t0 = 0.) const {
     T k2[ROWS], k3[ROWS], k4[ROWS], k5[ROWS], k6[ROWS];
     double2 v[ROWS];
     auto xr = [&](int r) { return X[r * G + lane]; };
     const double s21 = CK::a21 * dt;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(s21, kd(k1[r]), xr(r));
-    B::stage(v); B::gather(k2);
+    B::stage(v); this->setTime((t + CK::c2 * dt) - t0); B::gather(k2);
     { const double q1 = CK::a31 * dt, q2 = CK::a32 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (B::live(r)) v[r] = axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r))); }
-    B::stage(v); B::gather(k3);
+    B::stage(v); this->setTime((t + CK::c3 * dt) - t0); B::gather(k3);
     { const double q1 = CK::a41 * dt, q2 = CK::a42 * dt, q3 = CK::a43 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r)) v[r] = axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r)))); }
-    B::stage(v); B::gather(k4);
+    B::stage(v); this->setTime((t + CK::c4 * dt) - t0); B::gather(k4);
     { const double q1 = CK::a51 * dt, q2 = CK::a52 * dt, q3 = CK::a53 * dt, q4 = CK::a54 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r))
           v[r] = axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]), axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r))))); }
-    B::stage(v); B::gather(k5);
+    B::stage(v); this->setTime((t + CK::c5 * dt) - t0); B::gather(k5);
     { const double q1 = CK::a61 * dt, q2 = CK::a62 * dt, q3 = CK::a63 * dt, q4 = CK::a64 * dt, q5 = CK::a65 * dt;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
         if (B::live(r))
           v[r] = axpy(q5, kd(k5[r]), axpy(q4, kd(k4[r]), axpy(q3, kd(k3[r]),
                  axpy(q2, kd(k2[r]), axpy(q1, kd(k1[r]), xr(r)))))); }
-    B::stage(v); B::gather(k6);
+    B::stage(v); this->setTime((t + CK::c6 * dt) - t0); B::gather(k6);
     derivs += 5;
     const double q1 = CK::b1 * dt, q3 = CK::b3 * dt, q4 = CK::b4 * dt, q6 = CK::b6 * dt;
     double e = 0.;
@@ -409,10 +478,11 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   eng.sync();  // previous users of X / cnt are done
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = eng.valid(r) ? slot[r * G + lane] : make_double2(0., 0.);
-  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
+  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj], t0 = E.t0[traj];
   unsigned long long ord = E.ordinal[traj];
   const unsigned long long ord0 = ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
+  const double2* exz = S.exz;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
   eng.sync();
   const unsigned long long key = E.traj0 + (unsigned long long)traj;
@@ -424,11 +494,11 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
     if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-    eng.derivativeX(k1);
+    eng.derivativeX(k1, t - t0);
     long long derivs = 1, tries = 0, failed = 0;
     for (;;) {
       ++tries;
-      const double err = eng.tryStepX(k1, dtTry, epsAbs, epsRel, derivs);
+      const double err = eng.tryStepX(k1, dtTry, epsAbs, epsRel, derivs, t, t0);
       if (err > 1.0) {
         dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
         ++failed;
@@ -441,6 +511,19 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     dtDid = t - timeBefore;
     if (nextDtTry != 0.) dtTry = nextDtTry;
     if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
+    if (Eng::kTD && exz) {  // Exact::actWithU on the lane's rows (MCWF_Trajectory.h:269-272)
+      const double dtau = t - t0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (eng.valid(r)) {
+          const double2 z = exz[r * G + lane];
+          const double e = exp(-z.x * dtau);
+          double sn, cs;
+          sincos(-z.y * dtau, &sn, &cs);
+          X[r * G + lane] = cmul(X[r * G + lane], make_double2(e * cs, e * sn));
+        }
+      t0 = t;
+    }
     double s2 = 0.;  // renorm (StateVector.h:148-153)
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(X[r * G + lane]);
@@ -532,7 +615,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) slot[r * G + lane] = X[r * G + lane];
   eng.sync();
-  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
+  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.t0[traj] = t0; E.ordinal[traj] = ord; }
   if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
 }
 
@@ -557,10 +640,11 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   T k1[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? E.psi[(size_t)traj * S.dim + r * G + lane] : make_double2(0., 0.);
-  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj];
+  double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj], t0 = E.t0[traj];
   unsigned long long ord = E.ordinal[traj];
   const unsigned long long ord0 = ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
+  const double2* exz = S.exz;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
   eng.sync();
   const unsigned long long key = E.traj0 + (unsigned long long)traj;
@@ -573,11 +657,11 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
     if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-    eng.derivative(x, k1);
+    eng.derivative(x, k1, t - t0);
     long long derivs = 1, tries = 0, failed = 0;
     for (;;) {
       ++tries;
-      const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs);
+      const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs, t, t0);
       if (err > 1.0) {
         // default_step_adjuster::decrease_step: dt *= max(0.9 err^(-1/(error_order-1)), 1/5)
         dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
@@ -593,6 +677,19 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     for (int r = 0; r < ROWS; ++r) x[r] = xn[r];
     dtDid = t - timeBefore;
     if (nextDtTry != 0.) dtTry = nextDtTry;
+    if (Eng::kTD && exz) {  // Exact::actWithU, then the pictures coincide again (MCWF_Trajectory.h:269-272)
+      const double dtau = t - t0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (eng.valid(r)) {
+          const double2 z = exz[r * G + lane];
+          const double e = exp(-z.x * dtau);
+          double sn, cs;
+          sincos(-z.y * dtau, &sn, &cs);
+          x[r] = cmul(x[r], make_double2(e * cs, e * sn));
+        }
+      t0 = t;
+    }
     if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
     // renorm (StateVector.h:148-153)
     double s2 = 0.;
@@ -692,7 +789,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) E.psi[(size_t)traj * S.dim + r * G + lane] = x[r];
   eng.sync();
-  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.ordinal[traj] = ord; }
+  if (lane == 0) { E.t[traj] = t; E.dtDid[traj] = dtDid; E.dtTry[traj] = dtTry; E.t0[traj] = t0; E.ordinal[traj] = ord; }
   if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
 }
 
@@ -704,9 +801,9 @@ constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2)
 // batched persistent MCWF kernel: one system, one team = one work group
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32, int MINW>
+template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD>
 __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
-  using Eng = Engine<ROWS, NW, WH, F32>;
+  using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
   constexpr bool kPcLds = pc_in_lds<ROWS, NW>();
@@ -716,6 +813,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   __shared__ double red[NW > 1 ? NW : 1];
   __shared__ double rts[kMaxJumps];        // jump rates of the current step
   __shared__ long long cnt[kCounters];     // this trajectory's counters (lane 0 updates)
+  __shared__ T ph[kMaxFreqs];              // interaction-picture phasors of the current stage
   __shared__ int shTraj;
 
   const int lane = threadIdx.x;
@@ -731,14 +829,14 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // multi-wavefront team: psi in LDS
     __shared__ double2 Xs[ROWS * G];
     __shared__ T xs[ROWS * G];
-    LdsEngine<ROWS, NW, WH, F32> eng(S, xs, Xs, red, lane);
+    LdsEngine<ROWS, NW, WH, F32, false, TD> eng(S, xs, Xs, red, lane, ph);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
     // state copy share storage
     __shared__ double2 ps[ROWS * G];
     __shared__ double2 pc[kPcLds ? ROWS * G : 1];
-    Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane);
+    Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane, ph);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
   }
 }
@@ -994,22 +1092,24 @@ __global__ void __launch_bounds__(64 * NW) plain_obs_kernel(SysDev S, const doub
   for (int q = 0; q < S.no; ++q) { double v = eng.observable(q, x); if (threadIdx.x == 0) out[q] = v; }
 }
 
-// dpsidt += (H_nH/i) psi for a batch of state vectors, one group per vector (qt_apply_heff)
-template <int ROWS, int NW, int WH, bool F32>
-__global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const double2* psi, double2* dpsidt, int n) {
-  using Eng = Engine<ROWS, NW, WH, F32>;
+// dpsidt += (H_nH/i)(t - t0) psi for a batch of state vectors, one group per vector (qt_apply_heff)
+template <int ROWS, int NW, int WH, bool F32, bool TD>
+__global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const double2* psi, double2* dpsidt, int n,
+                                                             double tau) {
+  using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
   __shared__ T xs[ROWS * G];
   __shared__ double2 ps[1];
   __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
+  __shared__ T ph[kMaxFreqs];
+  Eng eng(S, xs, ps, red, (int)threadIdx.x, ph);
   for (int v = blockIdx.x; v < n; v += gridDim.x) {
     double2 x[ROWS];
     T k[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[(size_t)v * S.dim + r * G + threadIdx.x] : make_double2(0., 0.);
-    eng.derivative(x, k);
+    eng.derivative(x, k, tau);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r)
       if (eng.valid(r)) {
@@ -1045,6 +1145,11 @@ struct Layout { int rows, nw, wh; };
   X(1, 4, 8, 1) X(1, 4, 16, 1) X(2, 4, 8, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) \
   X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
 
+// Interaction-picture batched variants (rows, waves, ELL width): one per dimension class
+#define QT_BATCH_TD(X) \
+  X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1) \
+  X(1, 4, 16, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
+
 bool layout_supported(int rows, int nw, int wh) {
 #define QT_CASE(R, N, W) if (rows == R && nw == N && wh == W) return true;
   QT_LAYOUTS(QT_CASE)
@@ -1052,37 +1157,48 @@ bool layout_supported(int rows, int nw, int wh) {
   return false;
 }
 
-bool batch_supported(int rows, int nw, int wh, int minw) {
+bool batch_supported(int rows, int nw, int wh, int minw, bool td) {
 #define QT_CASE(R, N, W, M) if (rows == R && nw == N && wh == W && minw == M) return true;
-  QT_BATCH(QT_CASE)
+  if (td) { QT_BATCH_TD(QT_CASE) }
+  else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
   return false;
 }
 
 // resident work groups of a batched variant per compute unit (registers and LDS permitting)
-int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32) {
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td) {
   int n = 0;
-#define QT_CASE(R, N, W, M)                                                                                  \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                        \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M>, 64 * N, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M>, 64 * N, 0);    \
-    return n;                                                                                                \
+#define QT_CASE_T(R, N, W, M, TDV)                                                                                  \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                               \
+    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M, TDV>, 64 * N, 0); \
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M, TDV>, 64 * N, 0);    \
+    return n;                                                                                                       \
   }
-  QT_BATCH(QT_CASE)
+#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false)
+#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true)
+  if (td) { QT_BATCH_TD(QT_CASE_TD) }
+  else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
+#undef QT_CASE_TD
+#undef QT_CASE_T
   return 0;
 }
 
-hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E,
-                        hipStream_t st) {
-#define QT_CASE(R, N, W, M)                                                                            \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                  \
-    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M>), dim3(grid), dim3(64 * N), 0, st, S, E); \
-    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
-    return hipGetLastError();                                                                          \
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, int grid, const SysDev& S,
+                        const EnsDev& E, hipStream_t st) {
+#define QT_CASE_T(R, N, W, M, TDV)                                                                            \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                         \
+    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M, TDV>), dim3(grid), dim3(64 * N), 0, st, S, E); \
+    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M, TDV>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
+    return hipGetLastError();                                                                                 \
   }
-  QT_BATCH(QT_CASE)
+#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false)
+#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true)
+  if (td) { QT_BATCH_TD(QT_CASE_TD) }
+  else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
+#undef QT_CASE_TD
+#undef QT_CASE_T
   return hipErrorInvalidValue;
 }
 
@@ -1181,13 +1297,18 @@ hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const dou
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, const SysDev& S, const double2* psi, double2* dpsidt, int n,
-                             hipStream_t st) {
+hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const SysDev& S, const double2* psi,
+                             double2* dpsidt, int n, double tau, hipStream_t st) {
   const int grid = n < 4096 ? n : 4096;
 #define QT_CASE(R, N, W)                                                                                      \
   if (rows == R && nw == N && wh == W) {                                                                      \
-    if (f32) hipLaunchKernelGGL((apply_heff_kernel<R, N, W, true>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n); \
-    else hipLaunchKernelGGL((apply_heff_kernel<R, N, W, false>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n);    \
+    if (td) {                                                                                                 \
+      if (f32) hipLaunchKernelGGL((apply_heff_kernel<R, N, W, true, true>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau); \
+      else hipLaunchKernelGGL((apply_heff_kernel<R, N, W, false, true>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau);    \
+    } else {                                                                                                  \
+      if (f32) hipLaunchKernelGGL((apply_heff_kernel<R, N, W, true, false>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau); \
+      else hipLaunchKernelGGL((apply_heff_kernel<R, N, W, false, false>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau);    \
+    }                                                                                                         \
     return hipGetLastError();                                                                                 \
   }
   QT_LAYOUTS(QT_CASE)

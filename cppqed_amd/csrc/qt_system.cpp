@@ -446,6 +446,11 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
         throw std::invalid_argument("operator axes do not match the system dimensions");
     std::vector<int> lstr(r, 1);
     for (int i = r - 2; i >= 0; --i) lstr[i] = lstr[i + 1] * op.dims[i + 1];
+    // The reference's direct product of Tridiagonals carries frequencies only when every factor
+    // does (doDirect of an empty array is empty: Tridiagonal.tcc:61-67, ComplexArrayExtensions.h:80-92),
+    // so a term rotates with the interaction picture only if all of its frees are Exact.
+    bool allEx = true;
+    for (int i = 0; i < r; ++i) allEx = allEx && s.frees[t.axes[i]].hasEx;
     // group entries by local row
     std::vector<std::vector<const LocalOp::Entry*>> byRow(op.size());
     for (auto& e : op.entries) byRow[e.row].push_back(&e);
@@ -460,7 +465,7 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
           int rc = (e->row / lstr[i]) % op.dims[i], cc = (e->col / lstr[i]) % op.dims[i];
           col += (long)(cc - rc) * s.strides[ax];
           // interaction-picture phase of |r><c| under U = exp(-zI n t): exp(zI (r - c) (t - t0))
-          if (s.frees[ax].hasEx) freq += s.frees[ax].zI * (double)(rc - cc);
+          if (allEx) freq += s.frees[ax].zI * (double)(rc - cc);
         }
         auto& rv = rows[row];
         auto it = std::find_if(rv.begin(), rv.end(), [&](const E& x) { return x.col == (int)col && x.freq == freq; });

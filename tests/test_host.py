@@ -63,6 +63,36 @@ def test_tables_match_oracle(name):
     np.testing.assert_allclose(process(spec, vals), so.averages(psi), rtol=1e-12, atol=1e-12)
 
 
+IP_SPECS = {
+    "jc_ip": "free qbit deltaA=0.5 gamma=0.2 eta=0.3 picture=IP\nfree mode cutoff=6 delta=-1 kappa=1 eta=1 picture=IP\n"
+             "ia jc 0 1 g=3",
+    "plm_ip": "free mode cutoff=8 delta=-1 kappa=1 eta=2 picture=IP",
+    "jc_uip": "free qbit deltaA=0.5 gamma=0.2 eta=0.3 picture=UIP\nfree mode cutoff=5 delta=-1 kappa=1 eta=1 picture=UIP\n"
+              "ia jc 0 1 g=3 gI=0.5",
+    "bs_mixed_pictures": "free mode cutoff=3 delta=-1 kappa=1 eta=1 picture=IP\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0.2\n"
+                         "ia bs 0 1 g=0.5\nsystem composite",
+}
+
+
+@pytest.mark.parametrize("name", sorted(IP_SPECS))
+def test_interaction_picture_tables_match_oracle(name):
+    """H_nH/i(tau) assembled from the lowered entries and their frequencies, val * exp(freq tau), equals
+    the oracle's propagated Tridiagonals (Tridiagonal.tcc:153-163) applied at t - t0 = tau."""
+    spec = IP_SPECS[name]
+    sq, so = Q.System(spec), O.System(spec)
+    assert sq.info.time_dependent
+    cols, vals, freqs = sq.table(0)
+    rows = np.repeat(np.arange(sq.dim), cols.shape[1])
+    rng = np.random.default_rng(11)
+    psi = rng.normal(size=sq.dim) + 1j * rng.normal(size=sq.dim)
+    for tau in (0.0, 0.13, 0.71, 0.05):
+        H = np.zeros((sq.dim, sq.dim), np.complex128)
+        np.add.at(H, (rows, cols.ravel()), (vals * np.exp(freqs * tau)).ravel())
+        ref = so.apply_heff(psi, t=tau, t0=0.0)
+        np.testing.assert_allclose(H @ psi, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert sq.highest_frequency == so.highest_frequency
+
+
 def test_particle_position_operator_is_the_fft_population():
     """<X> of the dense momentum-basis operator equals sum_j x_j |psi_x(j)|^2 with psi_x from numpy's
     inverse FFT (the GSL backward transform up to 1/N), halves swapped (LazyDensityOperatorFFT.cc:16-40);
