@@ -147,8 +147,10 @@ struct Builder {
     } else {
       if (!kappa || !nonzero(eta)) throw std::invalid_argument("interaction-picture mode needs kappa and eta");
       f.hasEx = true;
-      if (pic == "IP") { h = modeHOverI(0., eta, dim, empty); f.zI = z; hf = std::abs(std::conj(z)); }
-      else { h = modeHOverI(cd(kappa, 0.), eta, dim, empty); f.zI = cd(0., -delta); hf = std::max(std::fabs(kappa * dim), std::fabs(delta)); }
+      cd zI;
+      if (pic == "IP") { h = modeHOverI(0., eta, dim, empty); zI = z; hf = std::abs(std::conj(z)); }
+      else { h = modeHOverI(cd(kappa, 0.), eta, dim, empty); zI = cd(0., -delta); hf = std::max(std::fabs(kappa * dim), std::fabs(delta)); }
+      for (int n = 0; n < dim; ++n) f.exd.push_back(zI * (double)n);  // mode::Exact, Mode.cc:42-45
       hf = std::max(hf, std::abs(eta * std::sqrt((double)dim)));
     }
     f.hasHa = true;  // every mode::Hamiltonian carries at least the (zero) Kerr term (Mode.cc:92-107)
@@ -185,8 +187,10 @@ struct Builder {
     } else {
       if (!gamma || !nonzero(eta)) throw std::invalid_argument("interaction-picture qbit needs gamma and eta");
       f.hasEx = true;
-      if (pic == "IP") { h = modeHOverI(0., -eta, 2, empty); f.zI = z; hf = std::abs(std::conj(z)); }
-      else { h = modeHOverI(cd(gamma, 0.), -eta, 2, empty); f.zI = cd(0., -delta); hf = std::max(std::fabs(gamma), std::fabs(delta)); }
+      cd zI;
+      if (pic == "IP") { h = modeHOverI(0., -eta, 2, empty); zI = z; hf = std::abs(std::conj(z)); }
+      else { h = modeHOverI(cd(gamma, 0.), -eta, 2, empty); zI = cd(0., -delta); hf = std::max(std::fabs(gamma), std::fabs(delta)); }
+      f.exd = {0., zI};
       hf = std::max(hf, std::abs(eta));
     }
     f.hasHa = true;
@@ -265,23 +269,80 @@ struct Builder {
     iaHF = std::max(iaHF, std::fabs(J));
   }
 
-  // ParticleSch (Particle.cc:79-85, 247-252): H/i = -i omrec k^2 on the momentum grid
-  // k_n = n - 2^(fin-1) (particle::Spatial with deltaK = 1, Particle.cc:276-290); no jumps.
+  // e^{iKx} on the momentum grid: sum_q |q+K><q| (expINKX, Particle.cc:60-72)
+  static LocalOp expINKX(int dim, int K) {
+    LocalOp r; r.dims = {dim};
+    const int a = std::abs(K);
+    for (int q = 0; q + a < dim; ++q) {
+      if (K >= 0) r.entries.push_back({q + a, q, 1., 0.});
+      else r.entries.push_back({q, q + a, 1., 0.});
+    }
+    return r;
+  }
+  // the pump's mode function at wavenumber nK (mfNKX, Particle.cc:274-289)
+  static LocalOp modeFunction(int dim, const std::string& mf, int nK) {
+    if (mf == "sin") return scale(plus(expINKX(dim, nK), scale(expINKX(dim, -nK), -1.)), 1. / cd(0., 2.));
+    if (mf == "cos") return scale(plus(expINKX(dim, nK), expINKX(dim, -nK)), 0.5);
+    if (mf == "plus") return expINKX(dim, nK);
+    return expINKX(dim, -nK);
+  }
+
+  // (Pumped)Particle(Sch) (Particle.cc:79-110, 210-252).  Momentum grid k_n = n - 2^(fin-1)
+  // (particle::Spatial with deltaK = 1, Particle.cc:276-290); kinetic term -i omrec k^2 (Sch) or
+  // the Exact part exp(-i omrec k^2 t) (IP); classical pump vClass (+-) cos(2 kPart x) / (2i) for
+  // a real mode function (hOverI, Particle.cc:84-87); no jumps.
   void particle(const KV& kv) {
     int ax = (int)s.frees.size();
     int fin = (int)num(kv, "fin", 6);
     if (fin < 1 || fin > 12) throw std::invalid_argument("particle finesse out of range");
-    if (picture(kv) != "Sch") throw std::invalid_argument("the particle element is built in the Schroedinger picture");
-    double omrec = num(kv, "omrec", 1.);
+    const std::string pic = picture(kv);
+    if (pic == "UIP") throw std::invalid_argument("the particle has no unitary interaction picture of its own");
+    double omrec = num(kv, "omrec", 1.), vclass = num(kv, "vclass", 0.);
+    int kpart = (int)num(kv, "kpart", 1);
+    auto it = kv.find("mf");
+    std::string mf = it == kv.end() ? "sin" : it->second;
+    if (mf != "sin" && mf != "cos" && mf != "plus" && mf != "minus") throw std::invalid_argument("unknown particle mode function");
     int dim = 1 << fin;
-    std::vector<cd> d(dim);
-    for (int n = 0; n < dim; ++n) { double k = n - dim / 2; d[n] = cd(0., -omrec * k * k); }
-    FreeInfo f; f.kind = "particle"; f.dim = dim; f.avKind = "particle"; f.hasHa = true;
-    f.highestFrequency = std::fabs(omrec) * (double)dim * dim;  // RF{"omrec", omrec, sqr(1<<fin)}
-    s.hTerms.push_back({{ax}, diagonal(dim, d)});
+    FreeInfo f; f.kind = "particle"; f.dim = dim; f.avKind = "particle";
+    f.pumped = vclass != 0.; f.mf = mf; f.kpart = kpart;
+    LocalOp h; h.dims = {dim};
+    if (f.pumped && (mf == "sin" || mf == "cos") && 2 * kpart < dim) {
+      LocalOp c = scale(plus(expINKX(dim, 2 * kpart), expINKX(dim, -2 * kpart)), 0.5);
+      h = scale(c, cd(vclass * (mf == "sin" ? -1. : 1.), 0.) / cd(0., 2.));
+    }
+    if (pic == "Sch") {
+      std::vector<cd> d(dim);
+      for (int n = 0; n < dim; ++n) { double k = n - dim / 2; d[n] = cd(0., -omrec * k * k); }
+      h = plus(diagonal(dim, d), h);
+      f.hasHa = true;
+      f.highestFrequency = std::fabs(omrec) * (double)dim * dim;  // RF{"omrec", omrec, sqr(1<<fin)}
+    } else {
+      f.hasEx = true;
+      for (int n = 0; n < dim; ++n) { double k = n - dim / 2; f.exd.push_back(cd(0., omrec * k * k)); }
+      f.hasHa = !h.entries.empty();
+      f.highestFrequency = std::fabs(omrec) * (double)dim;        // RF{"omrec", omrec, 1<<fin}
+    }
+    if (f.pumped) f.highestFrequency = std::max(f.highestFrequency, std::fabs(vclass));  // RF{"vClass", vClass, 1}
+    f.vclass = vclass;
+    if (f.hasHa) s.hTerms.push_back({{ax}, h});
     freeJumps.emplace_back(); freeObs.emplace_back(); freeVar.emplace_back(); freeSqrt.emplace_back();
     avParticle(ax, dim);
     s.frees.push_back(f);
+  }
+
+  // ParticleOrthogonalToCavity (ParticleCavity.cc:47-57, 79-85): the interference term
+  // etaeff (a (x) mf(kPart x) + h.c.) / i with etaeff = sign(u0) sqrt(u0 vClass) and the pumped
+  // particle's own mode function.
+  void pcavOrth(int m, int p, const KV& kv) {
+    const FreeInfo& fm = freeAt(m); const FreeInfo& fp = particleAt(p);
+    if (fm.kind != "mode") throw std::invalid_argument("pcav_orth needs a mode as its first free");
+    if (!fp.pumped) throw std::invalid_argument("pcav_orth needs a pumped particle");
+    double u0 = num(kv, "u0", -1.);
+    if (u0 * fp.vclass < 0.) throw std::invalid_argument("Unot VClass sign discrepancy");
+    const double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * fp.vclass);
+    LocalOp a = kron(destroy(fm.dim), modeFunction(fp.dim, fp.mf, fp.kpart));
+    s.hTerms.push_back({{m, p}, scale(plus(a, adjoint(a)), cd(etaeff, 0.) / cd(0., 1.))});
+    iaHF = std::max(iaHF, std::max(std::fabs(u0) * fm.dim, std::fabs(etaeff) * std::sqrt((double)fm.dim)));
   }
 
   // particle::Averaged (Particle.cc:138-167): <P>, VAR(P), <X>, DEV(X).  The position
@@ -393,6 +454,7 @@ System* buildFromSpec(const std::string& spec) {
       else if (kind == "bs") b.bs(axes[0], axes[1], kv);
       else if (kind == "zz") b.zz(axes[0], axes[1], kv);
       else if (kind == "pcav") b.pcav(axes[0], axes[1], kv);
+      else if (kind == "pcav_orth") b.pcavOrth(axes[0], axes[1], kv);
       else throw std::invalid_argument("unknown interaction " + kind);
       ++nIa;
     } else if (what == "system") {
@@ -424,7 +486,7 @@ System* buildFromSpec(const std::string& spec) {
     s.exactZ.assign(s.dim, 0.);
     for (long r = 0; r < s.dim; ++r)
       for (size_t a = 0; a < s.frees.size(); ++a)
-        if (s.frees[a].hasEx) s.exactZ[r] += s.frees[a].zI * (double)((r / s.strides[a]) % s.dims[a]);
+        if (s.frees[a].hasEx) s.exactZ[r] += s.frees[a].exd[(r / s.strides[a]) % s.dims[a]];
   }
   s.lower();
   return sp.release();
@@ -446,9 +508,12 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
         throw std::invalid_argument("operator axes do not match the system dimensions");
     std::vector<int> lstr(r, 1);
     for (int i = r - 2; i >= 0; --i) lstr[i] = lstr[i + 1] * op.dims[i + 1];
-    // The reference's direct product of Tridiagonals carries frequencies only when every factor
-    // does (doDirect of an empty array is empty: Tridiagonal.tcc:61-67, ComplexArrayExtensions.h:80-92),
-    // so a term rotates with the interaction picture only if all of its frees are Exact.
+    // The reference's direct product of Tridiagonals carries frequencies only where every factor
+    // does (doDirect of an empty array is empty: Tridiagonal.tcc:61-67, ComplexArrayExtensions.h:80-92);
+    // a factor has them on its off-diagonals only, and only when its free is Exact
+    // (Tridiagonal.cc:24-36 leaves the main diagonal's empty).  So an entry of a term rotates with
+    // the interaction picture only if all of the term's frees are Exact and the entry is off the
+    // main diagonal on every one of its axes (a single-axis term needs only the former).
     bool allEx = true;
     for (int i = 0; i < r; ++i) allEx = allEx && s.frees[t.axes[i]].hasEx;
     // group entries by local row
@@ -460,13 +525,16 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
       for (const LocalOp::Entry* e : byRow[lr]) {
         long col = row;
         cd freq = 0.;
+        bool offEverywhere = true;
         for (int i = 0; i < r; ++i) {
           int ax = t.axes[i];
           int rc = (e->row / lstr[i]) % op.dims[i], cc = (e->col / lstr[i]) % op.dims[i];
           col += (long)(cc - rc) * s.strides[ax];
-          // interaction-picture phase of |r><c| under U = exp(-zI n t): exp(zI (r - c) (t - t0))
-          if (allEx) freq += s.frees[ax].zI * (double)(rc - cc);
+          offEverywhere = offEverywhere && rc != cc;
+          // interaction-picture phase of |r><c| under U = exp(-exd[n] t): exp((exd[r] - exd[c]) (t - t0))
+          if (allEx) freq += s.frees[ax].exd[rc] - s.frees[ax].exd[cc];
         }
+        if (r > 1 && !offEverywhere) freq = 0.;
         auto& rv = rows[row];
         auto it = std::find_if(rv.begin(), rv.end(), [&](const E& x) { return x.col == (int)col && x.freq == freq; });
         if (it == rv.end()) rv.push_back({(int)col, e->val, freq});

@@ -54,10 +54,15 @@ struct FreeInfo {
   std::string kind;
   int dim = 0;
   bool hasHa = false, hasEx = false;
-  cd zI = 0.;
+  std::vector<cd> exd;       // Exact part: U = exp(-exd[n] (t - t0)) on level n (empty without one)
   double highestFrequency = 0.;
   int nJumps = 0;
   std::string avKind;  // "mode" | "qbit" | "particle" | ""
+  // pumped particle: classical pump strength, mode function and wavenumber
+  bool pumped = false;
+  double vclass = 0.;
+  std::string mf;
+  int kpart = 0;
 };
 
 // ELL matrix over the full space
@@ -82,7 +87,7 @@ struct System {
   double highestFrequency = 0.;
   bool isBinary = false;
   bool hasEx = false;
-  std::vector<cd> exactZ;                 // per basis state, sum over frees of zI * n (empty if no Exact)
+  std::vector<cd> exactZ;                 // per basis state, sum over frees of exd[n_free] (empty if no Exact)
 
   // lowered tables
   Ell H;
@@ -95,11 +100,13 @@ struct System {
 //   free mode  cutoff=N delta=.. kappa=.. eta=.. [etaI=..] [picture=Sch|IP|UIP]
 //   free qbit  deltaA=.. gamma=.. eta=.. [etaI=..] [picture=..]
 //   free ptla  gamma=.. deltaA=.. eta=..                   (PumpedTwoLevelAtomSch)
-//   free particle fin=.. omrec=..                          (ParticleSch, momentum basis)
+//   free particle fin=.. omrec=.. [vclass=.. kpart=.. mf=sin|cos|plus|minus] [picture=..]
+//                                                          ((Pumped)Particle(Sch), momentum basis)
 //   ia jc i j g=.. [gI=..]                                 (JaynesCummings)
 //   ia bs i j g=..                                         (CoupledModes, RWA beam splitter)
 //   ia zz i j J=..                                         (qubit sigma_z sigma_z coupling)
 //   ia pcav m p u0=.. kcav=.. mf=cos|sin                   (ParticleAlongCavity, dispersive term)
+//   ia pcav_orth m p u0=..                                 (ParticleOrthogonalToCavity, pumped particle)
 //   system binary|composite
 System* buildFromSpec(const std::string& spec);
 

@@ -302,8 +302,11 @@ struct Free {
   bool hasHa = false, timeDep = false;
   std::vector<Tridiag> hOverIs;
   CV cmatrix;                  // dim*dim, row-major (PTLA Sch)
-  // Exact: diagonal U = exp(-zI*(n*(t-t0)))  (Mode.cc:42-45, FreeExact.h:31)
-  bool hasEx = false; cd zI = 0.;
+  // Exact: diagonal U = exp(-exd[n] (t-t0)), exd = zI n for modes (Mode.cc:42-45, FreeExact.h:31),
+  // i omrec k^2 for the particle (Particle.cc:29-48); the same diagonal furnishes the frequencies
+  bool hasEx = false; cd zI = 0.; CV exd;
+  // pumped particle
+  double vClass = 0.; std::string mf; int kPart = 0;
   // Liouvillean
   std::vector<Jump> jumps;
   // Averaged: "mode" (n, VAR n, Re a, Im a) or "qbit" (rho00, rho11, Re rho10, Im rho10)
@@ -371,7 +374,7 @@ Tridiag hOverI(cd z, cd eta, double omegaKerr, int dim, bool& empty) {
 void modeHamiltonian(Free& f, bool isTD, cd zSch, cd zI, cd eta, int dim) {
   bool empty;
   Tridiag h = hOverI(zSch, eta, 0., dim, empty);
-  if (isTD) { furnishWithFreqs(h, mainDiagonal(zI, 0., dim)); f.hasEx = true; f.zI = zI; }
+  if (isTD) { furnishWithFreqs(h, mainDiagonal(zI, 0., dim)); f.hasEx = true; f.zI = zI; f.exd = mainDiagonal(zI, 0., dim); }
   f.hOverIs.push_back(h);
   f.hOverIs.push_back(unary(mainDiagonal(0., 0., dim), 0, CV(), CV()));  // (n-1)n*0/i == 0
   for (auto& x : f.hOverIs.back().diag[0]) x = cd(0, 0) * x;  // exact zeros
@@ -486,20 +489,65 @@ Free makePTLA(const std::map<std::string, std::string>& kv) {
 double particleK(int n, int N) { return (double)n - N / 2; }
 double particleX(int j, int N) { const double pi = 3.14159265358979323846; return j * (2. * pi / N) - pi; }
 
-// ParticleSch (Particle.cc:79-85, 247-252): Tridiagonal(mainDiagonal(space, -omrec)),
-// mainDiagonal = i omrec k^2 (Particle.cc:92-95); frequency RF{"omrec", omrec, sqr(1<<fin)}
+// expINKX(dim, nK), Particle.cc:60-72: e^{i nK x} as a Tridiagonal with offset |nK|
+Tridiag expINKX(int dim, int nK) {
+  int K = std::abs(nK), D = dim - K;
+  if (D < 0) return unary(CV(), 0, CV(), CV());
+  CV ones(D, 1.);
+  if (nK > 0) return unary(CV(), K, ones, CV());
+  if (nK == 0) return unary(ones, 0, CV(), CV());
+  return unary(CV(), K, CV(), ones);
+}
+// cosNKX / sinNKX, Particle.cc:76-80, Particle_.h:64-65
+Tridiag cosNKXof(int dim, int nK) { return scaled(sum(expINKX(dim, nK), expINKX(dim, -nK)), 0.5); }
+Tridiag sinNKXof(int dim, int nK) { return scaled(sum(expINKX(dim, nK), negated(expINKX(dim, -nK))), 1. / cd(0., 2.)); }
+
+// (Pumped)Particle(Sch), Particle.cc:79-110, 210-252: Sch: Tridiagonal(mainDiagonal(space, -omrec))
+// (+ hOverI of the pump), mainDiagonal = i omrec k^2 (Particle.cc:92-95); IP: Exact with
+// factorExponents -i omrec k^2 and hOverI furnished with the main diagonal.  hOverI =
+// vClass (sin: -1) cosNKX(dim, 2 kPart) / (2i) for a real mode function (Particle.cc:84-87).
+// Frequencies: RF{omrec, omrec, (1<<fin)^2 (Sch) or 1<<fin (IP)}, RF{vClass, vClass, 1}.
 Free makeParticle(const std::map<std::string, std::string>& kv) {
   Free f; f.kind = "particle"; f.avKind = "particle";
   int fin = (int)num(kv, "fin", 6);
   double omrec = num(kv, "omrec", 1.);
-  if (picture(kv) != Picture::Sch) throw std::invalid_argument("oracle: particle is Sch only");
+  f.vClass = num(kv, "vclass", 0.);
+  f.kPart = (int)num(kv, "kpart", 1);
+  auto it = kv.find("mf"); f.mf = it == kv.end() ? "sin" : it->second;
+  Picture p = picture(kv);
+  if (p == Picture::UIP) throw std::invalid_argument("oracle: particle has no UIP");
   f.dim = 1 << fin;
-  CV md(f.dim);
-  for (int n = 0; n < f.dim; ++n) md[n] = cd(0, 1) * (-omrec) * (particleK(n, f.dim) * particleK(n, f.dim));
-  f.hOverIs.push_back(unary(md, 0, CV(), CV()));
-  f.hasHa = true;
-  f.realFreqs.push_back({omrec, (double)f.dim * f.dim});
+  const bool pumped = f.vClass != 0.;
+  CV md(f.dim);  // i omrec k^2
+  for (int n = 0; n < f.dim; ++n) md[n] = cd(0, 1) * omrec * (particleK(n, f.dim) * particleK(n, f.dim));
+  bool hasPump = pumped && (f.mf == "sin" || f.mf == "cos");
+  Tridiag pump = hasPump ? scaled(cosNKXof(f.dim, 2 * f.kPart), cd(f.vClass * (f.mf == "sin" ? -1. : 1.), 0.) / cd(0., 2.))
+                         : unary(CV(), 0, CV(), CV());
+  if (p == Picture::Sch) {
+    CV mdm(f.dim); for (int n = 0; n < f.dim; ++n) mdm[n] = -md[n];
+    Tridiag h = unary(mdm, 0, CV(), CV());
+    if (hasPump) h = sum(h, pump);
+    f.hOverIs.push_back(h);
+    f.hasHa = true;
+    f.realFreqs.push_back({omrec, (double)f.dim * f.dim});
+  } else {
+    f.hasEx = true; f.exd = md;
+    if (hasPump) { furnishWithFreqs(pump, md); f.hOverIs.push_back(pump); f.hasHa = true; f.timeDep = true; }
+    f.realFreqs.push_back({omrec, (double)f.dim});
+  }
+  if (pumped) f.realFreqs.push_back({f.vClass, 1.});
   return f;
+}
+
+// mfNKX(particle, (mf, nK)), Particle.cc:274-289, furnished with the particle's main diagonal
+// when it is Exact (expINKX(Ptr, nK), Particle.cc:263-269)
+Tridiag mfNKXof(const Free& fp) {
+  auto fur = [&](Tridiag t) { if (fp.hasEx) furnishWithFreqs(t, fp.exd); return t; };
+  int nK = fp.kPart;
+  if (fp.mf == "sin") return scaled(sum(fur(expINKX(fp.dim, nK)), negated(fur(expINKX(fp.dim, -nK)))), 1. / cd(0., 2.));
+  if (fp.mf == "cos") return scaled(sum(fur(expINKX(fp.dim, nK)), fur(expINKX(fp.dim, -nK))), 0.5);
+  if (fp.mf == "plus") return fur(expINKX(fp.dim, nK));
+  return fur(expINKX(fp.dim, -nK));
 }
 
 // ffTransformCV with DIR_KX (LazyDensityOperatorFFT.cc:16-40): GSL backward transform
@@ -550,8 +598,8 @@ Interaction makeJC(const System& s, int q, int m, const std::map<std::string, st
   const Free& fq = s.frees[q]; const Free& fm = s.frees[m];
   Tridiag sig = aop(fq.dim), a = aop(fm.dim);
   // aop(Ptr) furnishes with freqs when the free is mode::Exact (Mode.cc:288-294)
-  if (fq.hasEx) furnishWithFreqs(sig, mainDiagonal(fq.zI, 0., fq.dim));
-  if (fm.hasEx) furnishWithFreqs(a, mainDiagonal(fm.zI, 0., fm.dim));
+  if (fq.hasEx) furnishWithFreqs(sig, fq.exd);
+  if (fm.hasEx) furnishWithFreqs(a, fm.exd);
   if (isNonZero(g)) {
     ia.hOverIs.push_back(minusHC(product(scaled(sig, std::conj(g)), hc(a))));
     ia.hasHa = true; ia.timeDep = true;
@@ -567,8 +615,8 @@ Interaction makeBS(const System& s, int m1, int m2, const std::map<std::string, 
   cd u(num(kv, "g", 0), num(kv, "gI", 0));
   const Free& f1 = s.frees[m1]; const Free& f2 = s.frees[m2];
   Tridiag a1 = aop(f1.dim), a2 = aop(f2.dim);
-  if (f1.hasEx) furnishWithFreqs(a1, mainDiagonal(f1.zI, 0., f1.dim));
-  if (f2.hasEx) furnishWithFreqs(a2, mainDiagonal(f2.zI, 0., f2.dim));
+  if (f1.hasEx) furnishWithFreqs(a1, f1.exd);
+  if (f2.hasEx) furnishWithFreqs(a2, f2.exd);
   ia.hOverIs.push_back(minusHC(product(scaled(a1, std::conj(u)), hc(a2))));
   ia.hasHa = true; ia.timeDep = true;
   ia.complexFreqs.push_back({u, f1.dim * std::sqrt((double)f2.dim)});
@@ -619,6 +667,26 @@ Interaction makePcav(const System& s, int m, int p, const std::map<std::string, 
   return ia;
 }
 
+// ParticleOrthogonalToCavity (ParticleCavity.cc:47-57, 79-85):
+// etaeff tridiagPlusHC_overI(aop(mode) mfNKX(particle, mf)), etaeff = sign(u0) sqrt(u0 vClass)
+// (ParticleCavity.cc:20-24; Tridiagonal.h: tridiagPlusHC_overI(t) = (t + t^dagger)/i)
+Interaction makePcavOrth(const System& s, int m, int p, const std::map<std::string, std::string>& kv) {
+  Interaction ia; ia.kind = "pcav_orth"; ia.axes = {m, p};
+  const Free& fm = s.frees[m]; const Free& fp = s.frees[p];
+  if (fm.kind != "mode" || fp.kind != "particle" || fp.vClass == 0.) throw std::invalid_argument("oracle: pcav_orth");
+  double u0 = num(kv, "u0", -1.);
+  if (u0 * fp.vClass < 0.) throw std::domain_error("Unot VClass sign discrepancy");
+  double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * fp.vClass);
+  Tridiag a = aop(fm.dim);
+  if (fm.hasEx) furnishWithFreqs(a, fm.exd);
+  Tridiag t = product(a, mfNKXof(fp));
+  ia.hOverIs.push_back(scaled(sum(t, hc(t)), cd(etaeff, 0.) / cd(0., 1.)));
+  ia.hasHa = true; ia.timeDep = true;
+  ia.realFreqs.push_back({u0, (double)fm.dim});
+  ia.realFreqs.push_back({etaeff, std::sqrt((double)fm.dim)});
+  return ia;
+}
+
 System* buildSystem(const std::string& spec) {
   std::unique_ptr<System> s(new System);
   std::string line;
@@ -643,6 +711,7 @@ System* buildSystem(const std::string& spec) {
       else if (kind == "bs") s->ias.push_back(makeBS(*s, axes.at(0), axes.at(1), kv));
       else if (kind == "zz") s->ias.push_back(makeZZ(*s, axes.at(0), axes.at(1), kv));
       else if (kind == "pcav") s->ias.push_back(makePcav(*s, axes.at(0), axes.at(1), kv));
+      else if (kind == "pcav_orth") s->ias.push_back(makePcavOrth(*s, axes.at(0), axes.at(1), kv));
       else throw std::invalid_argument("oracle: unknown interaction " + kind);
     } else if (what == "system") {
       forced = kind == "binary" ? 1 : 0;  // binary::make vs composite::make chosen explicitly
@@ -685,7 +754,7 @@ void actWithU(System& s, double t, cd* psi, double t0) {
     Free& f = s.frees[i];
     if (!f.hasEx) continue;
     CV u(f.dim);
-    for (int n = 0; n < f.dim; ++n) u[n] = std::exp(-f.zI * ((t - t0) * n));
+    for (int n = 0; n < f.dim; ++n) u[n] = std::exp(-f.exd[n] * (t - t0));
     forSlices(s.L, {(int)i}, [&](long off, const std::vector<long>& vs) { for (int n = 0; n < f.dim; ++n) psi[off + n * vs[0]] *= u[n]; });
   }
 }
@@ -1035,7 +1104,7 @@ struct Master {
   long d;
   Master(System* s, const CV& rho0, double dtInit, double epsRel, double epsAbs)
       : sys(s), rho(rho0), ode(dtInit, epsRel, epsAbs), d(s->L.total) {
-    for (auto& f : sys->frees) if (f.hasEx && f.zI.real() != 0.) throw std::runtime_error("SystemNotApplicable");
+    for (auto& f : sys->frees) if (f.hasEx) for (auto& z : f.exd) if (z.real() != 0.) throw std::runtime_error("SystemNotApplicable");
   }
 
   // hermitianConjugateSelf (ComplexArrayExtensions.h:40-46)
@@ -1105,15 +1174,18 @@ struct Master {
   void step(double deltaT) {
     ode.step(deltaT, [this](const CV& y, CV& dy, double tt) { derivs(y, dy, tt); }, t, rho);
     if (sys->hasEx()) {
+      // Master.h:187-193: U on the Left slices (columns), hermitianConjugateSelf, U on the Left
+      // slices, conj.  hermitianConjugateSelf conjugates the data but transposes only Blitz's
+      // view (transposeSelf, ComplexArrayExtensions.h:40-46, 113-116), so the second pass runs
+      // over the rows of the stored matrix: rho <- U rho U^dagger.
       CV c(d);
-      auto unaryU = [&]() {
-        for (long j = 0; j < d; ++j) {
-          for (long i = 0; i < d; ++i) c[i] = rho[i * d + j];
-          actWithU(*sys, t, c.data(), t0);
-          for (long i = 0; i < d; ++i) rho[i * d + j] = c[i];
-        }
-      };
-      unaryU(); hcSelf(rho, d); unaryU();
+      for (long j = 0; j < d; ++j) {
+        for (long i = 0; i < d; ++i) c[i] = rho[i * d + j];
+        actWithU(*sys, t, c.data(), t0);
+        for (long i = 0; i < d; ++i) rho[i * d + j] = c[i];
+      }
+      for (auto& x : rho) x = std::conj(x);
+      for (long i = 0; i < d; ++i) actWithU(*sys, t, rho.data() + i * d, t0);  // rows: unit stride
       for (auto& x : rho) x = std::conj(x);
     }
     t0 = t;

@@ -66,6 +66,35 @@ def jc_master_spec(cutoff=40, eta=1.0, g=10.0):
             f"ia jc 0 1 g={g!r}")
 
 
+# CPPQEDscripts/1particle1mode, configuration 1 (pumped particle moving orthogonal to the pumped
+# cavity), with the golden runs' options --cutoff 3 --fin 3 --eta 10 and the parameter defaults
+# (deltaC = -10, kappa = 10, omrec = 1, vClass = -10, kPart = 1, Sin mode function, Unot = -1),
+# interaction picture; the Master runs switch the lossy mode to its unitary interaction picture.
+P1M1 = ("free mode cutoff=3 delta=-10 kappa=10 eta=10 picture=IP\n"
+        "free particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin picture=IP\n"
+        "ia pcav_orth 0 1 u0=-1")
+P1M1_MASTER = P1M1.replace("eta=10 picture=IP", "eta=10 picture=UIP")
+
+
+def particle_wavepacket(fin, x0=-0.5, k0=0.0, sig=0.1):
+    """particle::wavePacket in x (Particle.cc:347-369), exp(-(x - pi x0)^2 / (4 sig^2) + i k0 x) on
+    x_j = j 2pi/N - pi, taken to the momentum basis by quantumdata::ffTransform(DIR_XK)
+    (LazyDensityOperatorFFT.cc:16-40: GSL forward transform, halves swapped with N^-1/2, odd
+    entries negated), normalised.  The defaults are the reference's pinit (ParsParticle.cc:14)."""
+    n = 1 << fin
+    x = np.arange(n) * (2 * np.pi / n) - np.pi
+    psi = np.exp(-(x - np.pi * x0) ** 2 / (4 * sig ** 2) + 1j * x * k0)
+    y = np.fft.fft(psi)
+    out = np.concatenate([y[n // 2:], y[:n // 2]]) / np.sqrt(n)
+    out[1::2] *= -1
+    return out / np.linalg.norm(out)
+
+
+def p1m1_initial():
+    """mode vacuum (mode::init with minit = 0, Mode.cc:336-339) (x) the particle's wave packet"""
+    return np.kron(basis(3, 0), particle_wavepacket(3))
+
+
 def ground(dim):
     psi = np.zeros(dim, np.complex128)
     psi[0] = 1

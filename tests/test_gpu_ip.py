@@ -6,7 +6,7 @@ import pytest
 
 import cppqed_amd as Q
 import oracle_lib as O
-from sysdefs import coherent, ground
+from sysdefs import P1M1, coherent, ground, p1m1_initial
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +46,8 @@ def test_apply_heff_interaction_picture(spec, prec, tol):
     ("jc_ip_stepbacks", JC_IP, ground(60), 2.0, 16, 1.0),
     ("jc_uip", JC_UIP, ground(24), 2.0, 16, 10.0),
     ("plm_ip", PLM_IP, coherent(0.5, 20), 2.0, 16, 10.0),
+    # the 1particle1mode system whose golden runs pin the oracle (tests/test_oracle.py)
+    ("p1m1", P1M1, p1m1_initial(), 0.5, 24, 10.0),
 ])
 def test_ip_trajectories_match_oracle(name, spec, psi0, T, ntraj, ovt, layout):
     """With overshootTol = 1 dpLimit step-backs are frequent, and the redone step runs from the

@@ -22,6 +22,8 @@ SPECS = {
     "pumped_mode": "free mode cutoff=7 delta=-2 kappa=0 eta=1.5",
     "pcav_small": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1\nfree particle fin=3 omrec=0.3\nia pcav 0 1 u0=-1 kcav=1 mf=cos",
     "pcav_sin": "free mode cutoff=3 delta=1 kappa=0.5 eta=0.5\nfree particle fin=4 omrec=1\nia pcav 0 1 u0=2 kcav=1 mf=sin",
+    "pumped_particle_sch": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin\n"
+                           "ia pcav_orth 0 1 u0=-1",
     "bs_chain": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0\n"
                 "free mode cutoff=2 delta=0 kappa=1 eta=0\nia bs 0 1 g=0.5\nia bs 1 2 g=0.25 gI=0.5\nsystem composite",
     "zz_triangle": "free qbit deltaA=0 gamma=0.2 eta=0.7\nfree qbit deltaA=0.3 gamma=0.1 eta=0.7\n"
@@ -71,6 +73,12 @@ IP_SPECS = {
               "ia jc 0 1 g=3 gI=0.5",
     "bs_mixed_pictures": "free mode cutoff=3 delta=-1 kappa=1 eta=1 picture=IP\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0.2\n"
                          "ia bs 0 1 g=0.5\nsystem composite",
+    "pcav_orth_ip": "free mode cutoff=3 delta=-10 kappa=10 eta=10 picture=IP\n"
+                    "free particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin picture=IP\nia pcav_orth 0 1 u0=-1",
+    "pcav_orth_cos": "free mode cutoff=4 delta=-2 kappa=1 eta=2 picture=IP\n"
+                     "free particle fin=4 omrec=0.5 vclass=3 kpart=1 mf=cos picture=IP\nia pcav_orth 0 1 u0=2",
+    "pcav_disp_ip": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1 picture=IP\nfree particle fin=3 omrec=0.3 picture=IP\n"
+                    "ia pcav 0 1 u0=-1 kcav=1 mf=cos",
 }
 
 

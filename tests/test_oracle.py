@@ -7,7 +7,7 @@ import pytest
 from scipy.linalg import expm
 
 import oracle_lib as O
-from sysdefs import JC60, PLM20, PTLA, coherent, ground
+from sysdefs import JC60, P1M1, P1M1_MASTER, PLM20, PTLA, coherent, ground, p1m1_initial
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -128,6 +128,39 @@ def test_ptla_matches_qbit_element():
     a = O.System("free ptla gamma=10 deltaA=-10 eta=8")
     b = O.System("free qbit deltaA=-10 gamma=10 eta=8")
     np.testing.assert_allclose(a.dense_heff(), b.dense_heff(), atol=1e-14)
+
+
+P1M1_MODES = {"dc": O.run_spec("dc", dc=1, T=0.1), "dt": O.run_spec("Dt", Dt=0.01, T=0.1),
+              "ndt": O.run_spec("NDt", Dt=0.01, NDt=10)}
+P1M1_MASTER_MODES = {"dc": O.run_spec("dc", dc=1, T=0.03), "dt": O.run_spec("Dt", Dt=0.01, T=0.03),
+                     "ndt": O.run_spec("NDt", Dt=0.01, NDt=3)}
+
+
+@pytest.mark.parametrize("mode", ["dc", "dt", "ndt"])
+@pytest.mark.parametrize("kind", ["single", "ensemble", "master"])
+def test_1particle1mode_golden(mode, kind):
+    """Testing/expected/run_1particle1mode_{dc,dt,ndt}_{single,ensemble,master}: a binary system in
+    the interaction picture with a pumped particle (K = 1, 2 Tridiagonal shifts, Exact k^2 phases),
+    the particle-cavity interference term and the particle's FFT-based <X>, DEV(X) (pcg64 seed
+    1001, stream 1; ensembles of 3; the Master runs in the unitary interaction picture).
+    The initial DEV(X) = 2.2e-7 is the root of a cancellation (<X^2> - <X>^2 of a state localised
+    on one grid point): that column is compared to 1e-6 absolute."""
+    s = O.System(P1M1 if kind != "master" else P1M1_MASTER)
+    p = O.params(epsRel=1e-6, epsAbs=1e-5, rng="pcg64", seed=1001, stream=1)
+    g = load_golden(f"run_1particle1mode_{mode}_{kind}")
+    psi0 = p1m1_initial()
+    if kind == "single":
+        rows = O.mcwf_run(s, psi0, p, P1M1_MODES[mode])["rows"]
+    elif kind == "ensemble":
+        r = P1M1_MODES[mode] if mode != "dc" else O.run_spec("Dt", Dt=0.1, T=0.1)
+        rows, _ = O.ensemble_run(s, psi0, p, r, 3)
+    else:
+        rows, _ = O.master_run(s, np.outer(psi0, psi0.conj()), p, P1M1_MASTER_MODES[mode])
+    assert rows.shape == g.shape, (rows.shape, g.shape)
+    tol = np.full(g.shape[1], 1e-8)
+    tol[-1] = 1e-6
+    bad = np.abs(rows - g) > 1e-5 * np.abs(g) + tol
+    assert not bad.any(), (np.argwhere(bad)[:5], rows[bad][:5], g[bad][:5])
 
 
 # ---------------------------------------------------------------- dynamics vs closed forms
