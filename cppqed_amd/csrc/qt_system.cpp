@@ -393,11 +393,14 @@ struct Builder {
     int kcav = (int)num(kv, "kcav", 1);
     auto mf = kv.find("mf");
     std::string mfs = mf == kv.end() ? "cos" : mf->second;
-    if (mfs != "cos" && mfs != "sin") throw std::invalid_argument("pcav mode function must be cos or sin");
+    if (mfs != "cos" && mfs != "sin" && mfs != "plus" && mfs != "minus")
+      throw std::invalid_argument("unknown pcav mode function");
+    if (fp.pumped) throw std::invalid_argument("pcav with a pumped particle is not built");
     const double sgn = mfs == "sin" ? -1. : 1.;
     const int K = 2 * kcav;
     LocalOp t; t.dims = {fm.dim, fp.dim};
-    if (u0 != 0. && K < fp.dim) {
+    // a complex mode function (plus / minus) has no dispersive term (ParticleCavity.cc:27)
+    if (u0 != 0. && K < fp.dim && (mfs == "cos" || mfs == "sin")) {
       LocalOp n; n.dims = {fm.dim};
       for (int a = 0; a < fm.dim; ++a) n.entries.push_back({a, a, (double)a, 0.});
       LocalOp c; c.dims = {fp.dim};
@@ -409,6 +412,42 @@ struct Builder {
     }
     s.hTerms.push_back({{m, p}, t});
     iaHF = std::max(iaHF, std::fabs(u0) * fm.dim);  // RF{"Unot", uNot, mode dim}
+  }
+
+  // mfComposition(particle, (mf0, K), (mf1, K)), Particle.cc:290-345, for positive wavenumbers
+  static LocalOp mfComposition(int dim, const std::string& a, const std::string& b, int nK) {
+    LocalOp id; id.dims = {dim};
+    for (int q = 0; q < dim; ++q) id.entries.push_back({q, q, 1., 0.});
+    auto cos2 = [&]() { return scale(plus(expINKX(dim, 2 * nK), expINKX(dim, -2 * nK)), 0.5); };
+    auto sin2 = [&]() { return scale(plus(expINKX(dim, 2 * nK), scale(expINKX(dim, -2 * nK), -1.)), 1. / cd(0., 2.)); };
+    const cd half_i = cd(0., 0.5);
+    if (a == "sin" && b == "sin") return scale(plus(id, scale(cos2(), -1.)), 0.5);
+    if (a == "cos" && b == "cos") return scale(plus(id, cos2()), 0.5);
+    if ((a == "sin" && b == "cos") || (a == "cos" && b == "sin")) return scale(sin2(), 0.5);
+    if (a == b) return id;  // plus-plus, minus-minus
+    if (a == "plus" && b == "minus") return expINKX(dim, -2 * nK);
+    if (a == "minus" && b == "plus") return expINKX(dim, 2 * nK);
+    if ((a == "plus" && b == "sin") || (a == "sin" && b == "minus")) return scale(plus(expINKX(dim, -2 * nK), scale(id, -1.)), half_i);
+    if ((a == "plus" && b == "cos") || (a == "cos" && b == "minus")) return scale(plus(id, expINKX(dim, -2 * nK)), 0.5);
+    if ((a == "minus" && b == "cos") || (a == "cos" && b == "plus")) return scale(plus(id, expINKX(dim, 2 * nK)), 0.5);
+    return scale(plus(expINKX(dim, 2 * nK), scale(id, -1.)), -half_i);  // minus-sin, sin-plus
+  }
+
+  // ParticleTwoModes with equal wavenumbers (ParticleTwoModes.cc:35-46, the special Hamiltonian):
+  // tridiagPlusHC_overI(f a0^dagger (x) a1 (x) mfComposition(mf0, mf1)), f = sign(u0) sqrt(u0 u1) e^{i phi}
+  void particleTwoModes(int m0, int m1, int p, const KV& kv) {
+    const FreeInfo& f0 = freeAt(m0); const FreeInfo& f1 = freeAt(m1); const FreeInfo& fp = particleAt(p);
+    if (f0.kind != "mode" || f1.kind != "mode") throw std::invalid_argument("ptm couples two modes and a particle");
+    double u0 = num(kv, "u0", -1.), u1 = num(kv, "u1", -1.), phi = num(kv, "phi", 0.);
+    if (u0 * u1 < 0) throw std::invalid_argument("Unots sign discrepancy");
+    int k0 = (int)num(kv, "k0", 1), k1 = (int)num(kv, "k1", 1);
+    if (k0 != k1) throw std::invalid_argument("ptm with unequal wavenumbers is not built");
+    auto g = [&](const char* key) { auto it = kv.find(key); return it == kv.end() ? std::string("plus") : it->second; };
+    const std::string mf0 = g("mf0"), mf1 = g("mf1");
+    const cd fac = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * u1) * std::polar(1., phi);
+    LocalOp x = scale(kron(kron(adjoint(destroy(f0.dim)), destroy(f1.dim)), mfComposition(fp.dim, mf0, mf1, k0)), fac);
+    s.hTerms.push_back({{m0, m1, p}, scale(plus(x, adjoint(x)), 1. / cd(0., 1.))});
+    iaHF = std::max(iaHF, std::max(std::fabs(u0) * std::sqrt((double)f0.dim), std::fabs(u1) * std::sqrt((double)f1.dim)));
   }
 
   std::vector<std::vector<char>> freeSqrt;
@@ -448,13 +487,17 @@ System* buildFromSpec(const std::string& spec) {
       else if (kind == "particle") b.particle(kv);
       else throw std::invalid_argument("unknown free element " + kind);
     } else if (what == "ia") {
-      if (axes.size() != 2) throw std::invalid_argument("interaction needs two axes");
-      if (axes[0] == axes[1]) throw std::invalid_argument("interaction axes must differ");
+      const size_t want = kind == "ptm" ? 3 : 2;
+      if (axes.size() != want) throw std::invalid_argument(kind == "ptm" ? "ptm needs three axes" : "interaction needs two axes");
+      for (size_t i = 0; i < axes.size(); ++i)
+        for (size_t j = i + 1; j < axes.size(); ++j)
+          if (axes[i] == axes[j]) throw std::invalid_argument("interaction axes must differ");
       if (kind == "jc") b.jc(axes[0], axes[1], kv);
       else if (kind == "bs") b.bs(axes[0], axes[1], kv);
       else if (kind == "zz") b.zz(axes[0], axes[1], kv);
       else if (kind == "pcav") b.pcav(axes[0], axes[1], kv);
       else if (kind == "pcav_orth") b.pcavOrth(axes[0], axes[1], kv);
+      else if (kind == "ptm") b.particleTwoModes(axes[0], axes[1], axes[2], kv);
       else throw std::invalid_argument("unknown interaction " + kind);
       ++nIa;
     } else if (what == "system") {

@@ -655,8 +655,10 @@ Interaction makePcav(const System& s, int m, int p, const std::map<std::string, 
   double u0 = num(kv, "u0", 0);
   int kcav = (int)num(kv, "kcav", 1);
   auto it = kv.find("mf");
-  double sgn = (it != kv.end() && it->second == "sin") ? -1. : 1.;
-  if (u0 != 0. && 2 * kcav < fp.dim) {
+  const std::string mfs = it == kv.end() ? "cos" : it->second;
+  double sgn = mfs == "sin" ? -1. : 1.;
+  // dispersive(): only for a real mode function (ParticleCavity.cc:26-32)
+  if (u0 != 0. && 2 * kcav < fp.dim && (mfs == "cos" || mfs == "sin")) {
     CV nd(fm.dim); for (int n = 0; n < fm.dim; ++n) nd[n] = (double)n;   // nop = Tridiagonal(mainDiagonal(1,0,dim))
     Tridiag t = product(scaled(unary(nd, 0, CV(), CV()), u0 * sgn), cosNKX(fp.dim, 2 * kcav));
     ia.hOverIs.push_back(scaled(t, 1. / cd(0., 2.)));
@@ -687,6 +689,49 @@ Interaction makePcavOrth(const System& s, int m, int p, const std::map<std::stri
   return ia;
 }
 
+// mfComposition, Particle.cc:290-345 (positive wavenumbers), with the particle's factors
+// furnished when it is Exact; identity(dim) is a bare main diagonal (Tridiagonal.cc:64-69)
+Tridiag mfComposition(const Free& fp, const std::string& a, const std::string& b, int nK) {
+  auto fur = [&](Tridiag t) { if (fp.hasEx) furnishWithFreqs(t, fp.exd); return t; };
+  CV ones(fp.dim, 1.);
+  Tridiag id = unary(ones, 0, CV(), CV());
+  auto e = [&](int k) { return fur(expINKX(fp.dim, k)); };
+  auto cos2 = [&]() { return scaled(sum(e(2 * nK), e(-2 * nK)), 0.5); };
+  auto sin2 = [&]() { return scaled(sum(e(2 * nK), negated(e(-2 * nK))), 1. / cd(0., 2.)); };
+  if (a == "sin" && b == "sin") return scaled(sum(id, negated(cos2())), 0.5);
+  if (a == "cos" && b == "cos") return scaled(sum(id, cos2()), 0.5);
+  if ((a == "sin" && b == "cos") || (a == "cos" && b == "sin")) return scaled(sin2(), 0.5);
+  if (a == b) return id;
+  if (a == "plus" && b == "minus") return e(-2 * nK);
+  if (a == "minus" && b == "plus") return e(2 * nK);
+  if ((a == "plus" && b == "sin") || (a == "sin" && b == "minus")) return scaled(sum(e(-2 * nK), negated(id)), cd(0., 0.5));
+  if ((a == "plus" && b == "cos") || (a == "cos" && b == "minus")) return scaled(sum(id, e(-2 * nK)), 0.5);
+  if ((a == "minus" && b == "cos") || (a == "cos" && b == "plus")) return scaled(sum(id, e(2 * nK)), 0.5);
+  return scaled(sum(e(2 * nK), negated(id)), cd(0., -0.5));
+}
+
+// ParticleTwoModes, special Hamiltonian for equal wavenumbers (ParticleTwoModes.cc:19-46):
+// tridiagPlusHC_overI(factor aop(mode0)^dagger aop(mode1) mfComposition(part, mf0, mf1)),
+// factor = sign(u0) sqrt(u0 u1) e^{i phi}; frequencies RF{Unot0, u0, sqrt(dim0)}, RF{Unot1, u1, sqrt(dim1)}
+Interaction makePTM(const System& s, int m0, int m1, int p, const std::map<std::string, std::string>& kv) {
+  Interaction ia; ia.kind = "ptm"; ia.axes = {m0, m1, p};
+  const Free& f0 = s.frees[m0]; const Free& f1 = s.frees[m1]; const Free& fp = s.frees[p];
+  double u0 = num(kv, "u0", -1.), u1 = num(kv, "u1", -1.), phi = num(kv, "phi", 0.);
+  int k0 = (int)num(kv, "k0", 1), k1 = (int)num(kv, "k1", 1);
+  if (k0 != k1 || u0 * u1 < 0) throw std::invalid_argument("oracle: ptm");
+  auto g = [&](const char* key) { auto it = kv.find(key); return it == kv.end() ? std::string("plus") : it->second; };
+  cd fac = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * u1) * std::exp(cd(0., phi));
+  Tridiag a0 = aop(f0.dim), a1 = aop(f1.dim);
+  if (f0.hasEx) furnishWithFreqs(a0, f0.exd);
+  if (f1.hasEx) furnishWithFreqs(a1, f1.exd);
+  Tridiag t = product(product(scaled(hc(a0), fac), a1), mfComposition(fp, g("mf0"), g("mf1"), k0));
+  ia.hOverIs.push_back(scaled(sum(t, hc(t)), 1. / cd(0., 1.)));
+  ia.hasHa = true; ia.timeDep = true;
+  ia.realFreqs.push_back({u0, std::sqrt((double)f0.dim)});
+  ia.realFreqs.push_back({u1, std::sqrt((double)f1.dim)});
+  return ia;
+}
+
 System* buildSystem(const std::string& spec) {
   std::unique_ptr<System> s(new System);
   std::string line;
@@ -712,6 +757,7 @@ System* buildSystem(const std::string& spec) {
       else if (kind == "zz") s->ias.push_back(makeZZ(*s, axes.at(0), axes.at(1), kv));
       else if (kind == "pcav") s->ias.push_back(makePcav(*s, axes.at(0), axes.at(1), kv));
       else if (kind == "pcav_orth") s->ias.push_back(makePcavOrth(*s, axes.at(0), axes.at(1), kv));
+      else if (kind == "ptm") s->ias.push_back(makePTM(*s, axes.at(0), axes.at(1), axes.at(2), kv));
       else throw std::invalid_argument("oracle: unknown interaction " + kind);
     } else if (what == "system") {
       forced = kind == "binary" ? 1 : 0;  // binary::make vs composite::make chosen explicitly

@@ -95,6 +95,26 @@ def p1m1_initial():
     return np.kron(basis(3, 0), particle_wavepacket(3))
 
 
+# CPPQEDscripts/Ring with the golden runs' options (--etaM 10 --etaP 10 --cutoffM 3 --cutoffP 3
+# --fin 3): particle (x) mode "plus" (x) mode "minus", composite, interaction picture; each mode's
+# detuning -10 is shifted by -Unot (complex mode functions, Ring.cc:22-23); the two
+# ParticleAlongCavity acts have no Hamiltonian part (complex mode functions, unpumped particle);
+# the three-body ParticleTwoModes act on (plus, minus, particle).
+RING = ("free particle fin=3 omrec=1 picture=IP\n"
+        "free mode cutoff=3 delta=-9 kappa=10 eta=10 picture=IP\n"
+        "free mode cutoff=3 delta=-9 kappa=10 eta=10 picture=IP\n"
+        "ia pcav 1 0 u0=-1 kcav=1 mf=plus\n"
+        "ia pcav 2 0 u0=-1 kcav=1 mf=minus\n"
+        "ia ptm 1 2 0 u0=-1 u1=-1 phi=0 k0=1 k1=1 mf0=plus mf1=minus\n"
+        "system composite")
+RING_MASTER = RING.replace("eta=10 picture=IP", "eta=10 picture=UIP")
+
+
+def ring_initial():
+    """init(pp) (x) init(pmP) (x) init(pmM): the particle's wave packet and two vacua"""
+    return np.kron(np.kron(particle_wavepacket(3), basis(3, 0)), basis(3, 0))
+
+
 def ground(dim):
     psi = np.zeros(dim, np.complex128)
     psi[0] = 1

@@ -79,6 +79,8 @@ IP_SPECS = {
                      "free particle fin=4 omrec=0.5 vclass=3 kpart=1 mf=cos picture=IP\nia pcav_orth 0 1 u0=2",
     "pcav_disp_ip": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1 picture=IP\nfree particle fin=3 omrec=0.3 picture=IP\n"
                     "ia pcav 0 1 u0=-1 kcav=1 mf=cos",
+    "ring": __import__("sysdefs").RING,
+    "ring_sin_cos": __import__("sysdefs").RING.replace("mf0=plus mf1=minus", "mf0=sin mf1=cos"),
 }
 
 

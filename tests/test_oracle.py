@@ -7,7 +7,8 @@ import pytest
 from scipy.linalg import expm
 
 import oracle_lib as O
-from sysdefs import JC60, P1M1, P1M1_MASTER, PLM20, PTLA, coherent, ground, p1m1_initial
+from sysdefs import (JC60, P1M1, P1M1_MASTER, PLM20, PTLA, RING, RING_MASTER, coherent, ground, p1m1_initial,
+                     ring_initial)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -159,6 +160,34 @@ def test_1particle1mode_golden(mode, kind):
     assert rows.shape == g.shape, (rows.shape, g.shape)
     tol = np.full(g.shape[1], 1e-8)
     tol[-1] = 1e-6
+    bad = np.abs(rows - g) > 1e-5 * np.abs(g) + tol
+    assert not bad.any(), (np.argwhere(bad)[:5], rows[bad][:5], g[bad][:5])
+
+
+RING_MASTER_MODES = {"dc": O.run_spec("dc", dc=1, T=0.0002), "dt": O.run_spec("Dt", Dt=0.0001, T=0.0002),
+                     "ndt": O.run_spec("NDt", Dt=0.0001, NDt=2)}
+
+
+@pytest.mark.parametrize("mode", ["dc", "dt", "ndt"])
+@pytest.mark.parametrize("kind", ["single", "ensemble", "master"])
+def test_ring_golden(mode, kind):
+    """Testing/expected/run_Ring_{dc,dt,ndt}_{single,ensemble,master}: a composite of three frees
+    with a three-body act on the axes (1, 2, 0) (slice iteration with a permuted V), interaction
+    picture, the composite's frequency rule (acts ignored, Q11)."""
+    s = O.System(RING if kind != "master" else RING_MASTER)
+    p = O.params(epsRel=1e-6, epsAbs=1e-5, rng="pcg64", seed=1001, stream=1)
+    g = load_golden(f"run_Ring_{mode}_{kind}")
+    psi0 = ring_initial()
+    if kind == "single":
+        rows = O.mcwf_run(s, psi0, p, P1M1_MODES[mode])["rows"]
+    elif kind == "ensemble":
+        r = P1M1_MODES[mode] if mode != "dc" else O.run_spec("Dt", Dt=0.1, T=0.1)
+        rows, _ = O.ensemble_run(s, psi0, p, r, 3)
+    else:
+        rows, _ = O.master_run(s, np.outer(psi0, psi0.conj()), p, RING_MASTER_MODES[mode])
+    assert rows.shape == g.shape, (rows.shape, g.shape)
+    tol = np.full(g.shape[1], 1e-8)
+    tol[5] = 1e-6   # the particle's DEV(X): root of a cancellation for the one-grid-point state
     bad = np.abs(rows - g) > 1e-5 * np.abs(g) + tol
     assert not bad.any(), (np.argwhere(bad)[:5], rows[bad][:5], g[bad][:5])
 
