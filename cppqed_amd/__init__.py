@@ -57,6 +57,20 @@ class ServerStats(C.Structure):
                 ("kernel_ms", C.c_double), ("active", C.c_int), ("waiting", C.c_int), ("bytes_in_use", C.c_size_t)]
 
 
+class TrajState(C.Structure):
+    _fields_ = [("t", C.c_double), ("dtDid", C.c_double), ("dtTry", C.c_double), ("ordinal", C.c_ulonglong),
+                ("counters", C.c_longlong * 8)]
+
+
+class MasterState(C.Structure):
+    _fields_ = [("t", C.c_double), ("dtDid", C.c_double), ("dtTry", C.c_double), ("counters", C.c_longlong * 8),
+                ("done", C.c_int), ("t_ss", C.c_double)]
+
+
+# counter slots (qt_mcwf_counters)
+C_MCWF, C_ODE, C_FAILED, C_DERIVS, C_TRIES, C_STEPBACKS, C_OVERSHOOTS, C_JUMPS = range(8)
+
+
 def lib():
     """Loads libqtraj.so; raises if it has not been built."""
     global _lib
@@ -82,6 +96,10 @@ def lib():
         l.qt_mcwf_jumplog.argtypes = [vp, ip, vp, vp, ip, C.POINTER(C.c_int)]
         l.qt_mcwf_counters.argtypes = [vp, vp, vp]
         l.qt_mcwf_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        l.qt_mcwf_get_states.argtypes = [vp, ip, ip, C.POINTER(TrajState), vp, vp, vp]
+        l.qt_mcwf_set_states.argtypes = [vp, ip, ip, C.POINTER(TrajState), vp, vp, vp]
+        l.qt_master_get_point.argtypes = [vp, ip, C.POINTER(MasterState), vp]
+        l.qt_master_set_point.argtypes = [vp, ip, C.POINTER(MasterState), vp]
         l.qt_mcwf_plain_run.argtypes = [vp, C.POINTER(McwfParams), C.c_ulonglong, vp, vp, ip, vp, vp, vp, vp, ip, vp]
         l.qt_batch_variants.argtypes = [vp, ip]
         l.qt_batch_create.argtypes = [vp, ip, C.POINTER(BatchParams), C.POINTER(vp)]
@@ -351,6 +369,94 @@ class Ensemble:
         check(lib().qt_mcwf_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    # ------------------------------------------------------------ checkpoint / resume
+
+    def get_states(self, first=0, count=None):
+        """(TrajState array, psi[count, dim], jump times[count, cap], jump channels[count, cap])."""
+        count = self.n - first if count is None else count
+        st = (TrajState * count)()
+        cap = max(0, self.pars.jump_log_cap)
+        psi = np.zeros((count, self.system.dim), np.complex128)
+        jt = np.zeros((count, max(cap, 1)))
+        jc = np.zeros((count, max(cap, 1)), np.int32)
+        check(lib().qt_mcwf_get_states(self.h, first, count, st, psi.ctypes.data, jt.ctypes.data, jc.ctypes.data))
+        return st, psi, jt, jc
+
+    def set_states(self, st, psi, jt=None, jc=None, first=0):
+        count = len(st)
+        psi = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128).reshape(count, self.system.dim))
+        arr = (TrajState * count)(*st)
+        jt = None if jt is None else np.ascontiguousarray(jt, dtype=np.float64)
+        jc = None if jc is None else np.ascontiguousarray(jc, dtype=np.int32)
+        check(lib().qt_mcwf_set_states(self.h, first, count, arr, psi.ctypes.data,
+                                       None if jt is None else jt.ctypes.data, None if jc is None else jc.ctypes.data))
+
+    def records(self):
+        """The ensemble as state-file records (one per trajectory), in archive order.  The device
+        keeps no maximal-overshoot or norm-deviation figures (logging only): they are written as 0."""
+        from . import statefile as SF
+        st, psi, jt, jc = self.get_states()
+        out = []
+        for i in range(self.n):
+            c = st[i].counters
+            nj = min(int(c[C_JUMPS]), self.pars.jump_log_cap)
+            out.append(SF.TrajRecord(
+                psi=psi[i].reshape(self.system.dims), t=st[i].t, dtDid=st[i].dtDid, dtTry=st[i].dtTry,
+                ode_derivs=c[C_DERIVS], ode_steps=c[C_ODE], ode_failed=c[C_FAILED],
+                rng=SF.format_philox(self.pars.seed, self.pars.traj0 + i, st[i].ordinal),
+                mcwf_steps=c[C_MCWF], overshot=c[C_OVERSHOOTS], tolerance_overshot=c[C_STEPBACKS],
+                jumps=[(float(jt[i, k]), int(jc[i, k])) for k in range(nj)]))
+        return out
+
+    def save_state(self, path, append=True):
+        """Appends the ensemble as one archive to a state file (writeViaSStream,
+        CPPQEDutils/Trajectory.h:395-410): an 'Ensemble of MCWF_Trajectory' archive, or an
+        'MCWF_Trajectory' archive for a single trajectory."""
+        from . import statefile as SF
+        recs = self.records()
+        kind = SF.MCWF_ID if self.n == 1 else SF.ENSEMBLE_ID
+        SF.write(path, [SF.Frame(trajectory_id=kind, rank=len(self.system.dims), trajs=recs)], append=append)
+
+    def load_state(self, path, rng="keep"):
+        """Continues from the last archive of a state file (readViaSStream, Trajectory.h:392-405).
+        rng='keep' requires the archived Philox records of this ensemble's own streams (seed and
+        trajectory keys) and continues them; rng='restart' takes psi, times and counters from any
+        archive (also the reference's pcg64 ones) and restarts this ensemble's own streams."""
+        from . import statefile as SF
+        fr = SF.read_last(path)
+        if fr.trajectory_id not in (SF.MCWF_ID, SF.ENSEMBLE_ID):
+            raise SF.StateFileError("not an MCWF archive: %r" % fr.trajectory_id)
+        if len(fr.trajs) != self.n:
+            raise SF.StateFileError("archive holds %d trajectories, the ensemble %d" % (len(fr.trajs), self.n))
+        cap = max(0, self.pars.jump_log_cap)
+        st = (TrajState * self.n)()
+        psi = np.zeros((self.n, self.system.dim), np.complex128)
+        jt = np.zeros((self.n, max(cap, 1)))
+        jc = np.zeros((self.n, max(cap, 1)), np.int32)
+        for i, r in enumerate(fr.trajs):
+            if r.psi.size != self.system.dim or tuple(r.psi.shape) != tuple(self.system.dims):
+                raise SF.StateFileError("archived state has shape %s, the system %s" % (r.psi.shape, self.system.dims))
+            if rng == "keep":
+                ph = SF.parse_philox(r.rng)
+                if ph is None or ph[:2] != (self.pars.seed, self.pars.traj0 + i):
+                    raise SF.StateFileError("trajectory %d: the archive's random stream is not this ensemble's "
+                                            "(use rng='restart' to continue with the ensemble's own streams)" % i)
+                st[i].ordinal = ph[2]
+            elif rng == "restart":
+                st[i].ordinal = 0
+            else:
+                raise ValueError("rng must be 'keep' or 'restart'")
+            st[i].t, st[i].dtDid, st[i].dtTry = r.t, r.dtDid, r.dtTry
+            c = st[i].counters
+            c[C_MCWF], c[C_ODE], c[C_FAILED], c[C_DERIVS] = r.mcwf_steps, r.ode_steps, r.ode_failed, r.ode_derivs
+            c[C_TRIES] = r.ode_steps + r.ode_failed
+            c[C_STEPBACKS], c[C_OVERSHOOTS], c[C_JUMPS] = r.tolerance_overshot, r.overshot, len(r.jumps)
+            psi[i] = r.psi.reshape(-1)
+            for k, (tj, ch) in enumerate(r.jumps[:cap]):
+                jt[i, k], jc[i, k] = tj, ch
+        self.set_states(list(st), psi, jt, jc)
+        return fr
+
 
 def batch_variants():
     """Compiled mixed-batch layouts: (waves per work group, rows/lane and ELL register width of the
@@ -531,3 +637,51 @@ class MasterSweep:
         n = C.c_int()
         check(lib().qt_master_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # ------------------------------------------------------------ checkpoint / resume
+
+    def get_point(self, point):
+        st = MasterState()
+        rho = np.zeros((self.dim, self.dim), np.complex128)
+        check(lib().qt_master_get_point(self.h, point, C.byref(st), rho.ctypes.data))
+        return st, rho
+
+    def set_point(self, point, st, rho):
+        rho = np.ascontiguousarray(np.asarray(rho, dtype=np.complex128).reshape(self.dim, self.dim))
+        check(lib().qt_master_set_point(self.h, point, C.byref(st), rho.ctypes.data))
+
+    def save_state(self, path, point=0, append=True):
+        """Appends one point as a 'Master' archive (Master::stateIO, Master.h:89-102)."""
+        from . import statefile as SF
+        st, rho = self.get_point(point)
+        dims = self.systems[point].dims
+        c = st.counters
+        rec = SF.MasterRecord(rho=rho.reshape(tuple(dims) + tuple(dims)), t=st.t, dtDid=st.dtDid, dtTry=st.dtTry,
+                              ode_derivs=c[C_DERIVS], ode_steps=c[C_ODE], ode_failed=c[C_FAILED])
+        SF.write(path, [SF.Frame(trajectory_id=SF.MASTER_ID, rank=2 * len(dims), master=rec)], append=append)
+
+    def load_state(self, path, point=0):
+        """Continues one point from the last archive of a state file ('Master', or 'ArrayOnly' for a
+        bare density operator as an initial condition)."""
+        from . import statefile as SF
+        fr = SF.read_last(path)
+        st = MasterState()
+        if fr.trajectory_id == SF.MASTER_ID:
+            m = fr.master
+            rho = m.matrix()
+            st.t, st.dtDid, st.dtTry = m.t, m.dtDid, m.dtTry
+            c = st.counters
+            c[C_MCWF], c[C_ODE], c[C_FAILED], c[C_DERIVS] = m.ode_steps, m.ode_steps, m.ode_failed, m.ode_derivs
+            c[C_TRIES] = m.ode_steps + m.ode_failed
+        elif fr.trajectory_id == SF.ARRAY_ONLY:
+            cur, _ = self.get_point(point)
+            d = self.dim
+            rho = fr.array.reshape(d, d)
+            st.t, st.dtDid, st.dtTry = cur.t, cur.dtDid, cur.dtTry
+        else:
+            raise SF.StateFileError("not a Master archive: %r" % fr.trajectory_id)
+        if rho.shape != (self.dim, self.dim):
+            raise SF.StateFileError("archived density operator has the wrong dimension")
+        st.t_ss = -1.0
+        self.set_point(point, st, rho)
+        return fr

@@ -706,6 +706,70 @@ int qt_mcwf_counters(qt_ens* e, long long* per, long long* totals) {
   });
 }
 
+// Checkpoint / resume (MCWF_Trajectory::stateIO, MCWF_Trajectory.h:134-147): psi, time,
+// dtDid, dtTry, RNG position, counters and jump records of trajectories [first, first+count).
+int qt_mcwf_get_states(qt_ens* e, int first, int count, qt_traj_state* st, double* psi, double* jt, int* jch) {
+  return guard([&] {
+    if (!e || !st || first < 0 || count < 0 || first + count > e->n) throw Fail(QT_EINVAL, "bad trajectory range");
+    if (!count) return;
+    const size_t d = e->sys->h->dim, cap = e->p.jump_log_cap, n = count;
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    std::vector<double> t(n), dd(n), dt(n);
+    std::vector<unsigned long long> ord(n);
+    std::vector<long long> c(n * qt::kCounters);
+    hipCheck(hipMemcpy(t.data(), e->t.p + first, n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(dd.data(), e->dtDid.p + first, n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(dt.data(), e->dtTry.p + first, n * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(ord.data(), e->ord.p + first, n * sizeof(unsigned long long), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(c.data(), e->cnt.p + (size_t)first * qt::kCounters, c.size() * sizeof(long long),
+                       hipMemcpyDeviceToHost), "D2H");
+    for (size_t i = 0; i < n; ++i) {
+      st[i].t = t[i]; st[i].dtDid = dd[i]; st[i].dtTry = dt[i]; st[i].ordinal = ord[i];
+      for (int k = 0; k < qt::kCounters; ++k) st[i].counters[k] = c[i * qt::kCounters + k];
+    }
+    if (psi) hipCheck(hipMemcpy(psi, e->psi.p + (size_t)first * d, n * d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
+    if (cap && jt) hipCheck(hipMemcpy(jt, e->jt.p + (size_t)first * cap, n * cap * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    if (cap && jch) hipCheck(hipMemcpy(jch, e->jch.p + (size_t)first * cap, n * cap * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+  });
+}
+
+int qt_mcwf_set_states(qt_ens* e, int first, int count, const qt_traj_state* st, const double* psi, const double* jt,
+                       const int* jch) {
+  return guard([&] {
+    if (!e || !st || !psi || first < 0 || count < 0 || first + count > e->n) throw Fail(QT_EINVAL, "bad trajectory range");
+    if (!count) return;
+    const size_t d = e->sys->h->dim, cap = e->p.jump_log_cap, n = count;
+    std::vector<double> t(n), dd(n), dt(n);
+    std::vector<unsigned long long> ord(n);
+    std::vector<long long> c(n * qt::kCounters);
+    for (size_t i = 0; i < n; ++i) {
+      if (!std::isfinite(st[i].t) || !std::isfinite(st[i].dtDid) || !(st[i].dtTry > 0.) || !std::isfinite(st[i].dtTry))
+        throw Fail(QT_EINVAL, "state with a non-finite time or a non-positive dtTry");
+      t[i] = st[i].t; dd[i] = st[i].dtDid; dt[i] = st[i].dtTry; ord[i] = st[i].ordinal;
+      for (int k = 0; k < qt::kCounters; ++k) c[i * qt::kCounters + k] = st[i].counters[k];
+    }
+    // the trajectories of an ensemble share one time (they all stop at each stream point): a load
+    // of the whole ensemble sets it, a partial load must match it
+    for (size_t i = 1; i < n; ++i)
+      if (t[i] != t[0]) throw Fail(QT_EINVAL, "the trajectories of an ensemble share one time");
+    const bool whole = first == 0 && count == e->n;
+    if (!whole && t[0] != e->time) throw Fail(QT_EINVAL, "a partial load must match the ensemble's time");
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hipCheck(hipMemcpy(e->psi.p + (size_t)first * d, psi, n * d * sizeof(double2), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(e->t.p + first, t.data(), n * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    // the picture origin coincides with the time at every saved instant (MCWF_Trajectory.h:140)
+    hipCheck(hipMemcpy(e->t0.p + first, t.data(), n * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(e->dtDid.p + first, dd.data(), n * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(e->dtTry.p + first, dt.data(), n * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(e->ord.p + first, ord.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(e->cnt.p + (size_t)first * qt::kCounters, c.data(), c.size() * sizeof(long long),
+                       hipMemcpyHostToDevice), "H2D");
+    if (cap && jt) hipCheck(hipMemcpy(e->jt.p + (size_t)first * cap, jt, n * cap * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    if (cap && jch) hipCheck(hipMemcpy(e->jch.p + (size_t)first * cap, jch, n * cap * sizeof(int), hipMemcpyHostToDevice), "H2D");
+    if (whole) e->time = t[0];
+  });
+}
+
 int qt_mcwf_kernel_ms(qt_ens* e, double* total, int* launches) {
   if (total) *total = e->kernelMs;
   if (launches) *launches = e->launches;
@@ -1122,6 +1186,57 @@ int qt_master_rho(qt_master* m, int point, double* rho) {
           const int i = 16 * (t / NB) + 4 * v + l / 16, j = 16 * (t % NB) + l % 16;
           if (i < d && j < d) { rho[2 * ((size_t)i * d + j)] = x[(4 * t + v) * 64 + l].x; rho[2 * ((size_t)i * d + j) + 1] = x[(4 * t + v) * 64 + l].y; }
         }
+  });
+}
+
+// Checkpoint / resume of one sweep point (Master::stateIO, Master.h:89-102)
+int qt_master_get_point(qt_master* m, int point, qt_master_state* st, double* rho) {
+  return guard([&] {
+    if (!m || !st || point < 0 || point >= m->n) throw Fail(QT_EINVAL, "bad arguments");
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hipCheck(hipMemcpy(&st->t, m->t.p + point, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(&st->dtDid, m->dtDid.p + point, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(&st->dtTry, m->dtTry.p + point, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(st->counters, m->cnt.p + (size_t)point * qt::kCounters, qt::kCounters * sizeof(long long),
+                       hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(&st->done, m->done.p + point, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    hipCheck(hipMemcpy(&st->t_ss, m->tss.p + point, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    if (rho) {
+      const int rc = qt_master_rho(m, point, rho);
+      if (rc != QT_OK) throw Fail(rc, "reading rho");
+    }
+  });
+}
+
+int qt_master_set_point(qt_master* m, int point, const qt_master_state* st, const double* rho) {
+  return guard([&] {
+    if (!m || !st || !rho || point < 0 || point >= m->n) throw Fail(QT_EINVAL, "bad arguments");
+    if (!std::isfinite(st->t) || !std::isfinite(st->dtDid) || !(st->dtTry > 0.) || !std::isfinite(st->dtTry))
+      throw Fail(QT_EINVAL, "state with a non-finite time or a non-positive dtTry");
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    // all points of a sweep share the stream times: point 0 sets the time, the others match it
+    if (point != 0) {
+      double t0 = 0.;
+      hipCheck(hipMemcpy(&t0, m->t.p, sizeof(double), hipMemcpyDeviceToHost), "D2H");
+      if (t0 != st->t) throw Fail(QT_EINVAL, "the points of a sweep share one time: load point 0 first");
+    }
+    const int d = m->d, NB = m->nb, DP = 16 * NB, NN = DP * DP;
+    std::vector<double2> x(NN, make_double2(0., 0.));
+    for (int t = 0; t < NB * NB; ++t)
+      for (int v = 0; v < 4; ++v)
+        for (int l = 0; l < 64; ++l) {
+          const int i = 16 * (t / NB) + 4 * v + l / 16, j = 16 * (t % NB) + l % 16;
+          if (i < d && j < d) x[(4 * t + v) * 64 + l] = make_double2(rho[2 * ((size_t)i * d + j)], rho[2 * ((size_t)i * d + j) + 1]);
+        }
+    hipCheck(hipMemcpy(m->x.p + (size_t)point * NN, x.data(), NN * sizeof(double2), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->t.p + point, &st->t, sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->dtDid.p + point, &st->dtDid, sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->dtTry.p + point, &st->dtTry, sizeof(double), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->cnt.p + (size_t)point * qt::kCounters, st->counters, qt::kCounters * sizeof(long long),
+                       hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->done.p + point, &st->done, sizeof(int), hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(m->tss.p + point, &st->t_ss, sizeof(double), hipMemcpyHostToDevice), "H2D");
+    if (point == 0) m->time = st->t;
   });
 }
 

@@ -129,6 +129,25 @@ int qt_mcwf_counters(qt_ens* ens, long long* per_traj, long long* totals);
 /* Average duration (ms) of the batched kernel launches so far, from HIP events on the launch stream. */
 int qt_mcwf_kernel_ms(qt_ens* ens, double* total_ms, int* launches);
 
+/* The complete dynamical state of one trajectory: what MCWF_Trajectory::stateIO archives
+ * (MCWF_Trajectory.h:134-147: array, time, ODE logger, dtDid, dtTry, random engine, MCWF logger). */
+typedef struct {
+  double t, dtDid, dtTry;
+  unsigned long long ordinal;   /* RNG position: counter of the trajectory's next MCWF step draw */
+  long long counters[8];        /* as qt_mcwf_counters; counters[7] = jumps so far */
+} qt_traj_state;
+
+/* Checkpoint of trajectories [first, first+count): st[count]; psi_host[count][dim] complex (may be
+ * NULL); jt/jch[count][jump_log_cap] jump records (may be NULL; the first min(counters[7], cap) are
+ * valid).  Waits for the device. */
+int qt_mcwf_get_states(qt_ens* ens, int first, int count, qt_traj_state* st, double* psi_host, double* jt, int* jch);
+/* Resume: overwrites trajectories [first, first+count) with archived states (the reading half of
+ * MCWF_Trajectory::stateIO, including t0 = t, MCWF_Trajectory.h:136-143).  All share one time; a
+ * load of the whole ensemble sets the ensemble's time, a partial one must match it.  jt/jch as
+ * above (may be NULL: the jump records are left as they are). */
+int qt_mcwf_set_states(qt_ens* ens, int first, int count, const qt_traj_state* st, const double* psi_host,
+                       const double* jt, const int* jch);
+
 /* ---------------------------------------------------------------- mixed batch */
 
 /* A mixed batch advances several ensembles (each its own system, trajectory count, state and
@@ -231,6 +250,21 @@ int qt_master_rho(qt_master* m, int point, double* rho_host);
 /* per point: steady-state flag, the time it was reached, current time, counters[8] */
 int qt_master_status(qt_master* m, int* done, double* t_ss, double* t, long long* counters);
 int qt_master_kernel_ms(qt_master* m, double* total_ms, int* launches);
+
+/* One sweep point's dynamical state: what Master::stateIO archives (Master.h:89-102: array,
+ * time, ODE logger, dtDid, dtTry) plus the sweep's steady-state flag. */
+typedef struct {
+  double t, dtDid, dtTry;
+  long long counters[8];   /* as qt_master_status */
+  int done;                /* steady state reached */
+  double t_ss;             /* time it was reached (-1: not yet) */
+} qt_master_state;
+
+/* Checkpoint of one point; rho_host (d*d complex, row-major) may be NULL.  Waits for the device. */
+int qt_master_get_point(qt_master* m, int point, qt_master_state* st, double* rho_host);
+/* Resume: overwrites one point (the reading half of Master::stateIO).  The points of a sweep
+ * share the stream times, so loading point 0 sets the sweep's time and the others must match it. */
+int qt_master_set_point(qt_master* m, int point, const qt_master_state* st, const double* rho_host);
 
 /* -------------------------------------------------------------- plain form */
 

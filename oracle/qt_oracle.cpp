@@ -18,7 +18,7 @@
 //   MCWF step .................. CPPQEDcore/quantumtrajectory/MCWF_Trajectory.h:103-118, 250-374
 //   ensemble ................... MCWF_Trajectory.h:213-234, 406-420; CPPQEDutils/StochasticTrajectory.h:220
 //   Master step ................ CPPQEDcore/quantumtrajectory/Master.h:152-208; CPPQEDutils/CMatrix.cc:14-30
-//   run loop ................... CPPQEDutils/Trajectory.h:43-50, 462-610
+//   run loop ................... CPPQEDutils/Trajectory.h:43-50, 462-610 (incl. continuing from a state file)
 //   RNG ........................ CPPQEDutils/Random.h:117-161 (pcg64) + libstdc++ generate_canonical;
 //                                Philox4x64-10 (Salmon et al., SC'11) as the counter-based generator.
 //
@@ -1058,14 +1058,14 @@ struct MCWF {
   double dpLimit, overshootTol;
   RngKind rk; Pcg64 pcg; uint64_t seed, traj, stepOrdinal = 0;
   long nMCWFSteps = 0, nStepBacks = 0, nOvershot = 0;
-  double maxNormDev = 0.;
+  double maxNormDev = 0., dpMaxOvershoot = 0., dpTolMaxOvershoot = 0.;  // MCWF_TrajectoryLogger.cc:45-67
   std::vector<JumpRec> jumps;
 
   MCWF(System* s, const CV& psi0, double dtInit, double epsRel, double epsAbs, double dpL, double ovT,
        RngKind k, uint64_t sd, uint64_t stream, Controller ctl)
       : sys(s), psi(psi0), ode(dtInit, epsRel, epsAbs), dpLimit(dpL), overshootTol(ovT), rk(k), pcg(sd, stream), seed(sd), traj(stream) {
     ode.controller = ctl;
-    if (sys->hasLi() && !t) { auto r = rates(*sys, psi.data()); manageTimeStep(r, 0, 0); }  // :114-117
+    if (sys->hasLi() && !t) { auto r = rates(*sys, psi.data()); manageTimeStep(r, 0, 0, false); }  // :114-117
   }
 
   double sample() { return rk == RngKind::Pcg64 ? pcg.uniform() : counter_uniform(seed, traj, stepOrdinal++); }
@@ -1092,15 +1092,19 @@ struct MCWF {
     maxNormDev = std::max(maxNormDev, std::fabs(1 - nrm));
   }
 
-  bool manageTimeStep(const std::vector<double>& r, double tCache, double dtDidCache) {  // :294-317
+  bool manageTimeStep(const std::vector<double>& r, double tCache, double dtDidCache, bool logControl = true) {  // :294-317
     double totalRate = 0; for (double x : r) totalRate += x;
     const double dtDid = ode.dtDid, dtTry = ode.dtTry;
     const double sugg = dpLimit / totalRate;
     if (totalRate * dtDid > overshootTol * dpLimit) {
       t = tCache; ode.dtDid = dtDidCache; ode.dtTry = sugg;
       nStepBacks++;
+      if (logControl) dpTolMaxOvershoot = std::max(dpTolMaxOvershoot, totalRate * dtDid);
       return true;
-    } else if (totalRate * dtTry > dpLimit) nOvershot++;
+    } else if (totalRate * dtTry > dpLimit) {
+      nOvershot++;
+      if (logControl) dpMaxOvershoot = std::max(dpMaxOvershoot, totalRate * dtTry);
+    }
     ode.dtTry = sys->hasHa() ? std::min(ode.dtTry, sugg) : sugg;
     return false;
   }
@@ -1409,6 +1413,107 @@ long orc_master_run(void* sp, const double* rho0, const orc_params* p, const orc
     } else {
       for (long count = 0; count <= run->NDt; ++count) { if (count) m.advance(run->Dt); stream(); }
     }
+    if (rho_out) std::memcpy(rho_out, m.rho.data(), sizeof(cd) * m.rho.size());
+    return nrow;
+  } catch (std::exception& e) { g_last_error = e.what(); return -1; }
+}
+
+// ---------------------------------------------------------------------------
+// Continuing from a saved state (trajectory::run with a state file, Trajectory.h:484-500:
+// the trajectory is constructed as usual, then every archived field overwrites it;
+// MCWF_Trajectory::stateIO sets t0 = t, MCWF_Trajectory.h:136-143, Master.h:93-99).
+// The continuing loop streams no row at count 0 (Trajectory.h:576-585).
+
+struct orc_traj_state {
+  double t, dtDid, dtTry;
+  double dpMaxOvershoot, dpTolMaxOvershoot, normMaxDev;
+  unsigned long long pcg[4];   // pcg64 state (high, low word), increment (high, low word)
+  long long ode[3];            // ODE logger: derivative calls, steps, failed steps (ODE.h:117)
+  long long mcwf[3];           // MCWF logger: steps, dpLimit overshoots, tolerance overshoots
+  unsigned long long ordinal;  // Philox step ordinal
+};
+
+static void continueLoop(const orc_run* run, double& t, const std::function<void(double)>& advance,
+                         const std::function<void(double)>& step, const std::function<void()>& stream) {
+  if (run->mode == 0) {
+    bool streamed = false;
+    for (long count = 0; t < run->T; ++count) {
+      if (count) { step(run->T - t); streamed = false; if (!(count % run->dc)) { stream(); streamed = true; } }
+    }
+    if (!streamed) stream();
+  } else if (run->mode == 1) {
+    for (long count = 0; t < run->T; ++count) if (count) { advance(std::min(run->Dt, run->T - t)); stream(); }
+  } else {
+    for (long count = 1; count <= run->NDt; ++count) { advance(run->Dt); stream(); }
+  }
+}
+
+// Continues one MCWF trajectory from an archived state; st is read and then
+// overwritten with the final state.  jt/jch hold *nj archived jumps on entry and
+// the whole jump list (at most jcap) on return.
+long orc_mcwf_resume(void* sp, const double* psi0, const orc_params* p, const orc_run* run, orc_traj_state* st,
+                     double* jt, int* jch, long* nj, long jcap, double* rows, long cap, double* psi_out) {
+  try {
+    System* s = (System*)sp;
+    CV psi((const cd*)psi0, (const cd*)psi0 + s->L.total);
+    double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s->highestFrequency());
+    MCWF tr(s, psi, dtInit, p->epsRel, p->epsAbs, p->dpLimit, p->overshootTol, (RngKind)p->rng, p->seed, p->stream,
+            (Controller)p->controller);
+    tr.psi = psi;
+    tr.t = tr.t0 = st->t; tr.ode.dtDid = st->dtDid; tr.ode.dtTry = st->dtTry;
+    tr.dpMaxOvershoot = st->dpMaxOvershoot; tr.dpTolMaxOvershoot = st->dpTolMaxOvershoot; tr.maxNormDev = st->normMaxDev;
+    tr.pcg.state = ((unsigned __int128)st->pcg[0] << 64) | st->pcg[1];
+    tr.pcg.inc = ((unsigned __int128)st->pcg[2] << 64) | st->pcg[3];
+    tr.stepOrdinal = st->ordinal;
+    tr.ode.st.nDerivs = st->ode[0]; tr.ode.st.nSteps = st->ode[1]; tr.ode.st.nFailed = st->ode[2];
+    tr.nMCWFSteps = st->mcwf[0]; tr.nOvershot = st->mcwf[1]; tr.nStepBacks = st->mcwf[2];
+    tr.jumps.clear();
+    for (long i = 0; i < *nj; ++i) tr.jumps.push_back({jt[i], jch[i]});
+    int nav = s->nAvr(); long nrow = 0;
+    auto stream = [&]() {
+      if (nrow < cap) {
+        double* r = rows + nrow * (2 + nav);
+        r[0] = tr.t; r[1] = tr.ode.dtDid;
+        averagesRaw(*s, tr.psi.data(), r + 2); processAverages(*s, r + 2);
+      }
+      ++nrow;
+    };
+    continueLoop(run, tr.t, [&](double dT) { tr.advance(dT); }, [&](double dT) { tr.step(dT); }, stream);
+    st->t = tr.t; st->dtDid = tr.ode.dtDid; st->dtTry = tr.ode.dtTry;
+    st->dpMaxOvershoot = tr.dpMaxOvershoot; st->dpTolMaxOvershoot = tr.dpTolMaxOvershoot; st->normMaxDev = tr.maxNormDev;
+    st->pcg[0] = (unsigned long long)(tr.pcg.state >> 64); st->pcg[1] = (unsigned long long)tr.pcg.state;
+    st->pcg[2] = (unsigned long long)(tr.pcg.inc >> 64); st->pcg[3] = (unsigned long long)tr.pcg.inc;
+    st->ordinal = tr.stepOrdinal;
+    st->ode[0] = tr.ode.st.nDerivs; st->ode[1] = tr.ode.st.nSteps; st->ode[2] = tr.ode.st.nFailed;
+    st->mcwf[0] = tr.nMCWFSteps; st->mcwf[1] = tr.nOvershot; st->mcwf[2] = tr.nStepBacks;
+    *nj = std::min((long)tr.jumps.size(), jcap);
+    for (long i = 0; i < *nj; ++i) { jt[i] = tr.jumps[i].t; jch[i] = tr.jumps[i].ch; }
+    if (psi_out) std::memcpy(psi_out, tr.psi.data(), sizeof(cd) * tr.psi.size());
+    return nrow;
+  } catch (std::exception& e) { g_last_error = e.what(); return -1; }
+}
+
+struct orc_master_state { double t, dtDid, dtTry; long long ode[3]; };
+
+long orc_master_resume(void* sp, const double* rho0, const orc_params* p, const orc_run* run, orc_master_state* st,
+                       double* rows, long cap, double* rho_out) {
+  try {
+    System* s = (System*)sp;
+    long d = s->L.total;
+    CV rho((const cd*)rho0, (const cd*)rho0 + d * d);
+    double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s->highestFrequency());
+    Master m(s, rho, dtInit, p->epsRel, p->epsAbs);
+    m.ode.controller = (Controller)p->controller;
+    m.t = m.t0 = st->t; m.ode.dtDid = st->dtDid; m.ode.dtTry = st->dtTry;
+    m.ode.st.nDerivs = st->ode[0]; m.ode.st.nSteps = st->ode[1]; m.ode.st.nFailed = st->ode[2];
+    int nav = s->nAvr(); long nrow = 0;
+    auto stream = [&]() {
+      if (nrow < cap) { double* r = rows + nrow * (2 + nav); r[0] = m.t; r[1] = m.ode.dtDid; averagesRawRho(*s, m.rho.data(), r + 2); processAverages(*s, r + 2); }
+      ++nrow;
+    };
+    continueLoop(run, m.t, [&](double dT) { m.advance(dT); }, [&](double dT) { m.step(dT); }, stream);
+    st->t = m.t; st->dtDid = m.ode.dtDid; st->dtTry = m.ode.dtTry;
+    st->ode[0] = m.ode.st.nDerivs; st->ode[1] = m.ode.st.nSteps; st->ode[2] = m.ode.st.nFailed;
     if (rho_out) std::memcpy(rho_out, m.rho.data(), sizeof(cd) * m.rho.size());
     return nrow;
   } catch (std::exception& e) { g_last_error = e.what(); return -1; }

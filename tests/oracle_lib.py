@@ -21,6 +21,17 @@ class Run(C.Structure):
     _fields_ = [("mode", C.c_int), ("dc", C.c_int), ("Dt", C.c_double), ("NDt", C.c_long), ("T", C.c_double)]
 
 
+class TrajState(C.Structure):
+    _fields_ = [("t", C.c_double), ("dtDid", C.c_double), ("dtTry", C.c_double),
+                ("dpMaxOvershoot", C.c_double), ("dpTolMaxOvershoot", C.c_double), ("normMaxDev", C.c_double),
+                ("pcg", C.c_ulonglong * 4), ("ode", C.c_longlong * 3), ("mcwf", C.c_longlong * 3),
+                ("ordinal", C.c_ulonglong)]
+
+
+class MasterState(C.Structure):
+    _fields_ = [("t", C.c_double), ("dtDid", C.c_double), ("dtTry", C.c_double), ("ode", C.c_longlong * 3)]
+
+
 _lib = None
 
 
@@ -56,6 +67,12 @@ def lib():
         l.orc_ensemble_run.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.c_long, dp, C.c_long, C.c_void_p]
         l.orc_master_run.restype = C.c_long
         l.orc_master_run.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), dp, C.c_long, C.c_void_p]
+        l.orc_mcwf_resume.restype = C.c_long
+        l.orc_mcwf_resume.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.POINTER(TrajState),
+                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_long, dp, C.c_long, C.c_void_p]
+        l.orc_master_resume.restype = C.c_long
+        l.orc_master_resume.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.POINTER(MasterState),
+                                        dp, C.c_long, C.c_void_p]
         _lib = l
     return _lib
 
@@ -173,3 +190,60 @@ def pcg64_uniform(seed, stream, n):
     raw = np.zeros(n, np.uint64)
     lib().orc_pcg64_uniform(seed, stream, n, out, raw.ctypes.data)
     return out, raw
+
+
+def _split128(x):
+    return (x >> 64) & ((1 << 64) - 1), x & ((1 << 64) - 1)
+
+
+def mcwf_resume(system, rec, p, r, cap=100000, jcap=100000):
+    """Continues a trajectory from a state-file record (cppqed_amd.statefile.TrajRecord with a
+    pcg64 random-engine record); returns the streamed rows and the final record."""
+    from cppqed_amd import statefile as SF
+    st = TrajState()
+    st.t, st.dtDid, st.dtTry = rec.t, rec.dtDid, rec.dtTry
+    st.dpMaxOvershoot, st.dpTolMaxOvershoot, st.normMaxDev = (rec.dp_max_overshoot, rec.dp_tolerance_max_overshoot,
+                                                              rec.norm_max_deviation)
+    state, inc = SF.parse_pcg64(rec.rng)
+    st.pcg[0], st.pcg[1] = _split128(state)
+    st.pcg[2], st.pcg[3] = _split128(inc)
+    st.ode[0], st.ode[1], st.ode[2] = rec.ode_derivs, rec.ode_steps, rec.ode_failed
+    st.mcwf[0], st.mcwf[1], st.mcwf[2] = rec.mcwf_steps, rec.overshot, rec.tolerance_overshot
+    jt = np.zeros(jcap)
+    jc = np.zeros(jcap, np.int32)
+    for k, (tj, ch) in enumerate(rec.jumps):
+        jt[k], jc[k] = tj, ch
+    nj = C.c_long(len(rec.jumps))
+    rows = np.zeros((cap, 2 + system.navr))
+    psi = np.zeros(system.dim, np.complex128)
+    n = lib().orc_mcwf_resume(system.h, _cplx(rec.psi.reshape(-1)), C.byref(p), C.byref(r), C.byref(st),
+                              jt.ctypes.data, jc.ctypes.data, C.byref(nj), jcap, rows, cap,
+                              psi.view(np.float64).ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    out = SF.TrajRecord(psi=psi.reshape(rec.psi.shape), t=st.t, dtDid=st.dtDid, dtTry=st.dtTry,
+                        ode_derivs=st.ode[0], ode_steps=st.ode[1], ode_failed=st.ode[2],
+                        rng=SF.format_pcg64((st.pcg[0] << 64) | st.pcg[1], (st.pcg[2] << 64) | st.pcg[3]),
+                        mcwf_steps=st.mcwf[0], overshot=st.mcwf[1], tolerance_overshot=st.mcwf[2],
+                        dp_max_overshoot=st.dpMaxOvershoot, dp_tolerance_max_overshoot=st.dpTolMaxOvershoot,
+                        norm_max_deviation=st.normMaxDev,
+                        jumps=[(float(jt[k]), int(jc[k])) for k in range(nj.value)])
+    return rows[:min(n, cap)], out
+
+
+def master_resume(system, rec, p, r, cap=100000):
+    """Continues a Master evolution from a state-file record (cppqed_amd.statefile.MasterRecord)."""
+    from cppqed_amd import statefile as SF
+    st = MasterState()
+    st.t, st.dtDid, st.dtTry = rec.t, rec.dtDid, rec.dtTry
+    st.ode[0], st.ode[1], st.ode[2] = rec.ode_derivs, rec.ode_steps, rec.ode_failed
+    d = system.dim
+    rows = np.zeros((cap, 2 + system.navr))
+    rho = np.zeros((d, d), np.complex128)
+    n = lib().orc_master_resume(system.h, _cplx(rec.matrix()), C.byref(p), C.byref(r), C.byref(st), rows, cap,
+                                rho.view(np.float64).ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    out = SF.MasterRecord(rho=rho.reshape(rec.rho.shape), t=st.t, dtDid=st.dtDid, dtTry=st.dtTry,
+                          ode_derivs=st.ode[0], ode_steps=st.ode[1], ode_failed=st.ode[2])
+    return rows[:min(n, cap)], out
