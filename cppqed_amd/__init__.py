@@ -40,7 +40,8 @@ class McwfParams(C.Structure):
 
 
 class MasterParams(C.Structure):
-    _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dtInit", C.c_double), ("ss_tol", C.c_double)]
+    _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dtInit", C.c_double), ("ss_tol", C.c_double),
+                ("autostop_eps", C.c_double), ("autostop_eps_abs", C.c_double), ("autostop_rep", C.c_int)]
 
 
 class BatchParams(C.Structure):
@@ -369,6 +370,12 @@ class Ensemble:
         check(lib().qt_mcwf_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def time(self):
+        """The ensemble's time (every trajectory stops at each stream point)."""
+        t = C.c_double()
+        check(lib().qt_mcwf_state(self.h, 0, None, C.byref(t)))
+        return t.value
+
     # ------------------------------------------------------------ checkpoint / resume
 
     def get_states(self, first=0, count=None):
@@ -592,7 +599,9 @@ def mcwf_plain_run(system, psi0, intervals, pars=None, traj=0, jcap=4096):
 class MasterSweep:
     """A batch of density operators on device, point k under systems[k] (master::make + Master)."""
 
-    def __init__(self, systems, rho0, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0):
+    def __init__(self, systems, rho0, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0, autostop=None):
+        """autostop = (epsilon, epsilonAbs, repetition): the reference's AutostopHandlerGeneric
+        criterion per point (CPPQEDutils/Trajectory.h:179-222); None: off."""
         self.systems = list(systems)
         self.n = len(self.systems)
         self.dim = self.systems[0].dim
@@ -601,7 +610,8 @@ class MasterSweep:
         if rho0.shape != (self.dim, self.dim):
             raise ValueError("rho0 has the wrong shape")
         arr = (C.c_void_p * self.n)(*[s_.h.value if hasattr(s_.h, "value") else s_.h for s_ in self.systems])
-        self.pars = MasterParams(epsRel, epsAbs, dtInit, ss_tol)
+        a_eps, a_abs, a_rep = autostop if autostop else (0.0, 0.0, 0)
+        self.pars = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep)
         h = C.c_void_p()
         check(lib().qt_master_create(arr, self.n, C.byref(self.pars), rho0.ctypes.data, C.byref(h)))
         self.h = h

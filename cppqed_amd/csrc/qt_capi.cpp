@@ -322,6 +322,8 @@ struct qt_master {
   DevBuf<int> jcol, done;
   DevBuf<double2> jval, x, k;
   DevBuf<long long> cnt;
+  DevBuf<double> asbuf;            // autostop: running average + queue ring per point
+  DevBuf<int> asmeta, pvar, psqrt; // autostop: started / queue size / head per point; processing rules
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double kernelMs = 0.;
   int launches = 0;
@@ -1053,6 +1055,10 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     if (!systems || !p || !rho0 || !out || n_points <= 0) throw Fail(QT_EINVAL, "bad arguments");
     if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
     const qt::System& s0 = *systems[0]->h;
+    if (p->autostop_rep < 0 || (p->autostop_rep > 0 && !(p->autostop_eps > 0.)))
+      throw Fail(QT_EINVAL, "autostop needs a positive epsilon and a non-negative repetition");
+    if (p->autostop_rep > 0 && (int)s0.O.size() > qt::kMaxAutostopAvr)
+      throw Fail(QT_ENOTSUP, "autostop watches at most 64 averages");
     const int d = (int)s0.dim;
     const int NB = (d + 15) / 16, DP = 16 * NB, KC = DP / 4, NN = DP * DP;
     if (!qt::master_supported(NB)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
@@ -1116,6 +1122,18 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     m->done.upload(std::vector<int>(n_points, 0));
     m->tss.upload(std::vector<double>(n_points, -1.));
     m->cnt.upload(std::vector<long long>((size_t)n_points * qt::kCounters, 0));
+    if (p->autostop_rep > 0) {
+      const int no = (int)s0.O.size();
+      m->asbuf.upload(std::vector<double>((size_t)n_points * (p->autostop_rep + 1) * std::max(1, no), 0.));
+      // the stream point at t = 0 of trajectory::run was the handler's first call (Trajectory.h:576-589)
+      std::vector<int> meta((size_t)n_points * 3, 0);
+      for (int k = 0; k < n_points; ++k) meta[3 * (size_t)k] = 1;
+      m->asmeta.upload(meta);
+      std::vector<int> pv(std::max(1, no), -1), ps(std::max(1, no), 0);
+      for (int q = 0; q < no; ++q) { pv[q] = s0.processVarOf[q]; ps[q] = s0.processSqrt[q]; }
+      m->pvar.upload(pv);
+      m->psqrt.upload(ps);
+    }
     hipCheck(hipEventCreate(&m->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&m->e1), "hipEventCreate");
     *out = m.release();
@@ -1143,6 +1161,8 @@ int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* 
     M.done = m->done.p; M.tss = m->tss.p; M.counters = m->cnt.p;
     M.epsAbs = m->p.epsRel; M.epsRel = m->p.epsAbs;  // ODE.h:205
     M.ss_tol = m->p.ss_tol;
+    M.as_rep = m->p.autostop_rep; M.as_eps = m->p.autostop_eps; M.as_epsAbs = m->p.autostop_eps_abs;
+    M.as_buf = m->asbuf.p; M.as_meta = m->asmeta.p; M.proc_var = m->pvar.p; M.proc_sqrt = m->psqrt.p;
     M.intervals = m->intervals.p; M.n_intervals = n_int;
     M.obs_out = m->obs.p; M.dtdid_out = m->dtd.p; M.dnorm_out = m->dn.p;
     hipCheck(hipEventRecord(m->e0, st), "hipEventRecord");
@@ -1236,6 +1256,11 @@ int qt_master_set_point(qt_master* m, int point, const qt_master_state* st, cons
                        hipMemcpyHostToDevice), "H2D");
     hipCheck(hipMemcpy(m->done.p + point, &st->done, sizeof(int), hipMemcpyHostToDevice), "H2D");
     hipCheck(hipMemcpy(m->tss.p + point, &st->t_ss, sizeof(double), hipMemcpyHostToDevice), "H2D");
+    if (m->p.autostop_rep > 0) {
+      // a continuing run streams no row at its start: the handler starts afresh (Trajectory.h:583)
+      const int zero[3] = {0, 0, 0};
+      hipCheck(hipMemcpy(m->asmeta.p + 3 * (size_t)point, zero, sizeof(zero), hipMemcpyHostToDevice), "H2D");
+    }
     if (point == 0) m->time = st->t;
   });
 }

@@ -122,6 +122,52 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
 }
 
 
+// AutostopHandlerGeneric::operator() (CPPQEDutils/Trajectory.h:187-214) for one point, on the
+// processed stream row (Mode.cc:198-201 processing of the raw averages `raw`).  The first call
+// only starts the running average; the queue then fills to as_rep rows; from there on a row
+// within as_eps (relative, max over the averages) of the running average stops the point,
+// otherwise it replaces the oldest row.  Returns 1 to stop.  One thread per point.
+__device__ int autostop(const MasterDev& M, int p, const double* raw) {
+  const int no = M.no, rep = M.as_rep;
+  double buf[kMaxAutostopAvr];
+  for (int q = 0; q < no; ++q) {
+    double v = raw[q];
+    const int vq = M.proc_var[q];
+    if (vq >= 0) {
+      v -= raw[vq] * raw[vq];
+      if (M.proc_sqrt[q]) v = sqrt(v);
+    }
+    buf[q] = fabs(v) < M.as_epsAbs ? M.as_epsAbs : v;
+  }
+  int* meta = M.as_meta + 3 * (size_t)p;
+  double* avg = M.as_buf + (size_t)p * (rep + 1) * no;
+  double* ring = avg + no;
+  if (!meta[0]) {
+    meta[0] = 1;
+    for (int q = 0; q < no; ++q) avg[q] = 0.;
+    return 0;
+  }
+  int qs = meta[1], qh = meta[2];
+  if (qs == rep) {
+    double m = -INFINITY;
+    for (int q = 0; q < no; ++q) {
+      const double r = fabs(avg[q] - buf[q]) / (fabs(avg[q]) + fabs(buf[q]));
+      m = (m < r) ? r : m;
+    }
+    if (m < M.as_eps) return 1;
+    double* front = ring + (size_t)qh * no;
+    for (int q = 0; q < no; ++q) { avg[q] = avg[q] + (buf[q] - front[q]) / (double)rep; front[q] = buf[q]; }
+    qh = (qh + 1) % rep;
+  } else {
+    double* slot = ring + (size_t)((qh + qs) % rep) * no;
+    for (int q = 0; q < no; ++q) { avg[q] = ((double)qs * avg[q] + buf[q]) / (double)(qs + 1); slot[q] = buf[q]; }
+    ++qs;
+  }
+  meta[1] = qs; meta[2] = qh;
+  return 0;
+}
+
+
 template <int NB>
 __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB, NN = DP * DP;
@@ -195,6 +241,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   if (tid < kCounters) cnt[tid] = M.counters[(size_t)p * kCounters + tid];
   __syncthreads();
   int done = M.done[p];
+  __shared__ int asStop;
 
   for (int kI = 0; kI < M.n_intervals; ++kI) {
     const double endTime = t + M.intervals[kI];
@@ -329,6 +376,15 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
     if (!done && M.ss_tol > 0. && dn < M.ss_tol) {
       done = 1;
       if (tid == 0) { M.done[p] = 1; M.tss[p] = t; }
+    }
+    if (!done && M.as_rep > 0) {
+      if (tid == 0) asStop = autostop(M, p, M.obs_out + ((size_t)p * M.n_intervals + kI) * M.no);
+      __syncthreads();
+      if (asStop) {
+        done = 1;
+        if (tid == 0) { M.done[p] = 1; M.tss[p] = t; }
+      }
+      __syncthreads();
     }
   }
   __syncthreads();

@@ -5,6 +5,8 @@
 
 namespace qt {
 
+constexpr int kMaxAutostopAvr = 64;  // averages a point's autostop can watch
+
 struct MasterDev {
   int n_points;
   int dim;               // true dimension d
@@ -29,6 +31,13 @@ struct MasterDev {
   double ss_tol;         // steady-state threshold on ||drho/dt||_F (<= 0: never stop)
   const double* intervals;
   int n_intervals;
+  // autostop (AutostopHandlerGeneric, CPPQEDutils/Trajectory.h:179-222) on the processed averages
+  int as_rep;            // queue length; 0: off
+  double as_eps, as_epsAbs;
+  double* as_buf;        // [point][(as_rep + 1) * no]: running average, then the queue as a ring
+  int* as_meta;          // [point][3]: started, queue size, queue head
+  const int* proc_var;   // [no]: index of the mean whose square is subtracted, or -1
+  const int* proc_sqrt;  // [no]: square root after the subtraction
   double* obs_out;       // [point][n_intervals][no]
   double* dtdid_out;     // [point][n_intervals]
   double* dnorm_out;     // [point][n_intervals]: ||drho/dt||_F at the stream point

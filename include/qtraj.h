@@ -230,6 +230,13 @@ typedef struct {
   double epsRel, epsAbs;   /* user values, as for MCWF (tolerance swap of ODE.h:205 applied inside) */
   double dtInit;           /* <= 0: 1/(10*highest_frequency) of each point's system */
   double ss_tol;           /* stop a point once ||drho/dt||_F < ss_tol at a stream point (<= 0: never) */
+  /* the reference's steady-state criterion, AutostopHandlerGeneric (CPPQEDutils/Trajectory.h:179-222,
+   * --autoStopEpsilon / --autoStopEpsilonAbs / --autoStopRepetition): a point stops at the stream
+   * point whose processed averages lie within autostop_eps (relative) of the moving average of
+   * the previous autostop_rep rows; averages below autostop_eps_abs count as autostop_eps_abs.
+   * autostop_rep = 0: off.  Either criterion stops a point (qt_master_status: done, t_ss). */
+  double autostop_eps, autostop_eps_abs;
+  int autostop_rep;
 } qt_master_params;
 
 /* A batch of density operators, point k evolving under systems[k] (same dimension and jump

@@ -29,6 +29,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -1248,6 +1249,37 @@ struct Master {
 
 }  // namespace orc
 
+// AutostopHandlerGeneric (CPPQEDutils/Trajectory.h:179-222): called with every streamed array.
+// The first call only sizes the running average (the array itself is not queued); then the
+// queue fills to `rep` arrays with their running mean; from there on, a streamed array whose
+// max_i |avg_i - v_i| / (|avg_i| + |v_i|) is below eps stops the run, otherwise it replaces the
+// oldest array of the moving average.  Entries below epsAbs in modulus are replaced by epsAbs.
+struct Autostop {
+  double eps, epsAbs; unsigned rep;
+  bool started = false;
+  std::vector<double> avg;
+  std::deque<std::vector<double>> q;
+  Autostop(double e, double ea, unsigned r) : eps(e), epsAbs(ea), rep(r) {}
+  bool operator()(const double* v, int n) {  // true: StoppingCriterionReachedException
+    std::vector<double> buf(v, v + n);
+    for (auto& x : buf) if (std::fabs(x) < epsAbs) x = epsAbs;
+    if (!rep) return false;
+    if (!started) { started = true; avg.assign(n, 0.); return false; }
+    if (q.size() == rep) {
+      double m = -INFINITY;
+      for (int i = 0; i < n; ++i) m = std::max(m, std::fabs(avg[i] - buf[i]) / (std::fabs(avg[i]) + std::fabs(buf[i])));
+      if (m < eps) return true;
+      for (int i = 0; i < n; ++i) avg[i] = avg[i] + (buf[i] - q.front()[i]) / double(rep);
+      q.pop_front();
+    } else {
+      const double k = (double)q.size();
+      for (int i = 0; i < n; ++i) avg[i] = (k * avg[i] + buf[i]) / (k + 1.);
+    }
+    q.push_back(buf);
+    return false;
+  }
+};
+
 // ===========================================================================
 // C ABI for the tests (ctypes)
 // ===========================================================================
@@ -1387,8 +1419,8 @@ long orc_ensemble_run(void* sp, const double* psi0, const orc_params* p, const o
   } catch (std::exception& e) { g_last_error = e.what(); return -1; }
 }
 
-long orc_master_run(void* sp, const double* rho0, const orc_params* p, const orc_run* run,
-                    double* rows, long cap, double* rho_out) {
+static long masterRun(void* sp, const double* rho0, const orc_params* p, const orc_run* run,
+                      double* rows, long cap, double* rho_out, Autostop* as) {
   try {
     System* s = (System*)sp;
     long d = s->L.total;
@@ -1397,26 +1429,49 @@ long orc_master_run(void* sp, const double* rho0, const orc_params* p, const orc
     Master m(s, rho, dtInit, p->epsRel, p->epsAbs);
     m.ode.controller = (Controller)p->controller;
     int nav = s->nAvr(); long nrow = 0;
-    auto stream = [&]() {
-      if (nrow < cap) { double* r = rows + nrow * (2 + nav); r[0] = m.t; r[1] = m.ode.dtDid; averagesRawRho(*s, m.rho.data(), r + 2); processAverages(*s, r + 2); }
+    std::vector<double> av(nav);
+    struct Stop {};
+    auto stream = [&](bool handler = true) {
+      averagesRawRho(*s, m.rho.data(), av.data()); processAverages(*s, av.data());
+      if (nrow < cap) { double* r = rows + nrow * (2 + nav); r[0] = m.t; r[1] = m.ode.dtDid; std::copy(av.begin(), av.end(), r + 2); }
       ++nrow;
+      if (handler && as && (*as)(av.data(), nav)) throw Stop{};  // Trajectory.h:583-589
     };
     bool arrayStreamed = false;
-    if (run->mode == 0) {
-      for (long count = 0; m.t < run->T; ++count) {
-        if (count) { m.step(run->T - m.t); arrayStreamed = false; }
-        if (!count || !(count % run->dc)) { stream(); arrayStreamed = true; }
+    try {
+      if (run->mode == 0) {
+        for (long count = 0; m.t < run->T; ++count) {
+          if (count) { m.step(run->T - m.t); arrayStreamed = false; }
+          if (!count || !(count % run->dc)) { arrayStreamed = true; stream(); }
+        }
+      } else if (run->mode == 1) {
+        for (long count = 0; m.t < run->T; ++count) { if (count) m.advance(std::min(run->Dt, run->T - m.t)); arrayStreamed = true; stream(); }
+      } else {
+        for (long count = 0; count <= run->NDt; ++count) { if (count) m.advance(run->Dt); arrayStreamed = true; stream(); }
       }
-      if (!arrayStreamed) stream();
-    } else if (run->mode == 1) {
-      for (long count = 0; m.t < run->T; ++count) { if (count) m.advance(std::min(run->Dt, run->T - m.t)); stream(); }
-    } else {
-      for (long count = 0; count <= run->NDt; ++count) { if (count) m.advance(run->Dt); stream(); }
-    }
+    } catch (Stop&) {}
+    if (!arrayStreamed) stream(false);  // the closing row goes to no handler (Trajectory.h:594)
     if (rho_out) std::memcpy(rho_out, m.rho.data(), sizeof(cd) * m.rho.size());
     return nrow;
   } catch (std::exception& e) { g_last_error = e.what(); return -1; }
 }
+
+long orc_master_run(void* sp, const double* rho0, const orc_params* p, const orc_run* run,
+                    double* rows, long cap, double* rho_out) {
+  return masterRun(sp, rho0, p, run, rows, cap, rho_out, nullptr);
+}
+
+// trajectory::run with AutostopHandlerGeneric (Trajectory.h:337): the rows end with the row at
+// which the criterion was met
+long orc_master_run_autostop(void* sp, const double* rho0, const orc_params* p, const orc_run* run,
+                             double eps, double epsAbs, unsigned rep, double* rows, long cap, double* rho_out) {
+  Autostop as(eps, epsAbs, rep);
+  return masterRun(sp, rho0, p, run, rows, cap, rho_out, &as);
+}
+
+void* orc_autostop_create(double eps, double epsAbs, unsigned rep) { return new Autostop(eps, epsAbs, rep); }
+void orc_autostop_destroy(void* h) { delete (Autostop*)h; }
+int orc_autostop_feed(void* h, const double* v, int n) { return (*(Autostop*)h)(v, n) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
 // Continuing from a saved state (trajectory::run with a state file, Trajectory.h:484-500:

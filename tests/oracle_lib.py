@@ -73,6 +73,13 @@ def lib():
         l.orc_master_resume.restype = C.c_long
         l.orc_master_resume.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.POINTER(MasterState),
                                         dp, C.c_long, C.c_void_p]
+        l.orc_master_run_autostop.restype = C.c_long
+        l.orc_master_run_autostop.argtypes = [C.c_void_p, dp, C.POINTER(Params), C.POINTER(Run), C.c_double,
+                                              C.c_double, C.c_uint, dp, C.c_long, C.c_void_p]
+        l.orc_autostop_create.restype = C.c_void_p
+        l.orc_autostop_create.argtypes = [C.c_double, C.c_double, C.c_uint]
+        l.orc_autostop_destroy.argtypes = [C.c_void_p]
+        l.orc_autostop_feed.argtypes = [C.c_void_p, dp, C.c_int]
         _lib = l
     return _lib
 
@@ -247,3 +254,29 @@ def master_resume(system, rec, p, r, cap=100000):
     out = SF.MasterRecord(rho=rho.reshape(rec.rho.shape), t=st.t, dtDid=st.dtDid, dtTry=st.dtTry,
                           ode_derivs=st.ode[0], ode_steps=st.ode[1], ode_failed=st.ode[2])
     return rows[:min(n, cap)], out
+
+
+class Autostop:
+    """The oracle's AutostopHandlerGeneric; call with a streamed array, True = stop."""
+
+    def __init__(self, eps, eps_abs, rep):
+        self.h = lib().orc_autostop_create(eps, eps_abs, rep)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_autostop_destroy(self.h)
+
+    def __call__(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        return bool(lib().orc_autostop_feed(self.h, v, len(v)))
+
+
+def master_run_autostop(system, rho0, p, r, eps, eps_abs, rep, cap=100000):
+    rows = np.zeros((cap, 2 + system.navr))
+    d = system.dim
+    rho = np.zeros((d, d), np.complex128)
+    n = lib().orc_master_run_autostop(system.h, _cplx(rho0), C.byref(p), C.byref(r), eps, eps_abs, rep, rows, cap,
+                                      rho.view(np.float64).ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return rows[:min(n, cap)], rho

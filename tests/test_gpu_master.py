@@ -42,3 +42,35 @@ def test_master_sweep_matches_oracle(cutoff, etas, T):
         np.testing.assert_allclose(rows[k][:, 2:], rows_o[1:, 2:], rtol=1e-8, atol=1e-10)
         assert np.allclose(rho_g, rho_g.conj().T, atol=0)
         assert abs(np.trace(rho_g) - 1) < 1e-13
+
+
+def test_master_sweep_autostop_matches_oracle():
+    """The reference's autostop criterion (AutostopHandlerGeneric, Trajectory.h:179-222) per sweep
+    point: every point stops at the oracle's stop row, on the oracle's density operator, and the
+    point's rows up to there are the oracle's."""
+    print("[gpu] master sweep autostop", flush=True)
+    kappas = [0.5, 1.0, 2.0, 3.0]  # the relaxation rate sets the stop time (oracle: 20.7, 10.8, 5.7, 4.1)
+    specs = [f"free mode cutoff=20 delta=-1 kappa={k} eta=2" for k in kappas]
+    rho0 = np.zeros((20, 20), complex)
+    rho0[0, 0] = 1
+    auto = (1e-5, 1e-12, 5)
+    m = Q.MasterSweep([Q.System(sp) for sp in specs], rho0, epsRel=1e-8, epsAbs=1e-12, autostop=auto)
+    iv = np.full(300, 0.1)
+    rows, _ = m.advance(iv)
+    done, tss, _, _ = m.status()
+    assert done.all()
+    for k, sp in enumerate(specs):
+        ro, rho_o = O.master_run_autostop(O.System(sp), rho0, O.params(epsRel=1e-8, epsAbs=1e-12),
+                                          O.run_spec("Dt", Dt=0.1, T=30.0), *auto)
+        assert abs(tss[k] - ro[-1, 0]) < 1e-9, (k, tss[k], ro[-1, 0])
+        n = len(ro) - 1  # the oracle's rows include t = 0
+        # the device's FP64 MFMA sums in another order, so the step-size controller's accept/grow
+        # decisions may tip differently near its thresholds: the averages agree to the ODE
+        # tolerance, the last step size (dtDid, column 1) need not
+        np.testing.assert_allclose(rows[k, :n, 0], ro[1:, 0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(rows[k, :n, 2:], ro[1:, 2:], rtol=1e-6, atol=1e-9)
+        assert (rows[k, :n, 1] > 0).all()
+        # both integrate to the absolute ODE tolerance 1e-8 (epsRel after the swap of ODE.h:205)
+        # along their own step sequences
+        assert np.abs(m.rho(k) - rho_o).max() < 1e-7
+    assert len(set(np.round(tss, 6))) > 1  # the points stop at different times
