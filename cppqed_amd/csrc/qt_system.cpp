@@ -383,9 +383,14 @@ struct Builder {
     return f;
   }
 
-  // ParticleAlongCavity with an unpumped particle (ParticleCavity.cc:26-32, 90-99): the dispersive
-  // term u0 n (x) (+-) cos(2 kcav x) / (2i); cos(K x) = (e^{iKx} + e^{-iKx}) / 2 shifts the momentum
-  // index by +-K (Particle.cc:60-80).  The constant u0 n / 2 of cos^2 belongs to the mode detuning.
+  // ParticleAlongCavity (ParticleCavity.cc:26-53, 87-110).  The dispersive term u0 n (x) (+-)
+  // cos(2 kcav x) / (2i) for a real cavity mode function; cos(K x) = (e^{iKx} + e^{-iKx}) / 2 shifts
+  // the momentum index by +-K (Particle.cc:60-80); the constant u0 n / 2 of cos^2 belongs to the
+  // mode detuning.  The interference term with etaeff = sign(u0) sqrt(u0 v): for a particle
+  // without its own pump, or with vclass= given (v = vclass), etaeff (a (x) mf(kcav x) + h.c.) / i;
+  // for a pumped particle (v = its vClass) and kcav = kPart, etaeff (a^dagger (x)
+  // mfComposition(mf, mfPart) + h.c.) / i.  A pumped particle of another wavenumber takes a
+  // product of two Tridiagonals (ParticleCavity.cc:113-140) that is not built.
   void pcav(int m, int p, const KV& kv) {
     const FreeInfo& fm = freeAt(m); const FreeInfo& fp = particleAt(p);
     if (fm.kind != "mode") throw std::invalid_argument("pcav needs a mode as its first free");
@@ -395,7 +400,12 @@ struct Builder {
     std::string mfs = mf == kv.end() ? "cos" : mf->second;
     if (mfs != "cos" && mfs != "sin" && mfs != "plus" && mfs != "minus")
       throw std::invalid_argument("unknown pcav mode function");
-    if (fp.pumped) throw std::invalid_argument("pcav with a pumped particle is not built");
+    const bool pumpedCtor = fp.pumped && !kv.count("vclass");
+    const double v = pumpedCtor ? fp.vclass : num(kv, "vclass", 0.);
+    if (u0 * v < 0.) throw std::invalid_argument("Unot VClass sign discrepancy");
+    const double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * v);
+    if (pumpedCtor && kcav != std::abs(fp.kpart))
+      throw std::invalid_argument("pcav with a pumped particle of another wavenumber is not built");
     const double sgn = mfs == "sin" ? -1. : 1.;
     const int K = 2 * kcav;
     LocalOp t; t.dims = {fm.dim, fp.dim};
@@ -410,8 +420,14 @@ struct Builder {
       }
       t = scale(kron(n, c), cd(u0 * sgn, 0.) / cd(0., 2.));
     }
+    if (etaeff != 0.) {
+      LocalOp x = pumpedCtor ? kron(adjoint(destroy(fm.dim)), mfComposition(fp.dim, mfs, fp.mf, kcav))
+                             : kron(destroy(fm.dim), modeFunction(fp.dim, mfs, kcav));
+      t = plus(t, scale(plus(x, adjoint(x)), cd(etaeff, 0.) / cd(0., 1.)));
+    }
     s.hTerms.push_back({{m, p}, t});
-    iaHF = std::max(iaHF, std::fabs(u0) * fm.dim);  // RF{"Unot", uNot, mode dim}
+    // RF{"Unot", uNot, mode dim}, RF{"etaeff", etaeff, sqrt(mode dim)}
+    iaHF = std::max(iaHF, std::max(std::fabs(u0) * fm.dim, std::fabs(etaeff) * std::sqrt((double)fm.dim)));
   }
 
   // mfComposition(particle, (mf0, K), (mf1, K)), Particle.cc:290-345, for positive wavenumbers

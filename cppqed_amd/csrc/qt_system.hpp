@@ -105,7 +105,8 @@ struct System {
 //   ia jc i j g=.. [gI=..]                                 (JaynesCummings)
 //   ia bs i j g=..                                         (CoupledModes, RWA beam splitter)
 //   ia zz i j J=..                                         (qubit sigma_z sigma_z coupling)
-//   ia pcav m p u0=.. kcav=.. mf=cos|sin                   (ParticleAlongCavity, dispersive term)
+//   ia pcav m p u0=.. kcav=.. mf=.. [vclass=..]            (ParticleAlongCavity: dispersive and
+//                                                          interference terms)
 //   ia pcav_orth m p u0=..                                 (ParticleOrthogonalToCavity, pumped particle)
 //   ia ptm m0 m1 p u0=.. u1=.. phi=.. k0=.. k1=.. mf0=.. mf1=..  (ParticleTwoModes, equal wavenumbers)
 //   system binary|composite

@@ -646,9 +646,24 @@ Tridiag cosNKX(int dim, int K) {
   return scaled(r, 0.5);
 }
 
-// ParticleAlongCavity with an unpumped particle (ParticleCavity.cc:26-32, 47-53, 90-99):
-// dispersive u0 nop(mode) (+-) cosNKX(particle, 2 kcav) / (2i); etaeff = 0.
-// Frequencies RF{"Unot", u0, mode dim}, RF{"etaeff", 0, sqrt(mode dim)} (ParticleCavity.cc:62-66)
+Tridiag mfComposition(const Free& fp, const std::string& a, const std::string& b, int nK);
+
+// mfNKX(particle, (mf, nK)) for a given mode function (Particle.cc:278-289), furnished when the
+// particle is Exact (expINKX(Ptr, nK), Particle.cc:263-269)
+Tridiag mfNKXgiven(const Free& fp, const std::string& mf, int nK) {
+  Free g = fp; g.mf = mf; g.kPart = nK;
+  return mfNKXof(g);
+}
+
+// ParticleAlongCavity (ParticleCavity.cc:26-53, 87-110):
+//   dispersive: u0 nop(mode) (+-) cosNKX(particle, 2 kcav) / (2i) for a real cavity mode function
+//     (nop = Tridiagonal(mainDiagonal(1,0,dim)) carries no frequencies, so neither does the term);
+//   particle without its own pump, or given vclass (1p1mconf 2, 4): etaeff = etaEff(u0, vclass),
+//     interfericOneModeFN = etaeff tridiagPlusHC_overI(aop(mode) mfNKX(particle, (mf, kcav)));
+//   pumped particle (1p1mconf 3): etaeff = etaEff(u0, particle's vClass); with kcav = |kPart|
+//     (isSpecialH_) interfericTwoModeFN = etaeff tridiagPlusHC_overI(aop(mode)^dagger
+//     mfComposition(particle, (mf, kcav), (mfPart, kPart))); other wavenumbers are not restated.
+// Frequencies RF{"Unot", u0, mode dim}, RF{"etaeff", etaeff, sqrt(mode dim)} (ParticleCavity.cc:62-66)
 Interaction makePcav(const System& s, int m, int p, const std::map<std::string, std::string>& kv) {
   Interaction ia; ia.kind = "pcav"; ia.axes = {m, p};
   const Free& fm = s.frees[m]; const Free& fp = s.frees[p];
@@ -657,16 +672,29 @@ Interaction makePcav(const System& s, int m, int p, const std::map<std::string, 
   int kcav = (int)num(kv, "kcav", 1);
   auto it = kv.find("mf");
   const std::string mfs = it == kv.end() ? "cos" : it->second;
+  const bool givenV = kv.count("vclass") > 0;
+  const bool pumpedCtor = fp.vClass != 0. && !givenV;
+  const double v = pumpedCtor ? fp.vClass : num(kv, "vclass", 0.);
+  if (u0 * v < 0.) throw std::domain_error("Unot VClass sign discrepancy");
+  const double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * v);
+  if (pumpedCtor && kcav != std::abs(fp.kPart)) throw std::invalid_argument("oracle: pcav with kcav != kPart");
   double sgn = mfs == "sin" ? -1. : 1.;
   // dispersive(): only for a real mode function (ParticleCavity.cc:26-32)
   if (u0 != 0. && 2 * kcav < fp.dim && (mfs == "cos" || mfs == "sin")) {
-    CV nd(fm.dim); for (int n = 0; n < fm.dim; ++n) nd[n] = (double)n;   // nop = Tridiagonal(mainDiagonal(1,0,dim))
+    CV nd(fm.dim); for (int n = 0; n < fm.dim; ++n) nd[n] = (double)n;
     Tridiag t = product(scaled(unary(nd, 0, CV(), CV()), u0 * sgn), cosNKX(fp.dim, 2 * kcav));
     ia.hOverIs.push_back(scaled(t, 1. / cd(0., 2.)));
     ia.hasHa = true; ia.timeDep = true;
   }
+  if (etaeff != 0.) {
+    Tridiag a = aop(fm.dim);
+    if (fm.hasEx) furnishWithFreqs(a, fm.exd);
+    Tridiag t = pumpedCtor ? product(hc(a), mfComposition(fp, mfs, fp.mf, kcav)) : product(a, mfNKXgiven(fp, mfs, kcav));
+    ia.hOverIs.push_back(scaled(sum(t, hc(t)), cd(etaeff, 0.) / cd(0., 1.)));
+    ia.hasHa = true; ia.timeDep = true;
+  }
   ia.realFreqs.push_back({u0, (double)fm.dim});
-  ia.realFreqs.push_back({0., std::sqrt((double)fm.dim)});
+  ia.realFreqs.push_back({etaeff, std::sqrt((double)fm.dim)});
   return ia;
 }
 

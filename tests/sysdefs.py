@@ -75,6 +75,16 @@ P1M1 = ("free mode cutoff=3 delta=-10 kappa=10 eta=10 picture=IP\n"
         "ia pcav_orth 0 1 u0=-1")
 P1M1_MASTER = P1M1.replace("eta=10 picture=IP", "eta=10 picture=UIP")
 
+# 1particle1mode --1p1mconf 2, 3, 4 (CPPQEDscripts/1particle1mode.cc:36-68): ParticleAlongCavity;
+# the cavity detuning is shifted by -Unot/2 = +0.5
+_P1M1_MODE = "free mode cutoff=3 delta=-9.5 kappa=10 eta=10 picture=IP\n"
+P1M1_CONF = {
+    2: _P1M1_MODE + "free particle fin=3 omrec=1 picture=IP\nia pcav 0 1 u0=-1 kcav=1 mf=sin vclass=-10",
+    3: _P1M1_MODE + "free particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin picture=IP\nia pcav 0 1 u0=-1 kcav=1 mf=sin",
+    4: _P1M1_MODE + "free particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin picture=IP\n"
+                    "ia pcav 0 1 u0=-1 kcav=1 mf=sin vclass=0",
+}
+
 
 def particle_wavepacket(fin, x0=-0.5, k0=0.0, sig=0.1):
     """particle::wavePacket in x (Particle.cc:347-369), exp(-(x - pi x0)^2 / (4 sig^2) + i k0 x) on

@@ -6,7 +6,7 @@ import pytest
 
 import cppqed_amd as Q
 import oracle_lib as O
-from sysdefs import P1M1, RING, coherent, ground, p1m1_initial, ring_initial
+from sysdefs import P1M1, P1M1_CONF, RING, coherent, ground, p1m1_initial, ring_initial
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +49,9 @@ def test_apply_heff_interaction_picture(spec, prec, tol):
     # the 1particle1mode system whose golden runs pin the oracle (tests/test_oracle.py)
     ("p1m1", P1M1, p1m1_initial(), 0.5, 24, 10.0),
     ("ring", RING, ring_initial(), 0.5, 16, 10.0),
+    # ParticleAlongCavity with the interference terms (1particle1mode --1p1mconf 2, 3)
+    ("p1m1_conf2", P1M1_CONF[2], p1m1_initial(), 0.5, 16, 10.0),
+    ("p1m1_conf3", P1M1_CONF[3], p1m1_initial(), 0.5, 16, 10.0),
 ])
 def test_ip_trajectories_match_oracle(name, spec, psi0, T, ntraj, ovt, layout):
     """With overshootTol = 1 dpLimit step-backs are frequent, and the redone step runs from the

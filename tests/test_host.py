@@ -22,6 +22,10 @@ SPECS = {
     "pumped_mode": "free mode cutoff=7 delta=-2 kappa=0 eta=1.5",
     "pcav_small": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1\nfree particle fin=3 omrec=0.3\nia pcav 0 1 u0=-1 kcav=1 mf=cos",
     "pcav_sin": "free mode cutoff=3 delta=1 kappa=0.5 eta=0.5\nfree particle fin=4 omrec=1\nia pcav 0 1 u0=2 kcav=1 mf=sin",
+    "pcav_sch_conf3": "free mode cutoff=3 delta=-9.5 kappa=10 eta=10\nfree particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin\n"
+                      "ia pcav 0 1 u0=-1 kcav=1 mf=sin",
+    "pcav_sch_conf2": "free mode cutoff=3 delta=-9.5 kappa=10 eta=10\nfree particle fin=3 omrec=1\n"
+                      "ia pcav 0 1 u0=-1 kcav=1 mf=cos vclass=-10",
     "pumped_particle_sch": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin\n"
                            "ia pcav_orth 0 1 u0=-1",
     "bs_chain": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0\n"
@@ -79,6 +83,14 @@ IP_SPECS = {
                      "free particle fin=4 omrec=0.5 vclass=3 kpart=1 mf=cos picture=IP\nia pcav_orth 0 1 u0=2",
     "pcav_disp_ip": "free mode cutoff=4 delta=-0.5 kappa=1 eta=1 picture=IP\nfree particle fin=3 omrec=0.3 picture=IP\n"
                     "ia pcav 0 1 u0=-1 kcav=1 mf=cos",
+    "p1m1_conf2": __import__("sysdefs").P1M1_CONF[2],
+    "p1m1_conf3": __import__("sysdefs").P1M1_CONF[3],
+    "p1m1_conf4": __import__("sysdefs").P1M1_CONF[4],
+    "pcav_pumped_cos": "free mode cutoff=4 delta=-2 kappa=1 eta=2 picture=IP\n"
+                       "free particle fin=4 omrec=0.5 vclass=3 kpart=2 mf=cos picture=IP\n"
+                       "ia pcav 0 1 u0=2 kcav=2 mf=sin",
+    "pcav_plus_vclass": "free mode cutoff=4 delta=-2 kappa=1 eta=2 picture=IP\n"
+                        "free particle fin=4 omrec=0.5 picture=IP\nia pcav 0 1 u0=2 kcav=1 mf=plus vclass=1.5",
     "ring": __import__("sysdefs").RING,
     "ring_sin_cos": __import__("sysdefs").RING.replace("mf0=plus mf1=minus", "mf0=sin mf1=cos"),
 }

@@ -7,8 +7,8 @@ import pytest
 from scipy.linalg import expm
 
 import oracle_lib as O
-from sysdefs import (JC60, P1M1, P1M1_MASTER, PLM20, PTLA, RING, RING_MASTER, coherent, ground, p1m1_initial,
-                     ring_initial)
+from sysdefs import (JC60, P1M1, P1M1_CONF, P1M1_MASTER, PLM20, PTLA, RING, RING_MASTER, coherent, ground,
+                     p1m1_initial, ring_initial)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -157,6 +157,31 @@ def test_1particle1mode_golden(mode, kind):
         rows, _ = O.ensemble_run(s, psi0, p, r, 3)
     else:
         rows, _ = O.master_run(s, np.outer(psi0, psi0.conj()), p, P1M1_MASTER_MODES[mode])
+    assert rows.shape == g.shape, (rows.shape, g.shape)
+    tol = np.full(g.shape[1], 1e-8)
+    tol[-1] = 1e-6
+    bad = np.abs(rows - g) > 1e-5 * np.abs(g) + tol
+    assert not bad.any(), (np.argwhere(bad)[:5], rows[bad][:5], g[bad][:5])
+
+
+@pytest.mark.parametrize("conf", [2, 3, 4])
+@pytest.mark.parametrize("kind", ["single", "ensemble", "master"])
+def test_1particle1mode_conf_golden(conf, kind):
+    """Testing/expected/run_1particle1mode_ndt_conf{2,3,4}_{single,ensemble,master}
+    (1particle1mode --1p1mconf 2|3|4 --dc 1 --T 0.1): ParticleAlongCavity with the dispersive
+    term and the interference term of an unpumped particle (etaeff from the given vClass, conf 2),
+    of a pumped particle through mfComposition (conf 3), and none (vClass 0, conf 4)."""
+    spec = P1M1_CONF[conf]
+    s = O.System(spec.replace("eta=10 picture=IP", "eta=10 picture=UIP") if kind == "master" else spec)
+    p = O.params(epsRel=1e-6, epsAbs=1e-5, rng="pcg64", seed=1001, stream=1)
+    g = load_golden(f"run_1particle1mode_ndt_conf{conf}_{kind}")
+    psi0 = p1m1_initial()
+    if kind == "single":
+        rows = O.mcwf_run(s, psi0, p, P1M1_MODES["dc"])["rows"]
+    elif kind == "ensemble":
+        rows, _ = O.ensemble_run(s, psi0, p, O.run_spec("Dt", Dt=0.1, T=0.1), 3)
+    else:
+        rows, _ = O.master_run(s, np.outer(psi0, psi0.conj()), p, P1M1_MASTER_MODES["dc"])
     assert rows.shape == g.shape, (rows.shape, g.shape)
     tol = np.full(g.shape[1], 1e-8)
     tol[-1] = 1e-6

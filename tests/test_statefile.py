@@ -14,7 +14,7 @@ import pytest
 
 import oracle_lib as O
 from cppqed_amd import statefile as SF
-from sysdefs import P1M1, P1M1_MASTER, PTLA, RING, RING_MASTER
+from sysdefs import P1M1, P1M1_CONF, P1M1_MASTER, PTLA, RING, RING_MASTER
 from test_oracle import GOLDEN_MODES, P1M1_MASTER_MODES, P1M1_MODES, RING_MASTER_MODES, load_golden
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -65,6 +65,12 @@ def test_array_only_and_rejections():
 
 def _setup(system, kind):
     p = O.params(epsRel=1e-6, epsAbs=1e-5, rng="pcg64", seed=1001, stream=1)
+    if system.startswith("1particle1mode_ndt_conf"):
+        spec = P1M1_CONF[int(system[-1])]
+        if kind == "master":
+            return (O.System(spec.replace("eta=10 picture=IP", "eta=10 picture=UIP")), p,
+                    {"dc": P1M1_MASTER_MODES["dc"]})
+        return O.System(spec), p, {"dc": P1M1_MODES["dc"]}
     if system == "PTLA_CPPQED":
         return O.System(PTLA), p, GOLDEN_MODES
     if system == "1particle1mode":
@@ -103,7 +109,14 @@ def _assert_traj_equal(a, b, tol):
         assert abs(getattr(a, f) - getattr(b, f)) <= 1e-6 * max(1e-3, abs(getattr(b, f))), f
 
 
-RUNS = [(s, m) for s in ("PTLA_CPPQED", "1particle1mode", "Ring") for m in ("dc", "dt", "ndt")]
+RUNS = ([(s, m) for s in ("PTLA_CPPQED", "1particle1mode", "Ring") for m in ("dc", "dt", "ndt")]
+        + [(f"1particle1mode_ndt_conf{c}", "dc") for c in (2, 3, 4)])
+
+
+def _stem(system, mode, kind):
+    if system.startswith("1particle1mode_ndt_conf"):
+        return f"run_{system}_{kind}"
+    return f"run_{system}_{mode}_{kind}"
 
 
 @pytest.mark.parametrize("system,mode", RUNS)
@@ -112,8 +125,8 @@ def test_single_continues_from_reference_state(system, mode):
     saved instant reaches the reference's final archive -- state, time, dtDid, dtTry, both
     loggers, jump list and the pcg64 state -- and streams the reference's remaining rows."""
     s, p, modes = _setup(system, "single")
-    frames = SF.read(os.path.join(HERE, "golden", "state", f"run_{system}_{mode}_single.state"))
-    g = load_golden(f"run_{system}_{mode}_single")
+    frames = SF.read(os.path.join(HERE, "golden", "state", _stem(system, mode, "single") + ".state"))
+    g = load_golden(_stem(system, mode, "single"))
     last = frames[-1].trajs[0]
     assert last.t == pytest.approx(g[-1, 0], rel=1e-5)
     tol = 1e-12 if system == "PTLA_CPPQED" else 1e-9
@@ -134,7 +147,7 @@ def test_ensemble_continues_from_reference_state(system, mode):
     """Ensemble archives hold every trajectory's stateIO in order (StochasticTrajectory.h:238);
     each continued member reaches its record in the final archive."""
     s, p, modes = _setup(system, "ensemble")
-    frames = SF.read(os.path.join(HERE, "golden", "state", f"run_{system}_{mode}_ensemble.state"))
+    frames = SF.read(os.path.join(HERE, "golden", "state", _stem(system, mode, "ensemble") + ".state"))
     assert all(len(f.trajs) == 3 for f in frames)
     run = modes[mode] if mode != "dc" else (GOLDEN_MODES["dt"] if system == "PTLA_CPPQED"
                                             else O.run_spec("Dt", Dt=0.1, T=0.1))
@@ -150,8 +163,8 @@ def test_master_continues_from_reference_state(system, mode):
     """Master archives (Master.h:89-102): continued from each saved instant the oracle reaches
     the final density operator, time step and ODE logger, and streams the remaining rows."""
     s, p, modes = _setup(system, "master")
-    frames = SF.read(os.path.join(HERE, "golden", "state", f"run_{system}_{mode}_master.state"))
-    g = load_golden(f"run_{system}_{mode}_master")
+    frames = SF.read(os.path.join(HERE, "golden", "state", _stem(system, mode, "master") + ".state"))
+    g = load_golden(_stem(system, mode, "master"))
     last = frames[-1].master
     tol = 1e-12 if system == "PTLA_CPPQED" else 1e-9
     for k in range(len(frames) - 1):
