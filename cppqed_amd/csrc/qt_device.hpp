@@ -50,10 +50,12 @@ __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.
 
 // Cross-lane moves through DPP (VALU latency, no LDS round trip).  Controls (GFX9 encoding):
 // quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
+// Every lane of these controls reads a lane of its own row, so no lane needs an old value
+// (mov_dpp: no initialising move of the destination).
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double readlane(double v, int l) {
@@ -97,6 +99,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp<0xB1>(v)); v = fmax(v, dpp<0x4E>(v)); v = fmax(v, dpp<0x141>(v)); v = fmax(v, dpp<0x140>(v));
   return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+}
+
+// three sums over the 64 lanes at once (interleaved butterflies: one latency for all three)
+__device__ __forceinline__ void wave_sum3(double v[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] += dpp<0xB1>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] += dpp<0x4E>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] += dpp<0x141>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] += dpp<0x140>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = (readlane(v[i], 0) + readlane(v[i], 16)) + (readlane(v[i], 32) + readlane(v[i], 48));
 }
 
 template <int NW> struct GroupReduce {
@@ -151,6 +167,62 @@ __device__ __forceinline__ double inv_root(double x) {
     r = fma(r * (1.0 / N), e, r);              // r (1 + e/N)
   }
   return r;
+}
+
+// x^N for a compile-time N by repeated squaring
+template <int N>
+__device__ __forceinline__ double ipow(double r) {
+  if constexpr (N == 1) return r;
+  else if constexpr (N % 2 == 0) { const double h = ipow<N / 2>(r); return h * h; }
+  else return r * ipow<N - 1>(r);
+}
+
+// m^(-1/N) for the controller on the SQUARED error norm m = err^2: err^(-1/3) = m^(-1/6),
+// err^(-1/5) = m^(-1/10).  FP32 estimate, two Newton steps in FP64 (as inv_root).  For m beyond
+// the FP32 range the estimate is 0, which the callers' clamps (dt *= max(..., 1/5)) absorb.
+template <int N>
+__device__ __forceinline__ double inv_root_sq(double m) {
+  double r = (double)exp2f(-__log2f((float)m) * (1.0f / N));
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-ipow<N>(r), m, 1.0);  // 1 - m r^N
+    r = fma(r * (1.0 / N), e, r);
+  }
+  return r;
+}
+
+// sqrt(x), x >= 0: the hardware reciprocal square root refined by a coupled Newton step and a
+// final residual correction (about an ulp from the IEEE result for 1e-300 < x < 1e300; 0 below)
+__device__ __forceinline__ double fast_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  s = fma(fma(-s, s, x), h, s);
+  return x > 1e-300 ? s : 0.;
+}
+
+// 1/sqrt(x), x > 0 in the normal range: hardware estimate and two Newton steps
+__device__ __forceinline__ double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// A read-only pointer into device (global) memory: loads go through the vector memory path
+// (global_load) instead of the flat path that generic pointers take.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
+// element i of a double2 array in global memory, loaded through the global path
+__device__ __forceinline__ double2 gld2(const double2* p, int i) {
+  const __attribute__((address_space(1))) double* q = (const __attribute__((address_space(1))) double*)p;
+  return make_double2(q[2 * i], q[2 * i + 1]);
 }
 
 // Cash-Karp RK4(5) tableau (as in Boost.Odeint's runge_kutta_cash_karp54)

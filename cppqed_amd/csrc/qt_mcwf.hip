@@ -253,7 +253,7 @@ struct Engine {
     sync();
   }
 
-  // One RKCK54 try: returns the odeint error norm; xn is the 5th-order proposal.
+  // One RKCK54 try: returns the SQUARE of the odeint error norm; xn is the 5th-order proposal.
   // k1 must hold (H/i) x on entry.
   // t, t0: time at the start of the try and picture origin (stage times t + c_i dt - t0 set the
   // interaction-picture phasors)
@@ -297,8 +297,8 @@ struct Engine {
         xe = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, xe))));
         if (valid(r)) {
           // odeint's rel_error: |xerr| / (epsAbs + epsRel (|x| + dt |dxdt|)); the maximum is taken
-          // over squared ratios and rooted once
-          const double ax = sqrt(cabs2(x[r])), ak = sqrt(cabs2(d1));
+          // over squared ratios and returned squared (the controller works on the square)
+          const double ax = fast_sqrt(cabs2(x[r])), ak = fast_sqrt(cabs2(d1));
           const double D = epsAbs + epsRel * (ax + dt * ak);
           double rc = __builtin_amdgcn_rcp(D);
           rc = fma(rc, fma(-D, rc, 1.0), rc);
@@ -307,7 +307,7 @@ struct Engine {
         }
       }
     }
-    return sqrt(gmax(e));
+    return gmax(e);
   }
 
   __device__ __forceinline__ void derivative(const double2 x[ROWS], T k[ROWS], double tau = 0.) const {
@@ -321,7 +321,8 @@ struct Engine {
     const int row = r * G + lane;
     double2 acc = make_double2(0., 0.);
     const int base = m * S.wj * S.dim + row;
-    for (int w = 0; w < S.wj; ++w) acc = cfma(S.jval[base + w * S.dim], ps[S.jcol[base + w * S.dim]], acc);
+    const auto jc = gbl(S.jcol);
+    for (int w = 0; w < S.wj; ++w) acc = cfma(gld2(S.jval, base + w * S.dim), ps[jc[base + w * S.dim]], acc);
     return acc;
   }
 
@@ -334,7 +335,33 @@ struct Engine {
     }
   }
 
-  __device__ double observable(int q, const double2 x[ROWS]) const {
+  // LDS scratch [8][64] for the batched stream-point reduction (one-wavefront teams; may be null)
+  double* ored = nullptr;
+
+  // all observables of a stream point at once (one wavefront with ored): each lane writes its
+  // partials of eight observables to LDS, lane 8j + c adds the partials of lanes 8c..8c+7 for
+  // observable j, three DPP steps finish the sum over the eight lanes, and lane 8j hands it to
+  // store(q, value) -- one cross-lane pass per eight observables instead of one each
+  template <class Store>
+  __device__ void observablesBatched(const double2 x[ROWS], int no, Store store) const {
+    for (int q0 = 0; q0 < no; q0 += 8) {
+      const int nq = no - q0 < 8 ? no - q0 : 8;
+      for (int j = 0; j < 8; ++j) ored[j * 64 + lane] = j < nq ? obsPartial(q0 + j, x) : 0.;
+      wave_sync();
+      const int j = lane >> 3, c = lane & 7;
+      double v = 0.;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v += ored[j * 64 + c * 8 + i];
+      v += dpp<0xB1>(v); v += dpp<0x4E>(v); v += dpp<0x141>(v);
+      if (c == 0 && j < nq) store(q0 + j, v);
+      wave_sync();
+    }
+  }
+
+  __device__ double observable(int q, const double2 x[ROWS]) const { return gsum(obsPartial(q, x)); }
+
+  // this lane's part of <x|O_q|x> (real or imaginary part as the observable asks)
+  __device__ double obsPartial(int q, const double2 x[ROWS]) const {
     double s = 0.;
     const bool im = S.oimag[q] != 0;
 #pragma unroll
@@ -343,12 +370,13 @@ struct Engine {
       const int row = r * G + lane;
       double2 acc = make_double2(0., 0.);
       const int base = q * S.wo * S.dim + row;
-      const int wq = S.owid[q];
-      for (int w = 0; w < wq; ++w) acc = cfma(S.oval[base + w * S.dim], ps[S.ocol[base + w * S.dim]], acc);
+      const int wq = gbl(S.owid)[q];
+      const auto oc = gbl(S.ocol);
+      for (int w = 0; w < wq; ++w) acc = cfma(gld2(S.oval, base + w * S.dim), ps[oc[base + w * S.dim]], acc);
       // conj(x) * acc
       s += im ? (x[r].x * acc.y - x[r].y * acc.x) : (x[r].x * acc.x + x[r].y * acc.y);
     }
-    return gsum(s);
+    return s;
   }
 };
 
@@ -426,7 +454,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
         const double2 d1 = kd(k1[r]);
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
         xe = axpy(f6, kd(k6[r]), axpy(f5, kd(k5[r]), axpy(f4, kd(k4[r]), axpy(f3, kd(k3[r]), xe))));
-        const double ax = sqrt(cabs2(xr(r))), ak = sqrt(cabs2(d1));
+        const double ax = fast_sqrt(cabs2(xr(r))), ak = fast_sqrt(cabs2(d1));
         const double D = epsAbs + epsRel * (ax + dt * ak);
         double rc = __builtin_amdgcn_rcp(D);
         rc = fma(rc, fma(-D, rc, 1.0), rc);
@@ -434,7 +462,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
         e = fmax(e, cabs2(xe) * (rc * rc));
       }
     }
-    const double err = sqrt(gmax(e));
+    const double err = gmax(e);  // squared error norm
     if (err <= 1.0) {
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
@@ -491,7 +519,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 
   auto coherent = [&](double Dt) {  // MCWF_Trajectory.h:250-276 with ODE.h:234-262
     const double timeBefore = t;
-    const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
+    const double nextDtTry = (fabs(Dt) * 10. < fabs(dtTry) ? dtTry : 0.);  // |Dt| < |dtTry| / 10
     if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
     eng.derivativeX(k1, t - t0);
@@ -499,13 +527,15 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     for (;;) {
       ++tries;
       const double err = eng.tryStepX(k1, dtTry, epsAbs, epsRel, derivs, t, t0);
+      // err is the squared norm m: err > 1 <=> m > 1, err < 0.5 <=> m < 0.25,
+      // err^(-1/3) = m^(-1/6), max(5^-5, err)^(-1/5) = max(5^-10, m)^(-1/10)
       if (err > 1.0) {
-        dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
+        dtTry *= fmax(9. / 10. * inv_root_sq<6>(err), 1. / 5.);
         ++failed;
         continue;
       }
       t += dtTry;
-      if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
+      if (err < 0.25) dtTry *= 9. / 10. * inv_root_sq<10>(fmax(0.00032 * 0.00032, err));
       break;
     }
     dtDid = t - timeBefore;
@@ -527,7 +557,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     double s2 = 0.;  // renorm (StateVector.h:148-153)
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(X[r * G + lane]);
-    const double inv = 1.0 / sqrt(eng.gsum(s2));
+    const double inv = fast_rsqrt(eng.gsum(s2));
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (eng.live(r)) { double2& x = X[r * G + lane]; x.x *= inv; x.y *= inv; }
     eng.sync();  // X complete before other lanes gather from it
@@ -550,7 +580,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
       const double Dt = endTime - t;
       if (Dt == 0.) break;
       const unsigned long long di = ord - ord0;
-      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
+      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? gbl(draws)[di] : -1.0;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) slot[r * G + lane] = X[r * G + lane];
       const double tCache = t, dtDidCache = dtDid;
@@ -654,7 +684,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   // coherent part of one MCWF step (MCWF_Trajectory.h:250-276 with ODE.h:234-262)
   auto coherent = [&](double Dt) {
     const double timeBefore = t;
-    const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
+    const double nextDtTry = (fabs(Dt) * 10. < fabs(dtTry) ? dtTry : 0.);  // |Dt| < |dtTry| / 10
     if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
     eng.derivative(x, k1, t - t0);
@@ -662,15 +692,16 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     for (;;) {
       ++tries;
       const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs, t, t0);
+      // err is the squared error norm m (tryStep): err > 1 <=> m > 1, err < 0.5 <=> m < 0.25
       if (err > 1.0) {
         // default_step_adjuster::decrease_step: dt *= max(0.9 err^(-1/(error_order-1)), 1/5)
-        dtTry *= fmax(9. / 10. * inv_root<3>(err), 1. / 5.);
+        dtTry *= fmax(9. / 10. * inv_root_sq<6>(err), 1. / 5.);
         ++failed;
         continue;
       }
       t += dtTry;
       // increase_step: if err < 0.5, dt *= 0.9 max(5^-stepper_order, err)^(-1/stepper_order)
-      if (err < 0.5) dtTry *= 9. / 10. * inv_root<5>(fmax(0.00032, err));
+      if (err < 0.25) dtTry *= 9. / 10. * inv_root_sq<10>(fmax(0.00032 * 0.00032, err));
       break;
     }
 #pragma unroll
@@ -691,25 +722,47 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
       t0 = t;
     }
     if (lane == 0) { cnt[kOdeSteps] += 1; cnt[kTries] += tries; cnt[kFailed] += failed; cnt[kDerivs] += derivs; }
-    // renorm (StateVector.h:148-153)
-    double s2 = 0.;
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) s2 += cabs2(x[r]);
-    const double inv = 1.0 / sqrt(eng.gsum(s2));
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
   };
 
-  auto computeRates = [&]() {
-    eng.stagePsi(x);
-    for (int m = 0; m < nj; ++m) {
-      double sacc = 0.;
+  // renorm (StateVector.h:148-153) and, when there are jumps, the rates of the renormalised
+  // state (MCWF_Trajectory.h:353-356): ||J_m psi/|psi| ||^2 = ||J_m psi||^2 / |psi|^2, so the norm
+  // and the rates come from one pass over the staged unnormalised state and one joint reduction.
+  // psi stays staged unnormalised in ps; jumpScale = 1/|psi| converts J ps to J psi.
+  double jumpScale = 1.;
+  auto renormRates = [&](bool withRates) {
+    double part[3] = {0., 0., 0.};
+    const bool fused = withRates && nj <= 2;
+    if (withRates) eng.stagePsi(x);
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
-      const double v = eng.gsum(sacc);
-      if (lane == 0) rts[m] = v;
+    for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) part[0] += cabs2(x[r]);
+    if (fused) {
+      for (int m = 0; m < nj; ++m) {
+        double sacc = 0.;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+        part[1 + m] = sacc;
+      }
+      wave_sum3(part);
+    } else {
+      part[0] = eng.gsum(part[0]);
     }
-    eng.sync();
+    const double inv = fast_rsqrt(part[0]);
+    const double inv2 = inv * inv;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
+    jumpScale = inv;
+    if (fused) {
+      if (lane == 0) for (int m = 0; m < nj; ++m) rts[m] = part[1 + m] * inv2;
+    } else if (withRates) {
+      for (int m = 0; m < nj; ++m) {
+        double sacc = 0.;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+        const double v = eng.gsum(sacc) * inv2;
+        if (lane == 0) rts[m] = v;
+      }
+    }
+    if (withRates) eng.sync();
   };
 
   for (int k = 0; k < E.n_intervals; ++k) {
@@ -720,7 +773,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
       // MCWF_Trajectory::step (MCWF_Trajectory.h:347-374)
       // the step's jump draw, fetched early so the load completes behind the RK work
       const unsigned long long di = ord - ord0;
-      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? draws[di] : -1.0;
+      const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? gbl(draws)[di] : -1.0;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         if constexpr (PC_LDS) pc[r * G + lane] = x[r];
@@ -728,8 +781,8 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
       }
       const double tCache = t, dtDidCache = dtDid;
       coherent(Dt);
+      renormRates(nj > 0);
       if (nj > 0) {
-        computeRates();
         for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
           double totalRate = 0.;
           for (int m = 0; m < nj; ++m) totalRate += rts[m];
@@ -743,7 +796,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
               else x[r] = pcr[r];
             }
             coherent(Dt);
-            computeRates();
+            renormRates(true);
             continue;
           } else if (totalRate * dtTry > E.dpLimit) bump(kOvershoots, 1);
           dtTry = fmin(dtTry, sugg);
@@ -759,7 +812,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
           --no;
           const double nf = sqrt(rts[no]);
           if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
-          const double inv = 1.0 / nf;
+          const double inv = jumpScale / nf;  // ps holds psi before the renorm
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
             const double2 y = eng.valid(r) ? eng.jumpRow(no, r) : make_double2(0., 0.);
@@ -779,9 +832,14 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     }
     // stream point: raw linear averages of the normalised state
     eng.stagePsi(x);
-    for (int q = 0; q < S.no; ++q) {
-      const double v = eng.observable(q, x);
-      if (lane == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = v;
+    double* obsRow = E.obs_out + ((size_t)traj * E.n_intervals + k) * S.no;
+    if (NW == 1 && eng.ored) {
+      eng.observablesBatched(x, S.no, [&](int q, double v) { obsRow[q] = v; });
+    } else {
+      for (int q = 0; q < S.no; ++q) {
+        const double v = eng.observable(q, x);
+        if (lane == 0) obsRow[q] = v;
+      }
     }
     if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
   }
@@ -836,7 +894,9 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // state copy share storage
     __shared__ double2 ps[ROWS * G];
     __shared__ double2 pc[kPcLds ? ROWS * G : 1];
+    __shared__ double ored[8 * 64];
     Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane, ph);
+    eng.ored = ored;
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
   }
 }
@@ -1031,7 +1091,7 @@ __global__ void __launch_bounds__(64 * NW) plain_try_kernel(SysDev S, PlainTry P
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? P.x[r * G + threadIdx.x] : make_double2(0., 0.);
   long long derivs = 0;
   eng.derivative(x, k1);  // controlled_runge_kutta::try_step_v1 recomputes dxdt on every try
-  const double err = eng.tryStep(x, k1, P.dt, P.epsAbs, P.epsRel, xn, derivs);
+  const double err = sqrt(eng.tryStep(x, k1, P.dt, P.epsAbs, P.epsRel, xn, derivs));  // tryStep: squared norm
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) P.xn[r * G + threadIdx.x] = xn[r];
   if (threadIdx.x == 0) *P.err = err;
