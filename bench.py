@@ -80,20 +80,25 @@ def main():
     def run_batched(precision, steps, warmup):
         ens = Q.Ensemble(sysq, args.ntraj, psi0, Q.mcwf_pars(seed=1001, precision=precision, jump_log_cap=64))
         ens.set_graph(True)  # every step replays the captured launch sequence
-        for _ in range(warmup):
+        per_step = None
+        for _ in range(max(warmup, 1)):
             ens.reset()
             ens.advance(iv)
+            per_step = int(ens.counters()[1][4])
         ms0, n0 = ens.kernel_ms()
-        tries = 0
         t0 = time.perf_counter()
         for _ in range(steps):
             ens.reset()
             ens.advance(iv)
-            tries += int(ens.counters()[1][4])
         el = time.perf_counter() - t0
         ms1, n1 = ens.kernel_ms()
         kernel_s = (ms1 - ms0) / 1e3
         _, tot = ens.counters()
+        # every step starts from the same reset state and RNG ordinals, so it makes the same
+        # tries; the counters are read outside the timed region and checked against a warm-up
+        if int(tot[4]) != per_step:
+            raise RuntimeError("a step's try count differs from the warm-up's: %d vs %d" % (tot[4], per_step))
+        tries = per_step * steps
         captures, replays = ens.set_graph(None)
         return dict(tries=tries, seconds=el, kernel_s=kernel_s, launches=n1 - n0, last_totals=tot.tolist(),
                     graph={"captures": captures, "replays": replays})
