@@ -219,10 +219,18 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 
-// element i of a double2 array in global memory, loaded through the global path
+// element i of a double2 / float2 array in global memory, loaded through the global path
 __device__ __forceinline__ double2 gld2(const double2* p, int i) {
   const __attribute__((address_space(1))) double* q = (const __attribute__((address_space(1))) double*)p;
   return make_double2(q[2 * i], q[2 * i + 1]);
+}
+__device__ __forceinline__ double2 gld2(const double2* p, unsigned i) {
+  const __attribute__((address_space(1))) double* q = (const __attribute__((address_space(1))) double*)p;
+  return make_double2(q[2 * i], q[2 * i + 1]);
+}
+__device__ __forceinline__ float2 gld2(const float2* p, unsigned i) {
+  const __attribute__((address_space(1))) float* q = (const __attribute__((address_space(1))) float*)p;
+  return make_float2(q[2 * i], q[2 * i + 1]);
 }
 
 // Cash-Karp RK4(5) tableau (as in Boost.Odeint's runge_kutta_cash_karp54)

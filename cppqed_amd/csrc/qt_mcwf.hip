@@ -72,8 +72,8 @@ struct Engine {
   int nfU = 0;
 
   // unsigned element offsets from a scalar base: loads address as scalar base + 32-bit vector offset
-  __device__ __forceinline__ T coef(int w, int row) const { return hvalU[(unsigned)(w * dimU + row)]; }
-  __device__ __forceinline__ int col(int w, int row) const { return hcolU[(unsigned)(w * dimU + row)]; }
+  __device__ __forceinline__ T coef(int w, int row) const { return gld2(hvalU, (unsigned)(w * dimU + row)); }
+  __device__ __forceinline__ int col(int w, int row) const { return gbl(hcolU)[(unsigned)(w * dimU + row)]; }
 
   // lane_ = index of this thread within its team of NW wavefronts; ph_ = LDS phasor table
   // [kMaxFreqs] (needed for interaction-picture systems only)
@@ -214,7 +214,7 @@ struct Engine {
         if (row < dimU) {
           for (int w = 0; w < whU; ++w) {
             const unsigned o = (unsigned)(w * dimU + row);
-            acc = cfma(cmul(hvalU[o], ph[hfidxU[o]]), xs[hcolU[o]], acc);
+            acc = cfma(cmul(gld2(hvalU, o), ph[gbl(hfidxU)[o]]), xs[gbl(hcolU)[o]], acc);
           }
         }
       }
@@ -483,7 +483,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
       double2 acc = make_double2(0., 0.);
       const int base = q * S.wo * S.dim + row;
       const int wq = S.owid[q];
-      for (int w = 0; w < wq; ++w) acc = cfma(S.oval[base + w * S.dim], X[S.ocol[base + w * S.dim]], acc);
+      for (int w = 0; w < wq; ++w) acc = cfma(gld2(S.oval, base + w * S.dim), X[gbl(S.ocol)[base + w * S.dim]], acc);
       const double2 x = X[row];
       s += im ? (x.x * acc.y - x.y * acc.x) : (x.x * acc.x + x.y * acc.y);
     }
