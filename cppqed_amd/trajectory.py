@@ -53,22 +53,44 @@ class AutostopHandler:
         return False
 
 
+def format_row(row, precision=6, groups=None):
+    """One streamed row as the reference writes it (trajectory::stream, Trajectory.h:426-434, with
+    FormDouble, FormDouble.h / FormDouble.cc): time and dtDid left-aligned in the width of a
+    positive number of this precision (exponent digits + precision + 4 = 13 at 6 digits), then
+    each element's block of averages after a tab, every average left-aligned in the width of any
+    number (14), C++ default float format.  groups: averages per element (default: one block)."""
+    p = max(6, precision)            # formdouble::actualPrecision
+    wpos = 3 + p + 4                 # formdouble::widthPositive: int(log10(307)) + 1 exponent digits
+    wany = wpos + 1                  # formdouble::widthAny
+    text = "".join("%-*s" % (wpos, "%.*g" % (p, v)) for v in row[:2])
+    av = list(row[2:])
+    for n in (groups or [len(av)]):
+        text += "\t" + "".join("%-*s" % (wany, "%.*g" % (p, v)) for v in av[:n])
+        av = av[n:]
+    return text + "\n"
+
+
 def run(ens, T=None, Dt=0.1, NDt=None, sdf=0, state_file=None, autostop=None, first_state_stream=True,
-        continuing=None):
+        continuing=None, trajectory_file=None, precision=6):
     """Runs an Ensemble in Dt mode up to time T, or for NDt intervals of Dt.
 
     Returns the streamed rows (time, mean dtDid, processed averages).  ``sdf``: state-dump
-    frequency in stream points (0: only the end); ``state_file``: the archive file; when it
-    exists and is not empty (or ``continuing`` is True) the run continues from its last archive
-    and streams no row at its start (Trajectory.h:484-500, 576-585); ``autostop``: an
-    AutostopHandler or a tuple (epsilon, epsilon_abs, repetition).
+    frequency in stream points (0: only the end); ``state_file``: the archive file (default
+    ``trajectory_file + ".state"``); ``trajectory_file``: the rows are appended to it in the
+    reference's text format; the run continues from the last archive of the state file when the
+    trajectory file (or, without one, the state file) exists and is not empty, or when
+    ``continuing`` is True, and then streams no row at its start (Trajectory.h:484-500,
+    576-585); ``autostop``: an AutostopHandler or a tuple (epsilon, epsilon_abs, repetition).
     """
     if (T is None) == (NDt is None):
         raise ValueError("give either T (end time) or NDt (number of intervals)")
     if autostop is not None and not isinstance(autostop, AutostopHandler):
         autostop = AutostopHandler(*autostop)
+    if trajectory_file and not state_file:
+        state_file = trajectory_file + ".state"
     if continuing is None:
-        continuing = bool(state_file) and os.path.exists(state_file) and os.path.getsize(state_file) > 0
+        probe = trajectory_file or state_file
+        continuing = bool(probe) and os.path.exists(probe) and os.path.getsize(probe) > 0
     if continuing:
         ens.load_state(state_file)
     t = ens.time()
@@ -76,6 +98,18 @@ def run(ens, T=None, Dt=0.1, NDt=None, sdf=0, state_file=None, autostop=None, fi
     if time_to_reach and time_to_reach <= t:
         return np.zeros((0, 2 + ens.system.n_avr))
 
+    out = open(trajectory_file, "a") if trajectory_file else None
+    groups = [4] * (ens.system.n_avr // 4)  # every element with averages streams four
+    if out:
+        if continuing:
+            out.write("# Continuing from time %s up to time %s\n" % ("%.*g" % (max(6, precision), t),
+                                                                      "%.*g" % (max(6, precision), time_to_reach)))
+        else:
+            out.write("# cppqed_amd MCWF ensemble of %d trajectories, dimension %d\n\n# Key to data:\n"
+                      "# Trajectory\n#  1. time\n#  2. dtDid\n#  3-%d. averages of the elements, in element order\n"
+                      "# Run Trajectory up to time %s -- Stream period: %s\n\n"
+                      % (ens.n, ens.system.dim, 2 + ens.system.n_avr, "%.*g" % (max(6, precision), time_to_reach),
+                         "%.*g" % (max(6, precision), Dt)))
     rows, saved, state_count = [], False, 0
     step_by_step = autostop is not None and autostop.rep > 0
 
@@ -88,6 +122,8 @@ def run(ens, T=None, Dt=0.1, NDt=None, sdf=0, state_file=None, autostop=None, fi
         state_count += 1
         if count or not continuing:
             rows.append(row)
+            if out:
+                out.write(format_row(row, precision, groups))
             if autostop is not None and autostop(row[2:]):
                 return True
         return False
@@ -120,6 +156,10 @@ def run(ens, T=None, Dt=0.1, NDt=None, sdf=0, state_file=None, autostop=None, fi
         for j in range(n):
             count += 1
             stop = at_stream_point(chunk[j], count)
+    if stop and out:
+        out.write("# Stopping criterion has been reached\n")
     if state_file and not saved:
         ens.save_state(state_file)
+    if out:
+        out.close()
     return np.array(rows)

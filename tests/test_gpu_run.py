@@ -40,3 +40,20 @@ def test_run_autostop_truncates(tmp_path):
     h = AutostopHandler(2e-2, 1e-12, 4)
     stops = [h(r[2:]) for r in full]
     assert stops.index(True) == len(rows) - 1
+
+
+def test_run_writes_the_reference_text_format(tmp_path):
+    """trajectory_file: rows in the reference's text format, appended by a continuing run."""
+    print("[gpu] run loop: trajectory file", flush=True)
+    s = Q.System(JC60)
+    pars = Q.mcwf_pars(seed=17, jump_log_cap=16)
+    path = str(tmp_path / "jc.d")
+    first = run(Q.Ensemble(s, 64, ground(60), pars), T=1.0, Dt=0.1, trajectory_file=path)
+    rest = run(Q.Ensemble(s, 64, ground(60), pars), T=2.0, Dt=0.1, trajectory_file=path)
+    text = open(path).read()
+    assert "# Continuing from time 1 up to time 2" in text
+    data = np.array([[float(x) for x in ln.split()] for ln in text.splitlines() if ln and not ln.startswith("#")])
+    # ten steps of 0.1 end 1.1e-16 short of 1: the first run streams that tail interval too, as
+    # the reference's golden files show ("1  1.11022e-16")
+    assert data.shape == (len(first) + len(rest), 2 + s.n_avr) and len(rest) == 10
+    np.testing.assert_allclose(data, np.vstack([first, rest]), rtol=1e-5, atol=1e-12)

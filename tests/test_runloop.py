@@ -1,4 +1,5 @@
-"""The reference's autostop criterion, AutostopHandlerGeneric (CPPQEDutils/Trajectory.h:179-222),
+"""The run loop around the path: the reference's autostop criterion, AutostopHandlerGeneric
+(CPPQEDutils/Trajectory.h:179-222),
 in the oracle and in the host run loop (CPU only).
 
 The known answers below follow from the handler's definition by hand: the first call only
@@ -74,3 +75,23 @@ def test_master_autostop_truncates_the_run():
     upto, rho_upto = O.master_run(s, rho0, p, O.run_spec("Dt", Dt=0.1, T=rows[-1, 0]))
     np.testing.assert_array_equal(upto, rows)
     np.testing.assert_array_equal(rho_upto, rho)
+
+
+def test_row_format_reproduces_the_reference_files():
+    """trajectory::stream's text (Trajectory.h:426-434, FormDouble): every data row of the golden
+    files, re-streamed from its own values, is byte-identical (time and dtDid in 13 columns, a
+    tab before each element's four averages, averages in 14 columns, 6 significant digits)."""
+    import glob
+    import os
+
+    from cppqed_amd.trajectory import format_row
+    files = glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "run_*"))
+    n = 0
+    for f in files:
+        for ln in open(f):
+            if ln.startswith("#") or not ln.strip():
+                continue
+            v = [float(x) for x in ln.split()]
+            assert format_row(v, groups=[4] * ((len(v) - 2) // 4)) == ln
+            n += 1
+    assert n > 400
