@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libqtraj.so")
 
 QT_FP64, QT_FP32_DERIV = 0, 1
+QT_CONTROLLER_ODEINT, QT_CONTROLLER_GSL = 0, 1
 
 _lib = None
 
@@ -293,9 +294,11 @@ def run_intervals(T=None, Dt=0.1, NDt=0):
 # ---------------------------------------------------------------- MCWF ensemble
 
 def mcwf_pars(epsRel=1e-6, epsAbs=1e-12, dpLimit=0.01, overshootTol=10.0, dtInit=0.0, precision=QT_FP64,
-              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0), occupancy=0, group_lanes=0):
-    """mcwf::Pars + ode_engine::Pars with the reference defaults (MCWF_Trajectory.h:28-47, ODE.h:31-63)."""
-    return McwfParams(epsRel, epsAbs, dpLimit, overshootTol, dtInit, precision, 0, seed, traj0,
+              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0), occupancy=0, group_lanes=0, controller="odeint"):
+    """mcwf::Pars + ode_engine::Pars with the reference defaults (MCWF_Trajectory.h:28-47, ODE.h:31-63).
+    controller: "odeint" (ODE_EngineBoost, the default) or "gsl" (ODE_EngineGSL, ODE_GSL.h)."""
+    ctl = {"odeint": QT_CONTROLLER_ODEINT, "gsl": QT_CONTROLLER_GSL}[controller]
+    return McwfParams(epsRel, epsAbs, dpLimit, overshootTol, dtInit, precision, ctl, seed, traj0,
                       jump_log_cap, layout[0], layout[1], occupancy, group_lanes)
 
 

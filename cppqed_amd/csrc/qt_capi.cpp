@@ -21,12 +21,12 @@
 
 namespace qt {
 bool layout_supported(int rows, int nw, int wh);
-bool batch_supported(int rows, int nw, int wh, int minw, bool td);
-int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td);
+bool batch_supported(int rows, int nw, int wh, int minw, bool td, bool gsl);
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl);
 bool sub_supported(int g, int rows, int wh);
 int sub_blocks_per_cu(int g, int rows, int wh, bool f32);
 hipError_t launch_sub(int g, int rows, int wh, bool f32, int grid, const SysDev& S, const EnsDev& E, hipStream_t st);
-hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, int grid, const SysDev& S,
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl, int grid, const SysDev& S,
                         const EnsDev& E, hipStream_t st);
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
                              int n_traj, int cap, double* draws, hipStream_t st);
@@ -155,7 +155,8 @@ void check_params(const qt_mcwf_params* p) {
   if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
   if (!(p->dpLimit > 0) || !(p->overshootTol >= 1)) throw Fail(QT_EINVAL, "dpLimit > 0 and overshootTol >= 1 required");
   if (p->precision != QT_FP64 && p->precision != QT_FP32_DERIV) throw Fail(QT_EINVAL, "unknown precision");
-  if (p->controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "only the odeint controller runs on the device");
+  if (p->controller != QT_CONTROLLER_ODEINT && p->controller != QT_CONTROLLER_GSL)
+    throw Fail(QT_EINVAL, "unknown controller");
 }
 
 }  // namespace
@@ -309,6 +310,7 @@ struct qt_ens {
     E.epsAbs = p.epsRel; E.epsRel = p.epsAbs;  // ODE.h:205 hands (epsRel, epsAbs) to make_controlled(abs, rel, ...)
     E.dpLimit = p.dpLimit; E.overshootTol = p.overshootTol;
     E.work = work.p; E.err_flag = errflag.p;
+    E.gsl = p.controller == QT_CONTROLLER_GSL;
     return E;
   }
 };
@@ -350,12 +352,12 @@ std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& p
   return r;
 }
 
-int resident_groups(const LayoutChoice& L, bool f32, bool td) {
+int resident_groups(const LayoutChoice& L, bool f32, bool td, bool gsl = false) {
   int dev = 0;
   hipCheck(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
   hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  int per_cu = L.glanes > 0 ? qt::sub_blocks_per_cu(L.glanes, L.rows, L.wh, f32) : qt::batch_blocks_per_cu(L.rows, L.nw, L.wh, L.occ, f32, td);
+  int per_cu = L.glanes > 0 ? qt::sub_blocks_per_cu(L.glanes, L.rows, L.wh, f32) : qt::batch_blocks_per_cu(L.rows, L.nw, L.wh, L.occ, f32, td, gsl);
   if (per_cu < 1) per_cu = 1;
   return prop.multiProcessorCount * per_cu;
 }
@@ -461,22 +463,27 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
                         p->layout_rows);
     } else {
       e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
-      if (sys->interactionPicture()) {
-        // interaction-picture variants: no occupancy hint, the ELL width may round up
+      const bool gsl = p->controller == QT_CONTROLLER_GSL;
+      if (gsl && e->lay.nw != 1)
+        throw Fail(QT_ENOTSUP, "the GSL controller runs in the one-wavefront engine (dimensions up to 256)");
+      if (sys->interactionPicture() || gsl) {
+        // interaction-picture and GSL variants: no occupancy hint, the ELL width may round up
         e->lay.occ = 1;
         int w = e->lay.wh;
-        while (w <= 16 && !qt::batch_supported(e->lay.rows, e->lay.nw, w, 1, true)) w *= 2;
-        if (w > 16) throw Fail(QT_ENOTSUP, "no interaction-picture variant for this layout");
+        while (w <= 16 && !qt::batch_supported(e->lay.rows, e->lay.nw, w, 1, sys->interactionPicture(), gsl)) w *= 2;
+        if (w > 16) throw Fail(QT_ENOTSUP, "no compiled variant of this picture and controller for this layout");
         e->lay.wh = w;
       } else {
         e->lay.occ = p->layout_occupancy > 0 ? p->layout_occupancy : default_occupancy(e->lay);
-        if (!qt::batch_supported(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, false))
+        if (!qt::batch_supported(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, false, false))
           throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
       }
     }
     const long d = s.dim;
     std::vector<qt::cd> psi(d);
     for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
+    if (p->controller == QT_CONTROLLER_GSL && useSub)
+      throw Fail(QT_ENOTSUP, "the GSL controller runs in the one-wavefront engine (dimensions up to 256)");
     double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
     // constructor check (MCWF_Trajectory.h:114-117): dtTry <= dpLimit / total rate at t = 0
     double dtTry = dtInit;
@@ -503,7 +510,8 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->work.alloc(1);
     e->errflag.upload(std::vector<int>(1, 0));
     const int perGroup = e->lay.glanes > 0 ? 64 / e->lay.glanes : 1;  // trajectories per work group
-    e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV, sys->interactionPicture()));
+    e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV, sys->interactionPicture(),
+                                                                        p->controller == QT_CONTROLLER_GSL));
     hipCheck(hipEventCreate(&e->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&e->e1), "hipEventCreate");
     *out = e.release();
@@ -547,7 +555,7 @@ static void ens_kernel(qt_ens* e, const qt::EnsDev& E, hipStream_t st) {
              "mcwf sub-wave launch");
   else
     hipCheck(qt::launch_batch(e->lay.rows, e->lay.nw, e->lay.wh, e->lay.occ, e->p.precision == QT_FP32_DERIV,
-                              e->sys->interactionPicture(), e->grid,
+                              e->sys->interactionPicture(), e->p.controller == QT_CONTROLLER_GSL, e->grid,
                               e->sys->dev, E, st),
              "mcwf batch launch");
 }
@@ -885,6 +893,8 @@ int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params
       if (!members[m]) throw Fail(QT_EINVAL, "null member");
       if (members[m]->sys->interactionPicture())
         throw Fail(QT_ENOTSUP, "mixed batches run the Schroedinger picture only");
+      if (members[m]->p.controller != QT_CONTROLLER_ODEINT)
+        throw Fail(QT_ENOTSUP, "mixed batches run the odeint controller only");
       b->mem.push_back(members[m]);
     }
     b->f32 = members[0]->p.precision == QT_FP32_DERIV;
@@ -1289,6 +1299,7 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     check_params(p);
     const qt::System& s = *sys->h;
     if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the plain form runs the Schroedinger picture only");
+    if (p->controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "the plain form runs the odeint controller only");
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
     sys->upload();
     LayoutChoice L = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
@@ -1452,6 +1463,7 @@ int qt_server_create(const qt_server_params* p, qt_server** out) {
     if (p->chunk_intervals < 1) throw Fail(QT_EINVAL, "chunk_intervals must be positive");
     if (p->hbm_budget == 0) throw Fail(QT_EINVAL, "zero memory budget");
     check_params(&p->mcwf);
+    if (p->mcwf.controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "the request loop runs the odeint controller only");
     std::unique_ptr<qt_server> s(new qt_server);
     s->P = *p;
     s->batch.f32 = p->mcwf.precision == QT_FP32_DERIV;

@@ -67,7 +67,7 @@ struct EnsDev {
   double* obs_out;              // [n_traj][n_intervals][no]
   double* dtdid_out;            // [n_traj][n_intervals]
   int* work;                    // work-queue head (trajectories are claimed in order)
-  int status_bad;               // unused
+  int gsl;                      // 1: GSL's controller (kernel variant chosen on the host), 0: odeint's
   int* err_flag;                // set to 1 when a trajectory hits a non-finite jump norm
   const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals ordinal[traj] + i (may be null)
   int draw_cap;

@@ -257,9 +257,12 @@ struct Engine {
   // k1 must hold (H/i) x on entry.
   // t, t0: time at the start of the try and picture origin (stage times t + c_i dt - t0 set the
   // interaction-picture phasors)
+  // GSL: GSL's controller instead (returns GSL's error ratio, not squared; see the end)
+  template <bool GSL = false>
   __device__ double tryStep(const double2 x[ROWS], const T k1[ROWS], double dt, double epsAbs, double epsRel,
                             double2 xn[ROWS], long long& derivs, double t = 0., double t0 = 0.) const {
     T k2[ROWS], k3[ROWS], k4[ROWS], k5[ROWS], k6[ROWS];
+    double2 xeg[GSL ? ROWS : 1];
     double2 v[ROWS];
     const double s21 = CK::a21 * dt;
 #pragma unroll
@@ -295,6 +298,7 @@ struct Engine {
         xn[r] = axpy(q6, d6, axpy(q4, d4, axpy(q3, d3, axpy(q1, d1, x[r]))));
         double2 xe = make_double2(f1 * d1.x, f1 * d1.y);
         xe = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, xe))));
+        if constexpr (GSL) { xeg[r] = xe; continue; }
         if (valid(r)) {
           // odeint's rel_error: |xerr| / (epsAbs + epsRel (|x| + dt |dxdt|)); the maximum is taken
           // over squared ratios and returned squared (the controller works on the square)
@@ -306,6 +310,24 @@ struct Engine {
           e = fmax(e, cabs2(xe) * (rc * rc));
         }
       }
+    }
+    if constexpr (GSL) {
+      // GSL's rkck + control_standard + hadjust (ODE_GSL.h:44-84): the derivative at the new point
+      // (dydt_out, a seventh evaluation), then max over the real components of
+      // |yerr| / (epsAbs + epsRel (|y| + |dt dydt|)) with the new state y (not squared)
+      T kg[ROWS];
+      derivative(xn, kg, (t + dt) - t0);
+      ++derivs;
+      double m = 2.2250738585072014e-308;  // DBL_MIN, gsl_odeiv2_control_hadjust's start
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (!valid(r)) continue;
+        const double2 g = to_d(kg[r]);
+        const double Dx = epsAbs + epsRel * (fabs(xn[r].x) + fabs(dt * g.x));
+        const double Dy = epsAbs + epsRel * (fabs(xn[r].y) + fabs(dt * g.y));
+        m = fmax(m, fmax(fabs(xeg[r].x) / Dx, fabs(xeg[r].y) / Dy));
+      }
+      return gmax(m);
     }
     return gmax(e);
   }
@@ -657,7 +679,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 // Called by every lane of a team (eng.lane); pc (LDS, when PC_LDS) keeps psi at the start of the
 // MCWF step for a step-back, rts / cnt (LDS) hold the step's jump rates and the trajectory's
 // counters.
-template <class Eng, bool PC_LDS>
+template <class Eng, bool PC_LDS, bool GSL = false>
 __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, const Eng& eng, int traj, double2* pc,
                                              double* rts, long long* cnt) {
   constexpr int ROWS = Eng::kRows, G = Eng::G, NW = Eng::kNW;
@@ -680,6 +702,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   const unsigned long long key = E.traj0 + (unsigned long long)traj;
   const int nj = S.nj;
   const double epsAbs = E.epsAbs, epsRel = E.epsRel;
+  constexpr bool gsl = GSL;  // GSL controller (ODE_GSL.h) instead of odeint's
 
   // coherent part of one MCWF step (MCWF_Trajectory.h:250-276 with ODE.h:234-262)
   auto coherent = [&](double Dt) {
@@ -689,7 +712,23 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
     eng.derivative(x, k1, t - t0);
     long long derivs = 1, tries = 0, failed = 0;
-    for (;;) {
+    if constexpr (gsl) {
+      // ControlledErrorStepperGSL::try_step (ODE_GSL.h:44-84): every step is accepted, then
+      // gsl_odeiv2_control_hadjust sets the next dt from the error ratio (RKCK error order 4)
+      ++tries;
+      const double rmax = eng.template tryStep<true>(x, k1, dtTry, epsAbs, epsRel, xn, derivs, t, t0);
+      const double hOld = dtTry;
+      t += dtTry;
+      if (rmax > 1.1) {
+        const double r = 0.9 * inv_root<4>(rmax);
+        dtTry = (r < 0.2 ? 0.2 : r) * hOld;
+      } else if (rmax < 0.5) {
+        // 0.9 rmax^(-1/5) exceeds the cap 5 below rmax = 0.18^5
+        const double r = rmax < 1.8895680000000003e-4 ? 5.0 : 0.9 * inv_root<5>(rmax);
+        dtTry = (r > 5.0 ? 5.0 : r < 1.0 ? 1.0 : r) * hOld;
+      }
+    }
+    for (; !gsl;) {
       ++tries;
       const double err = eng.tryStep(x, k1, dtTry, epsAbs, epsRel, xn, derivs, t, t0);
       // err is the squared error norm m (tryStep): err > 1 <=> m > 1, err < 0.5 <=> m < 0.25
@@ -859,8 +898,9 @@ constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2)
 // batched persistent MCWF kernel: one system, one team = one work group
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD>
+template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD, bool GSL = false>
 __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
+  static_assert(!GSL || NW == 1, "the GSL controller runs in the one-wavefront engine");
   using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
@@ -897,7 +937,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     __shared__ double ored[8 * 64];
     Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane, ph);
     eng.ored = ored;
-    for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds>(S, E, eng, traj, pc, rts, cnt);
+    for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds, GSL>(S, E, eng, traj, pc, rts, cnt);
   }
 }
 
@@ -1210,6 +1250,11 @@ struct Layout { int rows, nw, wh; };
   X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1) \
   X(1, 4, 16, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
 
+// GSL-controller variants (ODE_GSL.h), Schroedinger and interaction picture: the one-wavefront
+// layouts, the ELL width rounded up to a compiled one
+#define QT_BATCH_GSL(X) \
+  X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1)
+
 bool layout_supported(int rows, int nw, int wh) {
 #define QT_CASE(R, N, W) if (rows == R && nw == N && wh == W) return true;
   QT_LAYOUTS(QT_CASE)
@@ -1217,47 +1262,60 @@ bool layout_supported(int rows, int nw, int wh) {
   return false;
 }
 
-bool batch_supported(int rows, int nw, int wh, int minw, bool td) {
+bool batch_supported(int rows, int nw, int wh, int minw, bool td, bool gsl) {
 #define QT_CASE(R, N, W, M) if (rows == R && nw == N && wh == W && minw == M) return true;
-  if (td) { QT_BATCH_TD(QT_CASE) }
+  if (gsl) { QT_BATCH_GSL(QT_CASE) }
+  else if (td) { QT_BATCH_TD(QT_CASE) }
   else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
   return false;
 }
 
 // resident work groups of a batched variant per compute unit (registers and LDS permitting)
-int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td) {
+int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl) {
   int n = 0;
-#define QT_CASE_T(R, N, W, M, TDV)                                                                                  \
+#define QT_CASE_T(R, N, W, M, TDV, GV)                                                                               \
   if (rows == R && nw == N && wh == W && minw == M) {                                                               \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M, TDV>, 64 * N, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M, TDV>, 64 * N, 0);    \
+    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M, TDV, GV>, 64 * N, 0); \
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M, TDV, GV>, 64 * N, 0);    \
     return n;                                                                                                       \
   }
-#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false)
-#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true)
-  if (td) { QT_BATCH_TD(QT_CASE_TD) }
+#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false, false)
+#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true, false)
+#define QT_CASE_G(R, N, W, M) QT_CASE_T(R, N, W, M, false, true)
+#define QT_CASE_GTD(R, N, W, M) QT_CASE_T(R, N, W, M, true, true)
+  if (gsl && td) { QT_BATCH_GSL(QT_CASE_GTD) }
+  else if (gsl) { QT_BATCH_GSL(QT_CASE_G) }
+  else if (td) { QT_BATCH_TD(QT_CASE_TD) }
   else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
 #undef QT_CASE_TD
+#undef QT_CASE_G
+#undef QT_CASE_GTD
 #undef QT_CASE_T
   return 0;
 }
 
-hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, int grid, const SysDev& S,
+hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl, int grid, const SysDev& S,
                         const EnsDev& E, hipStream_t st) {
-#define QT_CASE_T(R, N, W, M, TDV)                                                                            \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                         \
-    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M, TDV>), dim3(grid), dim3(64 * N), 0, st, S, E); \
-    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M, TDV>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
-    return hipGetLastError();                                                                                 \
+#define QT_CASE_T(R, N, W, M, TDV, GV)                                                                              \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                               \
+    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E); \
+    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
+    return hipGetLastError();                                                                                       \
   }
-#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false)
-#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true)
-  if (td) { QT_BATCH_TD(QT_CASE_TD) }
+#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false, false)
+#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true, false)
+#define QT_CASE_G(R, N, W, M) QT_CASE_T(R, N, W, M, false, true)
+#define QT_CASE_GTD(R, N, W, M) QT_CASE_T(R, N, W, M, true, true)
+  if (gsl && td) { QT_BATCH_GSL(QT_CASE_GTD) }
+  else if (gsl) { QT_BATCH_GSL(QT_CASE_G) }
+  else if (td) { QT_BATCH_TD(QT_CASE_TD) }
   else { QT_BATCH(QT_CASE) }
 #undef QT_CASE
 #undef QT_CASE_TD
+#undef QT_CASE_G
+#undef QT_CASE_GTD
 #undef QT_CASE_T
   return hipErrorInvalidValue;
 }

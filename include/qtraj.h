@@ -80,7 +80,9 @@ typedef struct {
   double dpLimit, overshootTol;   /* mcwf::Pars, MCWF_Trajectory.h:28-47 */
   double dtInit;             /* <= 0: 1/(10*highest_frequency), Trajectory.h:126 */
   int precision;             /* QT_FP64 | QT_FP32_DERIV */
-  int controller;            /* QT_CONTROLLER_ODEINT (only) */
+  int controller;            /* QT_CONTROLLER_ODEINT (ODE.h:189-223) or QT_CONTROLLER_GSL (ODE_GSL.h:44-84:
+                                every step accepted, gsl_odeiv2_control_hadjust; ensembles of the
+                                one-wavefront engine, dimensions up to 256) */
   unsigned long long seed;   /* counter-RNG key word 0 */
   unsigned long long traj0;  /* RNG key word 1 of trajectory 0 (trajectory k uses traj0+k) */
   int jump_log_cap;          /* jump records kept per trajectory */

@@ -4,7 +4,7 @@ import pytest
 
 import cppqed_amd as Q
 import oracle_lib as O
-from sysdefs import JC60, PLM20, PTLA, coherent, ground
+from sysdefs import JC60, P1M1, PLM20, PTLA, coherent, ground, p1m1_initial
 
 pytestmark = pytest.mark.gpu
 
@@ -152,3 +152,42 @@ def test_graph_replay_matches_direct_launches():
     a.reset()
     np.testing.assert_array_equal(a.advance(iv)[:, 1:], first[:, 1:])
     assert a.set_graph(None) == (3, 6)
+
+
+@pytest.mark.parametrize("name,spec,psi0,T,ntraj", [
+    ("jc60", JC60, ground(60), 2.0, 32),
+    ("plm20", PLM20, coherent(0.5, 20), 2.0, 24),
+    ("ptla", PTLA, ground(2), 1.0, 24),
+    # interaction picture: the Exact propagation after each accepted step
+    ("p1m1_ip", P1M1, p1m1_initial(), 0.5, 16),
+    ("plm20_ip", PLM20.replace("kappa", "picture=IP kappa"), coherent(0.5, 20), 2.0, 16),
+])
+def test_gsl_controller_trajectories_match_oracle(name, spec, psi0, T, ntraj):
+    """ODE_EngineGSL (ControlledErrorStepperGSL, ODE_GSL.h:44-84): every step accepted, the next dt
+    from gsl_odeiv2_control_hadjust, seven derivative evaluations per step (dydt_out)."""
+    print(f"[gpu] gsl controller parity {name} ntraj={ntraj}", flush=True)
+    sq, so = Q.System(spec), O.System(spec)
+    seed = 4711
+    Dt = 0.1 if name != "ptla" else 0.05
+    iv = np.full(int(round(T / Dt)), Dt)
+    ens = Q.Ensemble(sq, ntraj, psi0, Q.mcwf_pars(seed=seed, jump_log_cap=256, controller="gsl"))
+    ens.advance(iv)
+    per, tot = ens.counters()
+    assert tot[2] == 0                       # no rejected tries
+    assert tot[3] == 7 * tot[4]              # k1, five stages, dydt_out
+    for k in range(ntraj):
+        res_o = _oracle_traj(so, psi0, k, seed, iv, controller="gsl")
+        psi_g, _ = ens.state(k)
+        jt, jch, nj = ens.jumplog(k)
+        _assert_traj_parity(res_o, psi_g, jt, jch, nj, per[k])
+        assert per[k][3] == res_o["counters"][3]
+
+
+def test_gsl_controller_is_refused_where_not_built():
+    sq = Q.System(JC60)
+    with pytest.raises(Q.QtError):
+        Q.mcwf_plain_run(sq, ground(60), np.full(2, 0.1), Q.mcwf_pars(controller="gsl"))
+    big = Q.System("\n".join([f"free mode cutoff=8 delta=-1 kappa=1 eta={1 if i == 0 else 0}" for i in range(3)]
+                             + ["ia bs 0 1 g=0.5", "ia bs 1 2 g=0.5", "system composite"]))
+    with pytest.raises(Q.QtError):
+        Q.Ensemble(big, 4, ground(big.dim), Q.mcwf_pars(controller="gsl"))
