@@ -714,17 +714,18 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     long long derivs = 1, tries = 0, failed = 0;
     if constexpr (gsl) {
       // ControlledErrorStepperGSL::try_step (ODE_GSL.h:44-84): every step is accepted, then
-      // gsl_odeiv2_control_hadjust sets the next dt from the error ratio (RKCK error order 4)
+      // gsl_odeiv2_control_hadjust sets the next dt from the error ratio with the order GSL's rkck
+      // reports, 5: dt *= max(0.9 rmax^(-1/5), 0.2) above 1.1, clamp(0.9 rmax^(-1/6), 1, 5) below 0.5
       ++tries;
       const double rmax = eng.template tryStep<true>(x, k1, dtTry, epsAbs, epsRel, xn, derivs, t, t0);
       const double hOld = dtTry;
       t += dtTry;
       if (rmax > 1.1) {
-        const double r = 0.9 * inv_root<4>(rmax);
+        const double r = 0.9 * inv_root<5>(rmax);
         dtTry = (r < 0.2 ? 0.2 : r) * hOld;
       } else if (rmax < 0.5) {
-        // 0.9 rmax^(-1/5) exceeds the cap 5 below rmax = 0.18^5
-        const double r = rmax < 1.8895680000000003e-4 ? 5.0 : 0.9 * inv_root<5>(rmax);
+        // 0.9 rmax^(-1/6) exceeds the cap 5 below rmax = 0.18^6
+        const double r = rmax < 3.4012224e-5 ? 5.0 : 0.9 * inv_root<6>(rmax);
         dtTry = (r > 5.0 ? 5.0 : r < 1.0 ? 1.0 : r) * hOld;
       }
     }

@@ -15,6 +15,9 @@
 //                                PumpedTwoLevelAtom.cc}, interactions/JaynesCummings_.h:54-66
 //   RKCK54 + odeint controller . CPPQEDutils/ODE.h:189-262 (Boost.Odeint controlled_runge_kutta,
 //                                runge_kutta_cash_karp54, default_error_checker, default_step_adjuster)
+//   GSL controller ............. CPPQEDutils/ODE_GSL.h:44-84 (GSL rkck, control_standard, hadjust);
+//                                pinned only by the reference's GSL tolerance tests
+//   Simulated trajectory ....... CPPQEDutils/Simulated_.h:26-79 (the stepper tests' classical ODEs)
 //   MCWF step .................. CPPQEDcore/quantumtrajectory/MCWF_Trajectory.h:103-118, 250-374
 //   ensemble ................... MCWF_Trajectory.h:213-234, 406-420; CPPQEDutils/StochasticTrajectory.h:220
 //   Master step ................ CPPQEDcore/quantumtrajectory/Master.h:152-208; CPPQEDutils/CMatrix.cc:14-30
@@ -33,6 +36,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -1028,7 +1032,9 @@ struct Ode {
   // GSL rkck + control_standard(eps_abs=user epsRel, eps_rel=user epsAbs, a_y=1, a_dydt=1)
   // + hadjust (ODE_GSL.h:44-84): the step is always accepted; hadjust then sees the new
   // state and the derivative at the new point (dydt_out, here in k1) as 2d real components
-  // (BlitzArray.h:141-163), with the RKCK error order 4.
+  // (BlitzArray.h:141-163).  hadjust's order is the stepper's reported order, 5 for GSL's rkck:
+  // the reference's GSL tolerance tests (EvolvedTestGSL.cc, HarmonicOscillatorComplexGSL.cc) pass
+  // with it and fail with 4.
   bool gslControl(double& t, double& dt, CV& x) {
     size_t n = x.size();
     double rmax = 2.2250738585072014e-308;  // DBL_MIN
@@ -1041,14 +1047,15 @@ struct Ode {
       }
     }
     const double h_old = dt, S = 0.9;
+    constexpr unsigned kOrd = 5;
     t += dt;
     x.swap(xn);
     if (rmax > 1.1) {
-      double r = S / std::pow(rmax, 1.0 / 4);
+      double r = S / std::pow(rmax, 1.0 / kOrd);
       if (r < 0.2) r = 0.2;
       dt = r * h_old;
     } else if (rmax < 0.5) {
-      double r = S / std::pow(rmax, 1.0 / (4 + 1.0));
+      double r = S / std::pow(rmax, 1.0 / (kOrd + 1.0));
       if (r > 5.0) r = 5.0;
       if (r < 1.0) r = 1.0;
       dt = r * h_old;
@@ -1500,6 +1507,71 @@ long orc_master_run_autostop(void* sp, const double* rho0, const orc_params* p, 
 void* orc_autostop_create(double eps, double epsAbs, unsigned rep) { return new Autostop(eps, epsAbs, rep); }
 void orc_autostop_destroy(void* h) { delete (Autostop*)h; }
 int orc_autostop_feed(void* h, const double* v, int n) { return (*(Autostop*)h)(v, n) ? 1 : 0; }
+
+// A classical ODE on the same engine: the reference's stepper tests drive
+// Simulated<CArray<1>> (CPPQEDutils/Simulated_.h:26-79; Simulated::step = ode_.step) through
+// run() in Dt mode or advanceTo (Trajectory.h:44-69).  which = 0: the Evolved equation of
+// EvolvedTest.cc / EvolvedTestGSL.cc, y'' = y'^2 + Z y' + Z^2 exp(2 Z t) (y - y^2), Z = -1+10i,
+// y(0) = e, y'(0) = Z e (solution exp(exp(Z t))); which = 1: the driven damped oscillator of
+// HarmonicOscillatorComplex.cc, y'' = exp(i omega t) - 2 gamma y' - y, omega = 5, gamma = 0.1,
+// y(0) = 10+5i, y'(0) = -1+i.  dtInit and the user tolerances as in those tests.
+// times == nullptr: run up to T (Trajectory.h:462-610) in dc mode when dc > 0 (a stream point
+// every dc ODE steps: run(S, 70., 1, ...) of HarmonicOscillatorComplex.cc passes an integer
+// stream frequency) or else in Dt mode, one row (t, dtDid, Re y, Im y, Re y', Im y') per stream
+// point; otherwise advanceTo(times[i]) for i < n (HarmonicOscillatorRandomTimes.cc:43-50), one
+// row after each.  stats: derivative calls, steps, failed tries, tries.
+long orc_simulated_run(int which, int controller, double dtInit, double epsRel, double epsAbs, double T, double Dt,
+                       int dc, const double* times, long n, double* rows, long cap, long long* stats) {
+  try {
+    const cd Z(-1., 10.), I(0., 1.);
+    const double omega = 5., gamma = 0.1;
+    CV y(2);
+    if (which == 0) { y[0] = std::exp(1.); y[1] = Z * std::exp(1.); }
+    else if (which == 1) { y[0] = cd(10., 5.); y[1] = cd(-1., 1.); }
+    else throw std::invalid_argument("unknown test equation");
+    auto f = [&](const CV& x, CV& dx, double tau) {
+      dx[0] = x[1];
+      if (which == 0) dx[1] = x[1] * x[1] + Z * x[1] + Z * Z * std::exp(2. * Z * tau) * (x[0] - x[0] * x[0]);
+      else dx[1] = std::exp(I * omega * tau) - 2. * gamma * x[1] - x[0];
+    };
+    Ode ode(dtInit, epsRel, epsAbs);
+    ode.controller = (Controller)controller;
+    double t = 0.;
+    long nrow = 0;
+    auto row = [&]() {
+      if (nrow < cap) {
+        double* r = rows + 6 * nrow;
+        r[0] = t; r[1] = ode.dtDid; r[2] = y[0].real(); r[3] = y[0].imag(); r[4] = y[1].real(); r[5] = y[1].imag();
+      }
+      ++nrow;
+    };
+    auto advance = [&](double deltaT) { double endTime = t + deltaT; while (double dt = endTime - t) ode.step(dt, f, t, y); };
+    if (!times && dc > 0) {
+      bool streamed = false;
+      for (long count = 0; t < T; ++count) {
+        if (count) { ode.step(T - t, f, t, y); streamed = false; }
+        if (!count || !(count % dc)) { row(); streamed = true; }
+      }
+      if (!streamed) row();
+    } else if (!times) {
+      for (long count = 0; t < T; ++count) {
+        if (count) advance(std::min(Dt, T - t));
+        row();
+      }
+    } else {
+      for (long i = 0; i < n; ++i) { advance(times[i] - t); row(); }
+    }
+    if (stats) { stats[0] = ode.st.nDerivs; stats[1] = ode.st.nSteps; stats[2] = ode.st.nFailed; stats[3] = ode.st.nTries; }
+    return nrow;
+  } catch (std::exception& e) { g_last_error = e.what(); return -1; }
+}
+
+// std::uniform_real_distribution<double>(a, b) over std::mt19937(seed), the sample times of
+// HarmonicOscillatorRandomTimes.cc:40-45 (libstdc++'s generate_canonical)
+void orc_mt19937_uniform(unsigned seed, long n, double a, double b, double* out) {
+  std::mt19937 re{seed};
+  for (long i = 0; i < n; ++i) out[i] = std::uniform_real_distribution<double>(a, b)(re);
+}
 
 // ---------------------------------------------------------------------------
 // Continuing from a saved state (trajectory::run with a state file, Trajectory.h:484-500:

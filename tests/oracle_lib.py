@@ -80,6 +80,10 @@ def lib():
         l.orc_autostop_create.argtypes = [C.c_double, C.c_double, C.c_uint]
         l.orc_autostop_destroy.argtypes = [C.c_void_p]
         l.orc_autostop_feed.argtypes = [C.c_void_p, dp, C.c_int]
+        l.orc_simulated_run.restype = C.c_long
+        l.orc_simulated_run.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_void_p, C.c_long, dp, C.c_long, C.c_void_p]
+        l.orc_mt19937_uniform.argtypes = [C.c_uint, C.c_long, C.c_double, C.c_double, dp]
         _lib = l
     return _lib
 
@@ -280,3 +284,32 @@ def master_run_autostop(system, rho0, p, r, eps, eps_abs, rep, cap=100000):
     if n < 0:
         raise RuntimeError(lib().orc_last_error().decode())
     return rows[:min(n, cap)], rho
+
+
+SIMULATED = {"evolved": 0, "oscillator": 1}
+
+
+def simulated_run(which, controller="odeint", dtInit=0.02, epsRel=1e-6, epsAbs=1e-18, T=1.0, Dt=0.1, dc=0,
+                  times=None, cap=200000):
+    """A classical test equation on the oracle's ODE engine (the reference's Simulated trajectory):
+    dc-mode rows (dc > 0: every dc ODE steps), Dt-mode rows, or one row after advancing to each of
+    ``times``.  Columns: t, dtDid, y, y' (complex).  Returns (rows, stats = [derivative calls, steps,
+    failed tries, tries])."""
+    rows = np.zeros((cap, 6))
+    st = np.zeros(4, np.int64)
+    tp = None
+    if times is not None:
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        tp = times.ctypes.data
+    n = lib().orc_simulated_run(SIMULATED[which], 1 if controller == "gsl" else 0, dtInit, epsRel, epsAbs, T, Dt, dc,
+                                tp, 0 if times is None else len(times), rows, cap, st.ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    r = rows[:min(n, cap)]
+    return np.column_stack([r[:, 0], r[:, 1], r[:, 2] + 1j * r[:, 3], r[:, 4] + 1j * r[:, 5]]), st
+
+
+def mt19937_uniform(seed, n, a, b):
+    out = np.zeros(n)
+    lib().orc_mt19937_uniform(seed, n, a, b, out)
+    return out
