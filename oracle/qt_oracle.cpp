@@ -1566,6 +1566,22 @@ long orc_simulated_run(int which, int controller, double dtInit, double epsRel, 
   } catch (std::exception& e) { g_last_error = e.what(); return -1; }
 }
 
+// Slice iteration as the element operators see it (forSlices, SliceIterator.h:277-354): for an
+// array of extents dims[rank] and retained axes V[nV], each slice's offset in iteration order
+// (at most cap) and the retained axes' strides in V's order.  Returns the number of slices.
+long orc_slices(const int* dims, int rank, const int* V, int nV, long* offsets, long cap, long* vstrides) {
+  try {
+    orc::Layout L(std::vector<int>(dims, dims + rank));
+    long n = 0;
+    orc::forSlices(L, std::vector<int>(V, V + nV), [&](long off, const std::vector<long>& vs) {
+      if (n < cap) offsets[n] = off;
+      if (n == 0) for (int k = 0; k < nV; ++k) vstrides[k] = vs[k];
+      ++n;
+    });
+    return n;
+  } catch (std::exception& e) { g_last_error = e.what(); return -1; }
+}
+
 // std::uniform_real_distribution<double>(a, b) over std::mt19937(seed), the sample times of
 // HarmonicOscillatorRandomTimes.cc:40-45 (libstdc++'s generate_canonical)
 void orc_mt19937_uniform(unsigned seed, long n, double a, double b, double* out) {

@@ -84,6 +84,8 @@ def lib():
         l.orc_simulated_run.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int, C.c_void_p, C.c_long, dp, C.c_long, C.c_void_p]
         l.orc_mt19937_uniform.argtypes = [C.c_uint, C.c_long, C.c_double, C.c_double, dp]
+        l.orc_slices.restype = C.c_long
+        l.orc_slices.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_long, C.c_void_p]
         _lib = l
     return _lib
 
@@ -313,3 +315,16 @@ def mt19937_uniform(seed, n, a, b):
     out = np.zeros(n)
     lib().orc_mt19937_uniform(seed, n, a, b, out)
     return out
+
+
+def slices(dims, V, cap=1 << 21):
+    """The oracle's slice iteration: (offsets of the slices in iteration order, strides of the
+    retained axes in V's order)."""
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    v = np.ascontiguousarray(V, dtype=np.int32)
+    off = np.zeros(cap, np.int64)
+    vs = np.zeros(max(len(V), 1), np.int64)
+    n = lib().orc_slices(d.ctypes.data, len(d), v.ctypes.data, len(v), off.ctypes.data, cap, vs.ctypes.data)
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return off[:min(n, cap)], vs[:len(V)]
