@@ -44,6 +44,27 @@ def valu_roofline(f32_flops, f64_flops, seconds):
     return achieved, peak
 
 
+def copy_calibration():
+    """A plain device-to-device copy of known size (3 x 256 MiB hipMemcpy): rocprofv3's FETCH_SIZE
+    and WRITE_SIZE of these dispatches give the factors that turn the counters into bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copy calibration: 3 device-to-device copies of %d bytes" % n, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +85,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the copy of known size that calibrates FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(trajectories would shard by index range, see DESIGN.md)" % args.gpus, file=sys.stderr)
