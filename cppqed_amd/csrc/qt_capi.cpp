@@ -414,6 +414,7 @@ int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
 
 int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* vals, double* freqs) {
   return guard([&] {
+    if (!sys) throw Fail(QT_EINVAL, "null argument");
     const qt::System& s = *sys->h;
     const qt::Ell* e = nullptr;
     if (which == 0) e = &s.H;
@@ -670,6 +671,7 @@ int qt_mcwf_set_graph(qt_ens* e, int enable, long long* captures, long long* rep
 
 int qt_mcwf_traj_averages(qt_ens* e, double* out) {
   return guard([&] {
+    if (!e || !out) throw Fail(QT_EINVAL, "null argument");
     const qt::System& s = *e->sys->h;
     const int no = (int)s.O.size();
     if (!e->lastIntervals) throw Fail(QT_EINVAL, "no interval advanced yet");
@@ -682,6 +684,7 @@ int qt_mcwf_traj_averages(qt_ens* e, double* out) {
 
 int qt_mcwf_state(qt_ens* e, int i, double* psi, double* t) {
   return guard([&] {
+    if (!e) throw Fail(QT_EINVAL, "null argument");
     if (i < 0 || i >= e->n) throw Fail(QT_EINVAL, "trajectory index out of range");
     const long d = e->sys->h->dim;
     if (psi) hipCheck(hipMemcpy(psi, e->psi.p + (size_t)i * d, d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
@@ -691,6 +694,7 @@ int qt_mcwf_state(qt_ens* e, int i, double* psi, double* t) {
 
 int qt_mcwf_jumplog(qt_ens* e, int i, double* t, int* ch, int cap, int* n) {
   return guard([&] {
+    if (!e) throw Fail(QT_EINVAL, "null argument");
     if (i < 0 || i >= e->n) throw Fail(QT_EINVAL, "trajectory index out of range");
     long long c[qt::kCounters];
     hipCheck(hipMemcpy(c, e->cnt.p + (size_t)i * qt::kCounters, sizeof(c), hipMemcpyDeviceToHost), "D2H");
@@ -706,6 +710,7 @@ int qt_mcwf_jumplog(qt_ens* e, int i, double* t, int* ch, int cap, int* n) {
 
 int qt_mcwf_counters(qt_ens* e, long long* per, long long* totals) {
   return guard([&] {
+    if (!e) throw Fail(QT_EINVAL, "null argument");
     std::vector<long long> c((size_t)e->n * qt::kCounters);
     hipCheck(hipMemcpy(c.data(), e->cnt.p, c.size() * sizeof(long long), hipMemcpyDeviceToHost), "D2H");
     if (per) std::memcpy(per, c.data(), c.size() * sizeof(long long));
@@ -781,6 +786,7 @@ int qt_mcwf_set_states(qt_ens* e, int first, int count, const qt_traj_state* st,
 }
 
 int qt_mcwf_kernel_ms(qt_ens* e, double* total, int* launches) {
+  if (!e) return QT_EINVAL;
   if (total) *total = e->kernelMs;
   if (launches) *launches = e->launches;
   return QT_OK;
@@ -1286,6 +1292,7 @@ int qt_master_status(qt_master* m, int* done, double* tss, double* t, long long*
 }
 
 int qt_master_kernel_ms(qt_master* m, double* total, int* launches) {
+  if (!m) return QT_EINVAL;
   if (total) *total = m->kernelMs;
   if (launches) *launches = m->launches;
   return QT_OK;
@@ -1596,6 +1603,7 @@ int qt_server_result(qt_server* s, long long id, double* rows, int cap_rows, int
 
 int qt_server_release(qt_server* s, long long id) {
   return guard([&] {
+    if (!s) throw Fail(QT_EINVAL, "null argument");
     auto it = s->all.find(id);
     if (it == s->all.end()) throw Fail(QT_EINVAL, "unknown request id");
     if (it->second->tDone == 0.) throw Fail(QT_EINVAL, "request not finished");

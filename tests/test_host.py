@@ -174,6 +174,30 @@ def test_header_symbols_exported():
     assert len(names) >= 15
 
 
+def test_null_handles_are_refused():
+    """A null handle gives QT_EINVAL and a message, never a crash (no GPU call is reached)."""
+    lib = ctypes.CDLL(Q.LIBPATH)
+    lib.qt_last_error.restype = ctypes.c_char_p
+    d = ctypes.c_double()
+    i = ctypes.c_int()
+    ll = (ctypes.c_longlong * 8)()
+    calls = [
+        ("qt_sys_info", (None, None)),
+        ("qt_sys_table", (None, 0, None, None, None, None)),
+        ("qt_mcwf_state", (None, 0, None, None)),
+        ("qt_mcwf_jumplog", (None, 0, None, None, 0, ctypes.byref(i))),
+        ("qt_mcwf_counters", (None, None, ll)),
+        ("qt_mcwf_traj_averages", (None, None)),
+        ("qt_mcwf_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
+        ("qt_master_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
+        ("qt_batch_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
+        ("qt_server_release", (None, ctypes.c_longlong(1))),
+        ("qt_server_stats", (None, None)),
+    ]
+    for name, args in calls:
+        assert getattr(lib, name)(*args) == 1, name  # QT_EINVAL
+
+
 def test_run_intervals_mirror_trajectory_run():
     """Dt mode: advance(min(Dt, T - t)) while t < T -- 0.1 summed ten times falls short of 1
     and leaves a 1.1e-16 interval, as the reference's golden files show (Trajectory.h:562-567)."""
