@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--ntraj", type=int, default=16384)
     ap.add_argument("--T", type=float, default=20.0)
     ap.add_argument("--plain-traj", type=int, default=4)
-    ap.add_argument("--cpu-traj", type=int, default=512)
+    ap.add_argument("--cpu-traj", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-master", action="store_true")
