@@ -171,7 +171,7 @@ struct qt_sys {
     if (h->H.timeDependent() && nfreq > qt::kMaxFreqs)
       throw Fail(QT_ENOTSUP, "interaction picture with more than 32 distinct frequencies");
   }
-  DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx;
+  DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx, hoff;
   DevBuf<double2> hval, jval, oval, freq, exz;
   int nfreq = 0;          // distinct interaction-picture frequencies of H (0: Schroedinger picture)
   DevBuf<float2> hval32;
@@ -191,6 +191,40 @@ struct qt_sys {
     };
     std::vector<int> c; std::vector<double2> v;
     colmajor(s.H, c, v);
+    // Constant-offset form: Kronecker-structured operators shift each axis by fixed amounts, so
+    // the entries' column offsets (col - row) often take no more distinct values than a row has
+    // entries.  Then the table is re-slotted by offset (slots in order of first appearance, rows
+    // in order) and the gathers compute columns instead of loading them.
+    std::vector<int> offs;
+    if (!s.H.timeDependent()) {
+      const int w0 = s.H.width;
+      for (int r = 0; r < d && (int)offs.size() <= w0; ++r)
+        for (int k = 0; k < w0; ++k) {
+          if (s.H.vals[(size_t)r * w0 + k] == qt::cd(0.)) continue;
+          const int o = s.H.cols[(size_t)r * w0 + k] - r;
+          if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
+        }
+      if ((int)offs.size() <= w0) {
+        offs.resize(w0, 0);
+        for (int k = 0; k < w0; ++k)
+          for (int r = 0; r < d; ++r) {
+            c[(size_t)k * d + r] = std::min(std::max(r + offs[k], 0), d - 1);
+            v[(size_t)k * d + r] = make_double2(0., 0.);
+          }
+        for (int r = 0; r < d; ++r)
+          for (int k = 0; k < w0; ++k) {
+            const qt::cd z = s.H.vals[(size_t)r * w0 + k];
+            if (z == qt::cd(0.)) continue;
+            const int col = s.H.cols[(size_t)r * w0 + k];
+            const int slot = (int)(std::find(offs.begin(), offs.end(), col - r) - offs.begin());
+            c[(size_t)slot * d + r] = col;
+            v[(size_t)slot * d + r] = make_double2(z.real(), z.imag());
+          }
+        hoff.upload(offs);
+      } else {
+        offs.clear();
+      }
+    }
     hcol.upload(c); hval.upload(v);
     std::vector<float2> v32;
     for (auto& z : v) v32.push_back(make_float2((float)z.x, (float)z.y));
@@ -260,6 +294,7 @@ struct qt_sys {
     dev.freq = freq.p; dev.hfidx = hfidx.p; dev.exz = s.hasEx ? exz.p : nullptr;
     dev.dim = d; dev.wh = s.H.width;
     dev.hcol = hcol.p; dev.hval = hval.p; dev.hval32 = hval32.p;
+    dev.hoff = offs.empty() ? nullptr : hoff.p;
     dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
     dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p; dev.owid = owid.p;
     uploaded = true;

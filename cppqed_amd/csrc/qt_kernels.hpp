@@ -42,6 +42,10 @@ struct SysDev {
   const double2* freq;     // [nfreq], freq[0] = 0
   const int* hfidx;        // [wh][dim]
   const double2* exz;      // [dim] (null without an Exact part)
+  // constant-offset form of the H table (Schroedinger picture): when non-null, slot w of every
+  // row holds the entry at column row + hoff[w] (zero where the row has none, its hcol clamped
+  // into range), so a gather may compute the column instead of loading it
+  const int* hoff;         // [wh]
 };
 
 // Device view of an MCWF ensemble and the per-launch work description.

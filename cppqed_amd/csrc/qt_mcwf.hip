@@ -55,6 +55,7 @@ struct Engine {
   // scalar copies of the H table's base addresses, width and dimension
   const int* hcolU;
   const T* hvalU;
+  const int* hoffU;  // constant-offset form of the table (SysDev::hoff), or null
   int whU, dimU;
   // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
   // launch's error flag (set when a wait times out)
@@ -80,6 +81,7 @@ struct Engine {
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
       : S(s), xs(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
+    hoffU = uniform(s.hoff);
     if constexpr (F32) hvalU = uniform(s.hval32);
     else hvalU = uniform(s.hval);
     whU = uniform(s.wh);
@@ -188,7 +190,14 @@ struct Engine {
         for (int w = 0; w < WH; ++w) acc = cfma(hval[r][w], xs[hcol[r][w]], acc);
       } else {
         const int row = r * G + lane;
-        if (row < dimU) {
+        if (row < dimU && hoffU) {
+          // constant offsets: the column is computed, only the coefficient is loaded
+#pragma unroll 4
+          for (int w = 0; w < whU; ++w) {
+            const int c = min(max(row + gbl(hoffU)[w], 0), dimU - 1);
+            acc = cfma(coef(w, row), xs[c], acc);
+          }
+        } else if (row < dimU) {
 #pragma unroll 4
           for (int w = 0; w < whU; ++w) acc = cfma(coef(w, row), xs[col(w, row)], acc);
         }
