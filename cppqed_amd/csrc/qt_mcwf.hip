@@ -46,7 +46,10 @@ struct Engine {
   using T = typename CT<F32>::T;
 
   const SysDev& S;
-  T* xs;            // LDS stage vector [ROWS*G] in derivative precision
+  mutable T* xs;    // LDS stage vector [ROWS*G] in derivative precision (the one being read)
+  T* xsA;           // its storage
+  T* xsB = nullptr; // a second stage buffer (multi-wavefront teams, when LDS allows): stages
+                    // alternate between the two and a hand-over needs one barrier, not two
   double2* ps;      // LDS FP64 state copy [ROWS*G] (rates, jumps, observables)
   double* red;      // LDS reduction scratch
   const int lane;
@@ -79,7 +82,7 @@ struct Engine {
   // lane_ = index of this thread within its team of NW wavefronts; ph_ = LDS phasor table
   // [kMaxFreqs] (needed for interaction-picture systems only)
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
-      : S(s), xs(xs_), ps(ps_), red(red_), lane(lane_) {
+      : S(s), xs(xs_), xsA(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
     hoffU = uniform(s.hoff);
     if constexpr (F32) hvalU = uniform(s.hval32);
@@ -249,6 +252,16 @@ struct Engine {
   // xs and ps may share storage: both hand-overs start with a barrier, so the readers of the
   // other buffer are done before it is overwritten
   __device__ __forceinline__ void stage(const double2 v[ROWS]) const {
+    if (xsB) {
+      // the buffer nobody reads now: its last readers (two hand-overs ago) have all passed the
+      // barrier of the previous hand-over
+      T* nb = xs == xsA ? xsB : xsA;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (live(r)) nb[r * G + lane] = from_d<T>(v[r]);
+      sync();
+      xs = nb;
+      return;
+    }
     sync();  // previous readers of xs / ps are done
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) if (live(r)) xs[r * G + lane] = from_d<T>(v[r]);
@@ -426,15 +439,25 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
   using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd; using B::gsum; using B::gmax;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
 
-  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_, T* ph_ = nullptr)
-      : B(s, xs_, X_, red_, lane_, ph_), X(X_) {}
+  // xs2: optional second stage buffer (see Engine::xsB)
+  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_, T* ph_ = nullptr,
+                       T* xs2 = nullptr)
+      : B(s, xs_, X_, red_, lane_, ph_), X(X_) { B::xsB = xs2; }
 
   // k = (H/i) X at picture time tau
   __device__ __forceinline__ void derivativeX(T k[ROWS], double tau = 0.) const {
-    this->sync();
+    if (B::xsB) {
+      T* nb = xs == B::xsA ? B::xsB : B::xsA;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) if (B::live(r)) xs[r * G + lane] = from_d<T>(X[r * G + lane]);
-    this->sync();
+      for (int r = 0; r < ROWS; ++r) if (B::live(r)) nb[r * G + lane] = from_d<T>(X[r * G + lane]);
+      this->sync();
+      xs = nb;
+    } else {
+      this->sync();
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) if (B::live(r)) xs[r * G + lane] = from_d<T>(X[r * G + lane]);
+      this->sync();
+    }
     this->setTime(tau);
     B::gather(k);
   }
@@ -937,7 +960,10 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // multi-wavefront team: psi in LDS
     __shared__ double2 Xs[ROWS * G];
     __shared__ T xs[ROWS * G];
-    LdsEngine<ROWS, NW, WH, F32, false, TD> eng(S, xs, Xs, red, lane, ph);
+    // a second stage buffer when it fits beside the state (saves a barrier per derivative)
+    constexpr bool kTwo = (size_t)ROWS * G * (sizeof(double2) + 2 * sizeof(T)) <= 150 * 1024;
+    __shared__ T xs2[kTwo ? ROWS * G : 1];
+    LdsEngine<ROWS, NW, WH, F32, false, TD> eng(S, xs, Xs, red, lane, ph, kTwo ? xs2 : nullptr);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
