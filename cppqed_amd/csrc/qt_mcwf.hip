@@ -84,7 +84,8 @@ struct Engine {
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
       : S(s), xs(xs_), xsA(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
-    hoffU = uniform(s.hoff);
+    if constexpr (kRegs) hoffU = nullptr;  // the register-cached columns are used instead
+    else hoffU = uniform(s.hoff);
     if constexpr (F32) hvalU = uniform(s.hval32);
     else hvalU = uniform(s.hval);
     whU = uniform(s.wh);
@@ -252,7 +253,7 @@ struct Engine {
   // xs and ps may share storage: both hand-overs start with a barrier, so the readers of the
   // other buffer are done before it is overwritten
   __device__ __forceinline__ void stage(const double2 v[ROWS]) const {
-    if (xsB) {
+    if (NW > 1 && xsB) {
       // the buffer nobody reads now: its last readers (two hand-overs ago) have all passed the
       // barrier of the previous hand-over
       T* nb = xs == xsA ? xsB : xsA;
