@@ -32,8 +32,11 @@ namespace qt {
 // SUB = the team is NW wavefronts of a larger work group: its hand-overs use a counter
 // barrier in LDS instead of the work-group barrier.  TD = interaction-picture system (H entries
 // rotate with the picture, psi gets the Exact U after each ODE step); the Schroedinger-picture
-// instantiations carry none of that code.
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false>
+// instantiations carry none of that code.  OPT: optional code paths, compiled only where they
+// pay (each adds registers to a kernel): kOffsets = gathers use the constant-offset form of the
+// H table when the system has one; kTwoStage = two stage buffers (multi-wavefront teams).
+enum { kOffsets = 1, kTwoStage = 2 };
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false, int OPT = 0>
 struct Engine {
   static constexpr int kRows = ROWS, kNW = NW;
   static constexpr bool kTD = TD;
@@ -84,7 +87,7 @@ struct Engine {
   __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
       : S(s), xs(xs_), xsA(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
-    if constexpr (kRegs) hoffU = nullptr;  // the register-cached columns are used instead
+    if constexpr (kRegs || !(OPT & kOffsets)) hoffU = nullptr;  // (register-cached columns otherwise)
     else hoffU = uniform(s.hoff);
     if constexpr (F32) hvalU = uniform(s.hval32);
     else hvalU = uniform(s.hval);
@@ -194,7 +197,7 @@ struct Engine {
         for (int w = 0; w < WH; ++w) acc = cfma(hval[r][w], xs[hcol[r][w]], acc);
       } else {
         const int row = r * G + lane;
-        if (row < dimU && hoffU) {
+        if ((OPT & kOffsets) && row < dimU && hoffU) {
           // constant offsets: the column is computed, only the coefficient is loaded
 #pragma unroll 4
           for (int w = 0; w < whU; ++w) {
@@ -253,7 +256,7 @@ struct Engine {
   // xs and ps may share storage: both hand-overs start with a barrier, so the readers of the
   // other buffer are done before it is overwritten
   __device__ __forceinline__ void stage(const double2 v[ROWS]) const {
-    if (NW > 1 && xsB) {
+    if ((OPT & kTwoStage) && NW > 1 && xsB) {
       // the buffer nobody reads now: its last readers (two hand-overs ago) have all passed the
       // barrier of the previous hand-over
       T* nb = xs == xsA ? xsB : xsA;
@@ -433,9 +436,9 @@ struct Engine {
 // 16-wavefront team) free of register spills.
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false>
-struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
-  using B = Engine<ROWS, NW, WH, F32, SUB, TD>;
+template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false, int OPT = 0>
+struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
+  using B = Engine<ROWS, NW, WH, F32, SUB, TD, OPT>;
   using T = typename B::T;
   using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd; using B::gsum; using B::gmax;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
@@ -447,7 +450,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD> {
 
   // k = (H/i) X at picture time tau
   __device__ __forceinline__ void derivativeX(T k[ROWS], double tau = 0.) const {
-    if (B::xsB) {
+    if ((OPT & kTwoStage) && B::xsB) {
       T* nb = xs == B::xsA ? B::xsB : B::xsA;
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) if (B::live(r)) nb[r * G + lane] = from_d<T>(X[r * G + lane]);
@@ -935,7 +938,7 @@ constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2)
 template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD, bool GSL = false>
 __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
   static_assert(!GSL || NW == 1, "the GSL controller runs in the one-wavefront engine");
-  using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
+  using Eng = Engine<ROWS, NW, WH, F32, false, TD, kOffsets>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
   constexpr bool kPcLds = pc_in_lds<ROWS, NW>();
@@ -964,7 +967,8 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // a second stage buffer when it fits beside the state (saves a barrier per derivative)
     constexpr bool kTwo = (size_t)ROWS * G * (sizeof(double2) + 2 * sizeof(T)) <= 150 * 1024;
     __shared__ T xs2[kTwo ? ROWS * G : 1];
-    LdsEngine<ROWS, NW, WH, F32, false, TD> eng(S, xs, Xs, red, lane, ph, kTwo ? xs2 : nullptr);
+    LdsEngine<ROWS, NW, WH, F32, false, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
+                                                                                 kTwo ? xs2 : nullptr);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
@@ -1003,7 +1007,7 @@ __device__ __noinline__ void run_wave_traj(const SysDev* S, const EnsDev* E, dou
 template <int NWB, int RB, int WHB, int RW, int WHW, bool F32>
 __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
   constexpr int NWQ = NWB / 4;
-  using EngB = LdsEngine<RB, NWB, WHB, F32>;
+  using EngB = LdsEngine<RB, NWB, WHB, F32, false, false, kOffsets>;
   using EngQ = LdsEngine<RB, NWQ, WHB, F32, true>;
   using EngW = Engine<RW, 1, WHW, F32>;
   using TB = typename EngB::T;
