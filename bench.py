@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--master-T", type=int, default=60)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-T", type=float, default=5.0)
+    ap.add_argument("--mixed-layout", default=None,
+                    help="variant[,team,...] of the mixed batch instead of the autotuner's choice")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=8.0)
@@ -263,9 +265,15 @@ def mixed_batch(Q, args):
                 for i, (s, n, (_, _, idx)) in enumerate(zip(systems, counts, MIXED))]
 
     mem = members()
-    b = Q.Batch(mem)
     t0 = time.perf_counter()
-    variant, probe_ms = b.autotune(probe_span=0.3, probe_traj=2048)
+    if args.mixed_layout is None:
+        b = Q.Batch(mem)
+        variant, probe_ms = b.autotune(probe_span=0.3, probe_traj=2048)
+    else:
+        # a fixed layout (e.g. for counter runs, whose serialised timing could sway the autotuner)
+        lay = [int(x) for x in args.mixed_layout.split(",")]
+        b = Q.Batch(mem, variant=lay[0], team=lay[1:] or None)
+        variant, probe_ms = lay[0], []
     tune_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     b.advance(iv)
