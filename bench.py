@@ -94,8 +94,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(trajectories would shard by index range, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported "
+              "(trajectories would shard by index range, one process per GPU; not built here)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import cppqed_amd as Q

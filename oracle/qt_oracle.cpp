@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
-// only as the checker (or the timed CPU baseline), never as the product path.
+// only as the parity reference (or the timed CPU baseline), never as the product path.
 //
 // It restates, in plain FP64 C++ written in the reference's own per-slice
 // style (not in the product's flattened-table style), the following reference

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
-Test infrastructure only: the oracle is the checker, never the product path.
+Test infrastructure only: the oracle is the parity reference, never the product path.
 """
 import ctypes as C
 import os

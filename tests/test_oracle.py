@@ -80,7 +80,7 @@ def test_ptla_golden(mode, kind):
 
 
 def test_golden_fails_with_wrong_stream():
-    """The golden replay is sensitive: the pcg64 stream ordinal 1001+1 (the survey's reading) does not match."""
+    """The golden replay is sensitive: the pcg64 stream ordinal 1001+1 (the other reading of Random.h:153-161) does not match."""
     s = O.System(PTLA)
     p = O.params(epsRel=1e-6, epsAbs=1e-5, rng="pcg64", seed=1001, stream=1002)
     rows = O.mcwf_run(s, ground(2), p, GOLDEN_MODES["dc"])["rows"]
