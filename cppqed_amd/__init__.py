@@ -29,7 +29,7 @@ class SysInfo(C.Structure):
     _fields_ = [("dim", C.c_long), ("rank", C.c_int), ("dims", C.c_int * 16), ("n_jumps", C.c_int),
                 ("n_avr", C.c_int), ("h_width", C.c_int), ("j_width", C.c_int), ("o_width", C.c_int),
                 ("time_dependent", C.c_int), ("highest_frequency", C.c_double), ("h_nnz", C.c_longlong),
-                ("m_terms", C.c_longlong)]
+                ("m_terms", C.c_longlong), ("h_offsets", C.c_int)]
 
 
 class McwfParams(C.Structure):
@@ -254,7 +254,8 @@ class System:
             self.h = None
 
     def table(self, which=0):
-        """(cols[dim, w], vals[dim, w], freqs[dim, w]) of H_nH/i (0), J_m (1+m) or observable (1+n_jumps+q)."""
+        """(cols[dim, w], vals[dim, w], freqs[dim, w]) of H_nH/i (0), J_m (1+m), observable (1+n_jumps+q),
+        or H_nH/i in its constant-offset form (-1)."""
         w = C.c_int()
         check(lib().qt_sys_table(self.h, which, C.byref(w), None, None, None))
         n, w = self.dim, w.value

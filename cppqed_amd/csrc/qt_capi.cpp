@@ -161,6 +161,46 @@ void check_params(const qt_mcwf_params* p) {
 
 }  // namespace
 
+namespace {
+
+// Constant-offset form of H (SysDev::hoff): Kronecker-structured operators shift each axis by
+// fixed amounts, so the entries' column offsets (col - row) often take no more distinct values
+// than a row has entries.  Then slot k of every row holds the entry at column row + offs[k]
+// (slots in order of first appearance, rows in order; zero and a clamped column where the row
+// has none).  cols / vals come out row-major [dim][width] like the ELL table.  False (and
+// nothing written) for tables with frequencies or with more distinct offsets than the width.
+bool offset_form(const qt::Ell& e, int d, std::vector<int>& offs, std::vector<int>& cols, std::vector<qt::cd>& vals) {
+  offs.clear();
+  if (e.timeDependent()) return false;
+  const int w = e.width;
+  for (int r = 0; r < d; ++r)
+    for (int k = 0; k < w; ++k) {
+      if (e.vals[(size_t)r * w + k] == qt::cd(0.)) continue;
+      const int o = e.cols[(size_t)r * w + k] - r;
+      if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+        if ((int)offs.size() == w) { offs.clear(); return false; }
+        offs.push_back(o);
+      }
+    }
+  offs.resize(w, 0);
+  cols.assign((size_t)d * w, 0);
+  vals.assign((size_t)d * w, qt::cd(0.));
+  for (int r = 0; r < d; ++r)
+    for (int k = 0; k < w; ++k) cols[(size_t)r * w + k] = std::min(std::max(r + offs[k], 0), d - 1);
+  for (int r = 0; r < d; ++r)
+    for (int k = 0; k < w; ++k) {
+      const qt::cd z = e.vals[(size_t)r * w + k];
+      if (z == qt::cd(0.)) continue;
+      const int col = e.cols[(size_t)r * w + k];
+      const int slot = (int)(std::find(offs.begin(), offs.end(), col - r) - offs.begin());
+      cols[(size_t)r * w + slot] = col;
+      vals[(size_t)r * w + slot] = z;
+    }
+  return true;
+}
+
+}  // namespace
+
 struct qt_sys {
   std::unique_ptr<qt::System> h;
   bool uploaded = false;
@@ -191,38 +231,20 @@ struct qt_sys {
     };
     std::vector<int> c; std::vector<double2> v;
     colmajor(s.H, c, v);
-    // Constant-offset form: Kronecker-structured operators shift each axis by fixed amounts, so
-    // the entries' column offsets (col - row) often take no more distinct values than a row has
-    // entries.  Then the table is re-slotted by offset (slots in order of first appearance, rows
-    // in order) and the gathers compute columns instead of loading them.
+    // constant-offset form of H when it has one (offset_form): uploaded in its place
     std::vector<int> offs;
-    if (!s.H.timeDependent()) {
-      const int w0 = s.H.width;
-      for (int r = 0; r < d && (int)offs.size() <= w0; ++r)
-        for (int k = 0; k < w0; ++k) {
-          if (s.H.vals[(size_t)r * w0 + k] == qt::cd(0.)) continue;
-          const int o = s.H.cols[(size_t)r * w0 + k] - r;
-          if (std::find(offs.begin(), offs.end(), o) == offs.end()) offs.push_back(o);
-        }
-      if ((int)offs.size() <= w0) {
-        offs.resize(w0, 0);
+    {
+      std::vector<int> oc;
+      std::vector<qt::cd> ov;
+      if (offset_form(s.H, d, offs, oc, ov)) {
+        const int w0 = s.H.width;
         for (int k = 0; k < w0; ++k)
           for (int r = 0; r < d; ++r) {
-            c[(size_t)k * d + r] = std::min(std::max(r + offs[k], 0), d - 1);
-            v[(size_t)k * d + r] = make_double2(0., 0.);
-          }
-        for (int r = 0; r < d; ++r)
-          for (int k = 0; k < w0; ++k) {
-            const qt::cd z = s.H.vals[(size_t)r * w0 + k];
-            if (z == qt::cd(0.)) continue;
-            const int col = s.H.cols[(size_t)r * w0 + k];
-            const int slot = (int)(std::find(offs.begin(), offs.end(), col - r) - offs.begin());
-            c[(size_t)slot * d + r] = col;
-            v[(size_t)slot * d + r] = make_double2(z.real(), z.imag());
+            c[(size_t)k * d + r] = oc[(size_t)r * w0 + k];
+            const qt::cd z = ov[(size_t)r * w0 + k];
+            v[(size_t)k * d + r] = make_double2(z.real(), z.imag());
           }
         hoff.upload(offs);
-      } else {
-        offs.clear();
       }
     }
     hcol.upload(c); hval.upload(v);
@@ -444,6 +466,9 @@ int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
     long long mt = 0;
     for (auto& t : s.hTerms) mt += (long long)t.op.entries.size() * (s.dim / std::max(1, t.op.size()));
     info->m_terms = mt;
+    std::vector<int> offs, oc;
+    std::vector<qt::cd> ov;
+    info->h_offsets = offset_form(s.H, (int)s.dim, offs, oc, ov) ? (int)offs.size() : 0;
   });
 }
 
@@ -452,7 +477,14 @@ int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* va
     if (!sys) throw Fail(QT_EINVAL, "null argument");
     const qt::System& s = *sys->h;
     const qt::Ell* e = nullptr;
-    if (which == 0) e = &s.H;
+    qt::Ell form;  // which == -1: H in its constant-offset form
+    if (which == -1) {
+      std::vector<int> offs;
+      if (!offset_form(s.H, (int)s.dim, offs, form.cols, form.vals))
+        throw Fail(QT_EINVAL, "H has no constant-offset form");
+      form.width = s.H.width;
+      e = &form;
+    } else if (which == 0) e = &s.H;
     else if (which - 1 < (int)s.J.size()) e = &s.J[which - 1];
     else if (which - 1 - (int)s.J.size() < (int)s.O.size()) e = &s.O[which - 1 - s.J.size()];
     else throw Fail(QT_EINVAL, "table index out of range");

@@ -57,12 +57,15 @@ typedef struct {
   long long m_terms;        /* complex multiply-adds of one H_nH/i application counted the reference's
                                way: stored entries of every term's operator times its slice count
                                (Tridiagonal<R>::apply over fullRange, Tridiagonal.tcc:220-263) */
+  int h_offsets;            /* slots of H's constant-offset form (every slot one fixed col - row, the
+                               device gathers then compute columns), 0 when H has none */
 } qt_sys_info_t;
 int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info);
 
 /* Host copies of the lowered tables (testing aid): *width receives the row width;
  * cols[dim*width] row-major, vals/freqs[2*dim*width] (any may be NULL).
- * which: 0 = H_nH/i, 1+m = J_m, 1+n_jumps+q = observable q. */
+ * which: 0 = H_nH/i, 1+m = J_m, 1+n_jumps+q = observable q, -1 = H_nH/i in its
+ * constant-offset form (as the device holds it; QT_EINVAL when h_offsets is 0). */
 int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* vals, double* freqs);
 
 /* dpsidt += (H_nH/i) psi for n state vectors, device pointers, on `stream`.

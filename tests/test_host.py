@@ -174,6 +174,44 @@ def test_header_symbols_exported():
     assert len(names) >= 15
 
 
+def _dense(cols, vals):
+    d = cols.shape[0]
+    m = np.zeros((d, d), complex)
+    np.add.at(m, (np.repeat(np.arange(d), cols.shape[1]), cols.ravel()), vals.ravel())
+    return m
+
+
+@pytest.mark.parametrize("name,n_off", [("pcav1024", 5), ("modes4096", 9), ("plm20", 3), ("jc60", 0),
+                                        ("qubits64", 0), ("jc_ip", 0)])
+def test_constant_offset_form(name, n_off):
+    """The device's re-slotted H (constant column offset per slot) is the same matrix as the ELL
+    table, every slot's entries sit at one offset, and its empty places point at the clamped
+    column; systems with more distinct offsets than the row width, or with picture
+    frequencies, keep the ELL table (h_offsets = 0)."""
+    from sysdefs import MIXED, PLM20
+    specs = {n: sp for n, sp, _ in MIXED}
+    specs.update(plm20=PLM20, jc60=JC60,
+                 jc_ip="free qbit deltaA=0.5 gamma=0.2 eta=0.3 picture=IP\n"
+                       "free mode cutoff=30 delta=-1 kappa=1 eta=1 picture=IP\nia jc 0 1 g=3")
+    s = Q.System(specs[name])
+    assert s.info.h_offsets == n_off
+    if not n_off:
+        with pytest.raises(Q.QtError):
+            s.table(-1)
+        return
+    cols, vals, _ = s.table(0)
+    oc, ov, _ = s.table(-1)
+    assert oc.shape == cols.shape
+    np.testing.assert_array_equal(_dense(oc, ov), _dense(cols, vals))
+    rows = np.arange(s.dim)
+    for k in range(oc.shape[1]):
+        nz = ov[:, k] != 0
+        offs = np.unique(oc[nz, k] - rows[nz])
+        assert len(offs) <= 1, (k, offs)
+        off = offs[0] if len(offs) else 0
+        np.testing.assert_array_equal(oc[:, k], np.clip(rows + off, 0, s.dim - 1))
+
+
 def test_null_handles_are_refused():
     """A null handle gives QT_EINVAL and a message, never a crash (no GPU call is reached)."""
     lib = ctypes.CDLL(Q.LIBPATH)
