@@ -73,3 +73,29 @@ def test_smallest_cases_run():
     for k in range(2):
         p, _ = eb.state(k)
         assert abs(np.linalg.norm(p) - 1) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["pcav1024", "modes4096"])
+def test_large_states_with_offset_gathers_match_oracle(name):
+    """The largest compiled layouts (4 rows x 4 and 16 wavefronts, psi in LDS) with the
+    constant-offset gathers, FP64, each trajectory against the oracle."""
+    from sysdefs import MIXED, basis
+    spec, idx = next((sp, i) for n, sp, i in MIXED if n == name)
+    print(f"[gpu] large state {name} vs oracle", flush=True)
+    s, so = Q.System(spec), O.System(spec)
+    assert s.info.h_offsets > 0
+    psi0 = basis(s.dim, idx)
+    seed, ntraj, iv = 31, 6, np.full(20, 0.1)
+    e = Q.Ensemble(s, ntraj, psi0, Q.mcwf_pars(seed=seed, jump_log_cap=256))
+    e.advance(iv)
+    per, _ = e.counters()
+    for k in range(ntraj):
+        res = O.mcwf_run(so, psi0, O.params(rng="philox", seed=seed, stream=k), O.run_spec("NDt", Dt=0.1, NDt=len(iv)))
+        psi, _ = e.state(k)
+        jt, jch, nj = e.jumplog(k)
+        assert nj == len(res["jump_t"]), (k, nj, len(res["jump_t"]))
+        np.testing.assert_array_equal(jch, res["jump_ch"])
+        np.testing.assert_allclose(jt, res["jump_t"], rtol=1e-10, atol=1e-12)
+        assert np.linalg.norm(psi - res["psi"]) <= 1e-9 * np.linalg.norm(res["psi"]), k
+        for c in (0, 1, 2, 4, 5, 7):
+            assert per[k][c] == res["counters"][c], (k, c, per[k][c], res["counters"][c])
