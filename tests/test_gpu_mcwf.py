@@ -191,3 +191,19 @@ def test_gsl_controller_is_refused_where_not_built():
                              + ["ia bs 0 1 g=0.5", "ia bs 1 2 g=0.5", "system composite"]))
     with pytest.raises(Q.QtError):
         Q.Ensemble(big, 4, ground(big.dim), Q.mcwf_pars(controller="gsl"))
+
+
+@pytest.mark.parametrize("prec", [Q.QT_FP64, Q.QT_FP32_DERIV])
+def test_gsl_controller_pumped_mode_closed_form(prec):
+    """GSL controller, both derivative precisions: PLM-20 from a coherent state stays coherent,
+    <n> = |alpha(t)|^2 with alpha(t) = alpha_ss + (alpha_0 - alpha_ss) exp((-kappa + i Delta) t)."""
+    print(f"[gpu] gsl closed form prec={prec}", flush=True)
+    s = Q.System(PLM20)
+    iv = Q.run_intervals(T=5.0, Dt=0.1)
+    rows = Q.Ensemble(s, 64, coherent(0.5, 20), Q.mcwf_pars(seed=2, precision=prec, controller="gsl")).advance(iv)
+    a0, D, eta, kappa = 0.5, -1.0, 2.0, 1.0
+    ass = eta / (kappa - 1j * D)
+    alpha = ass + (a0 - ass) * np.exp((-kappa + 1j * D) * rows[:, 0])
+    err = np.abs(rows[:, 2] - np.abs(alpha) ** 2).max()
+    print(f"  max |<n> - |alpha|^2| = {err:.2e}", flush=True)
+    assert err < (1e-6 if prec == Q.QT_FP64 else 2e-5)
