@@ -152,10 +152,16 @@ def main():
     achieved, peak = valu_roofline(fd * b32["tries"], fc * b32["tries"], b32["kernel_s"])
 
     traffic = None
+    issue = None
     tpath = os.path.join(ROOT, "profiles", "mcwf_batch_jc60_16k_traffic.json")
     if os.path.exists(tpath) and args.ntraj == 16384 and args.T == 20.0:
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_launch"]
+            tj = json.load(f)
+        traffic = tj["traffic_bytes_per_launch"]
+        # VALU issue: the counted instructions per try x 4 cycles per wave64 instruction over the
+        # SIMDs' cycles in the measured time (256 CUs x 4 SIMDs at 2.4 GHz)
+        if "sq_insts_valu_per_try" in tj:
+            issue = tj["sq_insts_valu_per_try"] * b32["tries"] * 4 / (b32["kernel_s"] * 2.4e9 * 1024)
 
     cpu = None
     if not args.no_cpu and args.cpu_traj > 0:
@@ -222,9 +228,10 @@ def main():
                                "Dt=0.1 streams, FP32 derivatives + FP64 control",
                    "n_traj": args.ntraj, "T": args.T, "dim": 60, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
+                     "frac": achieved / peak, "traffic": traffic, "valu_issue_frac": issue,
                      "note": f"F = 6*8*{sysq.info.m_terms} FP32 + 124*60 FP64 flop per RK try; "
-                             "peak = time-weighted mix of 157.3 FP32 / 78.6 FP64 vector"},
+                             "peak = time-weighted mix of 157.3 FP32 / 78.6 FP64 vector; valu_issue_frac = "
+                             "counted VALU instructions x 4 cycles / SIMD cycles of the measured time"},
         "batched_fp64": {"value": v64, "seconds": b64["seconds"], "tries": b64["tries"]},
         "plain_fp64": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
