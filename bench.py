@@ -324,8 +324,13 @@ def serve(Q, args):
     t_next = t_start + rng.exponential(1.0 / args.serve_rate)
     pending, lat, waits, tries, kinds = {}, [], [], 0, {}
     submitted = 0
+    t_report = t_start + 60.0
     while True:
         now = time.perf_counter()
+        if now >= t_report:  # progress of a long load, once a minute
+            print(f"serving: {now - t_start:.0f} s, {submitted} submitted, {len(lat)} finished", file=sys.stderr,
+                  flush=True)
+            t_report += 60.0
         while now >= t_next and now - t_start < args.serve_seconds:
             k = int(rng.integers(0, len(pool) + 1))
             if k == len(pool):
