@@ -699,3 +699,6 @@ class MasterSweep:
         st.t_ss = -1.0
         self.set_point(point, st, rho)
         return fr
+
+
+from . import statefile, trajectory  # noqa: E402,F401  (Q.statefile, Q.trajectory)
