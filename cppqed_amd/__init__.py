@@ -117,6 +117,7 @@ def lib():
         l.qt_server_result.argtypes = [vp, C.c_longlong, vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         l.qt_server_release.argtypes = [vp, C.c_longlong]
+        l.qt_server_submit_sweep.argtypes = [vp, vp, ip, vp, dp, dp, C.POINTER(MasterParams), C.POINTER(C.c_longlong)]
         l.qt_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
         l.qt_master_create.argtypes = [vp, ip, C.POINTER(MasterParams), vp, C.POINTER(vp)]
         l.qt_master_destroy.argtypes = [vp]
@@ -552,6 +553,20 @@ class Server:
         self._systems[rid.value] = system
         return rid.value
 
+    def submit_sweep(self, systems, rho0, T, Dt=0.1, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0):
+        """A Master-equation parameter sweep (one point per system) as a request; its result rows
+        are point-major, shape (points * intervals, 2 + n_avr)."""
+        rho = np.ascontiguousarray(np.asarray(rho0, dtype=np.complex128))
+        arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
+        mp = MasterParams(epsRel, epsAbs, dtInit, ss_tol, 0.0, 0.0, 0)
+        rid = C.c_longlong()
+        check(lib().qt_server_submit_sweep(self.h, arr, len(systems), rho.ctypes.data, float(T), float(Dt),
+                                           C.byref(mp), C.byref(rid)))
+        self._systems[rid.value] = systems[0]
+        self._keep = getattr(self, "_keep", {})
+        self._keep[rid.value] = list(systems)
+        return rid.value
+
     def step(self):
         a, w = C.c_int(), C.c_int()
         check(lib().qt_server_step(self.h, C.byref(a), C.byref(w)))
@@ -572,6 +587,7 @@ class Server:
     def release(self, rid):
         check(lib().qt_server_release(self.h, rid))
         self._systems.pop(rid, None)
+        getattr(self, "_keep", {}).pop(rid, None)
 
     def stats(self):
         st = ServerStats()

@@ -1499,7 +1499,17 @@ struct Request {
   std::vector<double> rows;
   double tSubmit = 0., tAdmit = 0., tDone = 0.;
   long long tries = 0;
-  ~Request() { if (ens) qt_mcwf_destroy(ens); }
+  // a Master-equation parameter sweep (qt_server_submit_sweep) instead of an MCWF ensemble
+  bool sweep = false;
+  std::vector<qt_sys*> systems;
+  std::vector<double> rho0;
+  qt_master_params mp{};
+  qt_master* m = nullptr;
+  std::vector<std::vector<double>> prow;  // per point, the streamed rows so far
+  ~Request() {
+    if (ens) qt_mcwf_destroy(ens);
+    if (m) qt_master_destroy(m);
+  }
 };
 
 // device bytes a request holds while admitted (qt_ens buffers for one chunk of `chunk` intervals)
@@ -1526,6 +1536,8 @@ struct qt_server_s {
   qt_batch_s batch;
   long long steps = 0, tries = 0, finished = 0;
   double kernelMs = 0.;
+  hipStream_t sweepStream = nullptr;  // non-blocking: sweeps run beside the mixed launch
+  ~qt_server_s() { if (sweepStream) (void)hipStreamDestroy(sweepStream); }
 };
 
 extern "C" {
@@ -1581,6 +1593,41 @@ int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0, 
   });
 }
 
+int qt_server_submit_sweep(qt_server* s, qt_sys* const* systems, int n_points, const double* rho0, double T,
+                           double Dt, const qt_master_params* p, long long* id) {
+  return guard([&] {
+    if (!s || !systems || !rho0 || !p || n_points <= 0 || !(T > 0) || !(Dt > 0)) throw Fail(QT_EINVAL, "bad arguments");
+    for (int k = 0; k < n_points; ++k) if (!systems[k]) throw Fail(QT_EINVAL, "null system");
+    const qt::System& h = *systems[0]->h;
+    // what qt_master_create would refuse is refused here, not at admission inside a step
+    for (int k = 0; k < n_points; ++k) {
+      const qt::System& hk = *systems[k]->h;
+      if (hk.dim != h.dim || hk.J.size() != h.J.size()) throw Fail(QT_EINVAL, "sweep points differ in dimension or jumps");
+      if (hk.H.timeDependent() || hk.hasEx) throw Fail(QT_ENOTSUP, "the request loop runs the Schroedinger picture only");
+    }
+    if (h.dim > 96) throw Fail(QT_ENOTSUP, "Master sweeps run dimensions up to 96");
+    std::unique_ptr<Request> r(new Request);
+    r->id = s->nextId++;
+    r->sweep = true;
+    r->sys = systems[0];
+    r->systems.assign(systems, systems + n_points);
+    r->n = n_points;
+    r->rho0.assign(rho0, rho0 + 2 * (size_t)h.dim * h.dim);
+    r->mp = *p;
+    r->iv = dt_intervals(T, Dt);
+    r->prow.resize(n_points);
+    // device bytes while admitted: per point rho, the seven RK slots and the H fragments on the
+    // padded 16 x 16 tile grid, plus a chunk of rows
+    const size_t dp = ((size_t)h.dim + 15) / 16 * 16;
+    r->bytes = (size_t)n_points * (10 * dp * dp * 16 + (size_t)s->P.chunk_intervals * (3 + h.O.size()) * 8);
+    if (r->bytes > s->P.hbm_budget) throw Fail(QT_ENOMEM, "request larger than the server's memory budget");
+    r->tSubmit = now_s();
+    if (id) *id = r->id;
+    s->waiting.push_back(r.get());
+    s->all[r->id] = std::move(r);
+  });
+}
+
 // One continuous-batching iteration: admit waiting requests in arrival order while slots and
 // the memory budget allow (a request that does not fit lets later, smaller ones pass), advance
 // every admitted request by its next chunk of stream intervals in ONE mixed launch, append the
@@ -1592,7 +1639,8 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
       Request* r = *it;
       if ((int)s->active.size() >= s->P.max_active) break;
       if (s->used + r->bytes > s->P.hbm_budget) { ++it; continue; }
-      int rc = qt_mcwf_create(r->sys, r->n, &r->p, r->psi0.data(), &r->ens);
+      int rc = r->sweep ? qt_master_create(r->systems.data(), r->n, &r->mp, r->rho0.data(), &r->m)
+                        : qt_mcwf_create(r->sys, r->n, &r->p, r->psi0.data(), &r->ens);
       if (rc) throw Fail(rc, g_err);
       r->tAdmit = now_s();
       s->used += r->bytes;
@@ -1600,45 +1648,84 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
       it = s->waiting.erase(it);
     }
     if (!s->active.empty()) {
-      const size_t na = s->active.size();
+      std::vector<Request*> ens, sw;
+      for (auto* r : s->active) (r->sweep ? sw : ens).push_back(r);
+      const size_t na = ens.size();
       std::vector<const double*> ivp(na);
       std::vector<int> ni(na);
       std::vector<double> t0(na);
       for (size_t m = 0; m < na; ++m) {
-        Request* r = s->active[m];
+        Request* r = ens[m];
         ivp[m] = r->iv.data() + r->next;
         ni[m] = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
         t0[m] = r->ens->time;
       }
-      s->batch.mem.clear();
-      for (auto* r : s->active) s->batch.mem.push_back(r->ens);
-      s->batch.choose(-1, nullptr);
-      batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
-      batch_check(&s->batch, nullptr);
-      for (size_t m = 0; m < na; ++m) {
-        Request* r = s->active[m];
-        const int no = (int)r->sys->h->O.size();
-        std::vector<double> rows((size_t)ni[m] * (2 + no));
-        for (int k = 0; k < ni[m]; ++k) r->ens->time += ivp[m][k];
-        ens_rows(r->ens, t0[m], ivp[m], ni[m], rows.data(), nullptr);
-        r->rows.insert(r->rows.end(), rows.begin(), rows.end());
-        r->next += ni[m];
+      // the ensembles in one mixed launch (asynchronous), the sweeps beside it on their own stream
+      if (na) {
+        s->batch.mem.clear();
+        for (auto* r : ens) s->batch.mem.push_back(r->ens);
+        s->batch.choose(-1, nullptr);
+        batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
       }
-      float ms = 0.f;
-      hipCheck(hipEventElapsedTime(&ms, s->batch.e0, s->batch.e1), "hipEventElapsedTime");
-      s->kernelMs += ms;
+      if (!sw.empty() && !s->sweepStream)
+        hipCheck(hipStreamCreateWithFlags(&s->sweepStream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+      for (auto* r : sw) {
+        const int nk = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
+        const int no = (int)r->sys->h->O.size();
+        std::vector<double> rows((size_t)r->n * nk * (2 + no));
+        int rc = qt_master_advance(r->m, r->iv.data() + r->next, nk, rows.data(), nullptr, (qt_stream)s->sweepStream);
+        if (rc) throw Fail(rc, g_err);
+        for (int k = 0; k < r->n; ++k)
+          r->prow[k].insert(r->prow[k].end(), rows.begin() + (size_t)k * nk * (2 + no),
+                            rows.begin() + (size_t)(k + 1) * nk * (2 + no));
+        r->next += nk;
+      }
+      if (na) {
+        batch_check(&s->batch, nullptr);
+        for (size_t m = 0; m < na; ++m) {
+          Request* r = ens[m];
+          const int no = (int)r->sys->h->O.size();
+          std::vector<double> rows((size_t)ni[m] * (2 + no));
+          for (int k = 0; k < ni[m]; ++k) r->ens->time += ivp[m][k];
+          ens_rows(r->ens, t0[m], ivp[m], ni[m], rows.data(), nullptr);
+          r->rows.insert(r->rows.end(), rows.begin(), rows.end());
+          r->next += ni[m];
+        }
+      }
+      if (na) {
+        float ms = 0.f;
+        hipCheck(hipEventElapsedTime(&ms, s->batch.e0, s->batch.e1), "hipEventElapsedTime");
+        s->kernelMs += ms;
+      }
       ++s->steps;
       // retire
       std::vector<Request*> keep;
       for (auto* r : s->active) {
         if (r->next < r->iv.size()) { keep.push_back(r); continue; }
-        long long tot[qt::kCounters];
-        int rc = qt_mcwf_counters(r->ens, nullptr, tot);
-        if (rc) throw Fail(rc, g_err);
-        r->tries = tot[qt::kTries];
+        if (r->sweep) {
+          std::vector<int> done(r->n);
+          std::vector<double> tss(r->n), tt(r->n);
+          std::vector<long long> c((size_t)r->n * qt::kCounters);
+          int rc = qt_master_status(r->m, done.data(), tss.data(), tt.data(), c.data());
+          if (rc) throw Fail(rc, g_err);
+          r->tries = 0;
+          for (int k = 0; k < r->n; ++k) r->tries += c[(size_t)k * qt::kCounters + qt::kTries];
+          double ms = 0.; int nl = 0;
+          qt_master_kernel_ms(r->m, &ms, &nl);
+          s->kernelMs += ms;
+          qt_master_destroy(r->m);
+          r->m = nullptr;
+          for (auto& pr : r->prow) r->rows.insert(r->rows.end(), pr.begin(), pr.end());  // point-major
+          r->prow.clear();
+        } else {
+          long long tot[qt::kCounters];
+          int rc = qt_mcwf_counters(r->ens, nullptr, tot);
+          if (rc) throw Fail(rc, g_err);
+          r->tries = tot[qt::kTries];
+          qt_mcwf_destroy(r->ens);
+          r->ens = nullptr;
+        }
         s->tries += r->tries;
-        qt_mcwf_destroy(r->ens);
-        r->ens = nullptr;
         s->used -= r->bytes;
         r->tDone = now_s();
         ++s->finished;

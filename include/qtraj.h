@@ -263,6 +263,16 @@ int qt_master_rho(qt_master* m, int point, double* rho_host);
 int qt_master_status(qt_master* m, int* done, double* t_ss, double* t, long long* counters);
 int qt_master_kernel_ms(qt_master* m, double* total_ms, int* launches);
 
+/* A parameter-sweep solve for the request loop: n_points Master-equation points (systems[k], the
+ * same dimension and jump structure, all from rho0_host, d*d complex row-major; parameters as
+ * qt_master_create) advanced to T in stream intervals of Dt, a chunk per qt_server_step, on a
+ * second stream beside the step's mixed MCWF launch.  Admitted by slots and memory like the
+ * ensembles.  qt_server_result's rows are point-major: [point][interval][2 + n_avr], as
+ * qt_master_advance writes them.  No counterpart in the reference (its Master runs one point
+ * per process, Master.h:50-208; sweeps are separate runs). */
+int qt_server_submit_sweep(qt_server* s, qt_sys* const* systems, int n_points, const double* rho0_host, double T,
+                           double Dt, const qt_master_params* p, long long* id);
+
 /* One sweep point's dynamical state: what Master::stateIO archives (Master.h:89-102: array,
  * time, ODE logger, dtDid, dtTry) plus the sweep's steady-state flag. */
 typedef struct {

@@ -86,3 +86,32 @@ def test_served_results_match_oracle():
         ro, _ = O.ensemble_run(O.System(spec), psi, O.params(rng="philox", seed=seed, stream=0),
                                O.run_spec("Dt", Dt=0.1, T=T), n)
         np.testing.assert_allclose(res[r][0], ro[1:], rtol=1e-9, atol=1e-9)
+
+
+def test_sweep_requests_same_alone_and_in_mix():
+    """Parameter-sweep solves (Master-equation points) as requests: the rows are the same alone,
+    among ensemble requests arriving before and after, and as the sweep driven directly."""
+    from sysdefs import jc_master_spec
+    print("[gpu] serve sweep alone vs mix", flush=True)
+    pts = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in (0.5, 4.0, 9.0)]
+    rho0 = np.zeros((80, 80), complex)
+    rho0[0, 0] = 1
+    T = 1.0
+    srv = Q.Server(chunk_intervals=4)
+    rid = srv.submit_sweep(pts, rho0, T=T, epsRel=1e-8, epsAbs=1e-8)
+    alone = _drain(srv, [rid])[rid][0]
+    iv = Q.run_intervals(T=T, Dt=0.1)
+    assert alone.shape == (len(pts) * len(iv), 2 + pts[0].n_avr)
+    jc = Q.System(JC60)
+    srv = Q.Server(chunk_intervals=4, pars=Q.mcwf_pars(precision=Q.QT_FP32_DERIV))
+    ids = [srv.submit(jc, 512, ground(60), T=1.5, seed=3)]
+    srv.step()
+    rid = srv.submit_sweep(pts, rho0, T=T, epsRel=1e-8, epsAbs=1e-8)
+    srv.step()
+    ids.append(srv.submit(jc, 256, ground(60), T=0.8, seed=4))
+    res = _drain(srv, ids + [rid])
+    np.testing.assert_array_equal(res[rid][0], alone)
+    assert res[rid][3] > 0 and all(res[r] is not None for r in ids)
+    m = Q.MasterSweep(pts, rho0, epsRel=1e-8, epsAbs=1e-8)
+    rows, _ = m.advance(iv)
+    np.testing.assert_array_equal(alone.reshape(len(pts), len(iv), -1), rows)
