@@ -310,14 +310,17 @@ def mixed_batch(Q, args):
 
 def serve(Q, args):
     """Serving load: Poisson arrivals at a fixed rate, each an MCWF ensemble (256-4096 trajectories,
-    log-uniform) of a system drawn from the PLM-20, JC-60, mixed-batch and JC-80 builds, T=2, Dt=0.1,
-    FP32 derivatives; continuous batching with memory-based admission."""
+    log-uniform) of a system drawn from the PLM-20, JC-60, mixed-batch and JC-80 builds, or a
+    Master-equation parameter sweep (4-16 JC-80 points), T=2, Dt=0.1, FP32 derivatives for the
+    ensembles; continuous batching with memory-based admission."""
     from sysdefs import JC60, MIXED, PLM20, basis, coherent, ground, jc_master_spec
     rng = np.random.default_rng(4242)
     pool = [("plm20", Q.System(PLM20), coherent(0.5, 20)), ("jc60", Q.System(JC60), ground(60))]
     pool += [(n, Q.System(spec), None) for n, spec, _ in MIXED[:3]]
     idx = {n: i for n, _, i in MIXED}
     jc80 = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in np.linspace(0.5, 15, 8)]
+    rho80 = np.zeros((80, 80), complex)
+    rho80[0, 0] = 1
     srv = Q.Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_active=16, chunk_intervals=10,
                    pars=Q.mcwf_pars(precision=Q.QT_FP32_DERIV))
     t_start = time.perf_counter()
@@ -332,7 +335,16 @@ def serve(Q, args):
                   flush=True)
             t_report += 60.0
         while now >= t_next and now - t_start < args.serve_seconds:
-            k = int(rng.integers(0, len(pool) + 1))
+            k = int(rng.integers(0, len(pool) + 2))
+            if k == len(pool) + 1:
+                # a parameter-sweep solve: 4-16 JC-80 Master-equation points (eta values), T=2
+                npts = int(rng.integers(4, 17))
+                pts = [jc80[int(i)] for i in rng.integers(0, len(jc80), npts)]
+                rid = srv.submit_sweep(pts, rho80, T=2.0, Dt=0.1)
+                pending[rid] = "sweep_jc80"
+                submitted += 1
+                t_next += rng.exponential(1.0 / args.serve_rate)
+                continue
             if k == len(pool):
                 name, s = "jc80", jc80[int(rng.integers(0, len(jc80)))]
                 psi = ground(80)
@@ -362,7 +374,8 @@ def serve(Q, args):
     lat = np.array(lat)
     return {"workload": f"Poisson arrivals at {args.serve_rate}/s for {args.serve_seconds} s; ensembles of 256-4096 "
                         "trajectories (log-uniform) of PLM-20, JC-60, particle-cavity 1024, four modes 4096, six qubits 64 "
-                        "or JC-80 (eta in [0.5, 15]); T=2, Dt=0.1; FP32 derivatives; up to 16 requests per launch, "
+                        "or JC-80 (eta in [0.5, 15]), or Master-equation sweeps of 4-16 JC-80 points; T=2, Dt=0.1; "
+                        "FP32 derivatives; up to 16 requests per step, "
                         f"{args.serve_budget_gib} GiB admission budget",
             "requests": int(len(lat)), "submitted": submitted, "seconds": el, "tries": tries, "tries_per_s": tries / el,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
