@@ -243,3 +243,12 @@ def test_run_intervals_mirror_trajectory_run():
     assert len(iv) == 11 and iv[-1] == pytest.approx(1.1102230246251565e-16, rel=1e-6)
     assert len(Q.run_intervals(NDt=10, Dt=0.1)) == 10
     assert len(Q.run_intervals(T=20.0, Dt=0.1)) == 200 or len(Q.run_intervals(T=20.0, Dt=0.1)) == 201
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message instead of a number."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and not r.stdout.strip()
