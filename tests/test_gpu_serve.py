@@ -115,3 +115,27 @@ def test_sweep_requests_same_alone_and_in_mix():
     m = Q.MasterSweep(pts, rho0, epsRel=1e-8, epsAbs=1e-8)
     rows, _ = m.advance(iv)
     np.testing.assert_array_equal(alone.reshape(len(pts), len(iv), -1), rows)
+
+
+def test_sweep_request_refusals():
+    """Sweep points must share dimension and jumps, run the Schroedinger picture and fit the
+    Master kernel (d <= 96); refused at submission, so the loop itself never fails on them."""
+    from sysdefs import jc_master_spec
+    srv = Q.Server(chunk_intervals=4)
+    a = Q.System(jc_master_spec(40, eta=1.0, g=10.0))
+    b = Q.System(jc_master_spec(30, eta=1.0, g=10.0))
+    rho = np.zeros((80, 80), complex)
+    rho[0, 0] = 1
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([a, b], rho, T=1.0)                 # dimensions differ
+    big = Q.System(jc_master_spec(60, eta=1.0, g=10.0))
+    rho_big = np.zeros((120, 120), complex)
+    rho_big[0, 0] = 1
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([big], rho_big, T=1.0)              # beyond the Master kernel's 96
+    ip = Q.System("free mode cutoff=20 delta=-1 kappa=1 eta=2 picture=IP")
+    rho20 = np.zeros((20, 20), complex)
+    rho20[0, 0] = 1
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([ip], rho20, T=1.0)                 # interaction picture
+    assert srv.step() == (0, 0)
