@@ -386,7 +386,10 @@ struct qt_master {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double kernelMs = 0.;
   int launches = 0;
+  double* hbuf = nullptr;  // pinned staging for the stream outputs (copies stay asynchronous)
+  size_t hcap = 0;
   ~qt_master() {
+    if (hbuf) (void)hipHostFree(hbuf);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
   }
@@ -1225,10 +1228,11 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
 
 int qt_master_destroy(qt_master* m) { delete m; return QT_OK; }
 
-int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* rows, double* dnorm, qt_stream stream) {
-  return guard([&] {
-    if (!m || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
-    hipStream_t st = (hipStream_t)stream;
+// qt_master_advance in two halves, so that a caller can have several sweeps (and other work)
+// in flight before waiting: the launch half uploads the intervals, launches and queues the copies
+// of the stream outputs into pinned host memory, all asynchronous on st; the finish half waits
+// for st and turns the outputs into rows.
+static void master_launch(qt_master* m, const double* intervals, int n_int, hipStream_t st) {
     const qt::System& s0 = *m->sys0->h;
     const int no = (int)s0.O.size();
     m->intervals.alloc(n_int);
@@ -1251,11 +1255,26 @@ int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* 
     hipCheck(hipEventRecord(m->e0, st), "hipEventRecord");
     hipCheck(qt::launch_master(m->nb, M, st), "master launch");
     hipCheck(hipEventRecord(m->e1, st), "hipEventRecord");
-    std::vector<double> ob((size_t)m->n * n_int * std::max(1, no)), dd((size_t)m->n * n_int), dnv((size_t)m->n * n_int);
-    hipCheck(hipMemcpyAsync(ob.data(), m->obs.p, ob.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-    hipCheck(hipMemcpyAsync(dd.data(), m->dtd.p, dd.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-    hipCheck(hipMemcpyAsync(dnv.data(), m->dn.p, dnv.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    const size_t nob = (size_t)m->n * n_int * std::max(1, no), nd = (size_t)m->n * n_int;
+    if (m->hcap < nob + 2 * nd) {
+      if (m->hbuf) hipCheck(hipHostFree(m->hbuf), "hipHostFree");
+      m->hbuf = nullptr; m->hcap = 0;
+      hipCheck(hipHostMalloc((void**)&m->hbuf, (nob + 2 * nd) * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+      m->hcap = nob + 2 * nd;
+    }
+    hipCheck(hipMemcpyAsync(m->hbuf, m->obs.p, nob * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(m->hbuf + nob, m->dtd.p, nd * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(m->hbuf + nob + nd, m->dn.p, nd * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+}
+
+static void master_finish(qt_master* m, const double* intervals, int n_int, double* rows, double* dnorm, hipStream_t st) {
     hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const qt::System& s0 = *m->sys0->h;
+    const int no = (int)s0.O.size();
+    const size_t nob = (size_t)m->n * n_int * std::max(1, no), nd = (size_t)m->n * n_int;
+    const double* ob = m->hbuf;
+    const double* dd = m->hbuf + nob;
+    const double* dnv = m->hbuf + nob + nd;
     float ms = 0.f;
     hipCheck(hipEventElapsedTime(&ms, m->e0, m->e1), "hipEventElapsedTime");
     m->kernelMs += ms; m->launches += 1;
@@ -1274,6 +1293,13 @@ int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* 
         if (dnorm) dnorm[(size_t)pt * n_int + k] = dnv[(size_t)pt * n_int + k];
       }
     }
+}
+
+int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* rows, double* dnorm, qt_stream stream) {
+  return guard([&] {
+    if (!m || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    master_launch(m, intervals, n_int, (hipStream_t)stream);
+    master_finish(m, intervals, n_int, rows, dnorm, (hipStream_t)stream);
   });
 }
 
@@ -1505,10 +1531,12 @@ struct Request {
   std::vector<double> rho0;
   qt_master_params mp{};
   qt_master* m = nullptr;
+  hipStream_t st = nullptr;               // its own non-blocking stream while admitted
   std::vector<std::vector<double>> prow;  // per point, the streamed rows so far
   ~Request() {
     if (ens) qt_mcwf_destroy(ens);
     if (m) qt_master_destroy(m);
+    if (st) (void)hipStreamDestroy(st);
   }
 };
 
@@ -1536,8 +1564,6 @@ struct qt_server_s {
   qt_batch_s batch;
   long long steps = 0, tries = 0, finished = 0;
   double kernelMs = 0.;
-  hipStream_t sweepStream = nullptr;  // non-blocking: sweeps run beside the mixed launch
-  ~qt_server_s() { if (sweepStream) (void)hipStreamDestroy(sweepStream); }
 };
 
 extern "C" {
@@ -1667,14 +1693,19 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
         s->batch.choose(-1, nullptr);
         batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
       }
-      if (!sw.empty() && !s->sweepStream)
-        hipCheck(hipStreamCreateWithFlags(&s->sweepStream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-      for (auto* r : sw) {
-        const int nk = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
-        const int no = (int)r->sys->h->O.size();
+      // every sweep launched on its own non-blocking stream before anything is waited for
+      std::vector<int> nks(sw.size());
+      for (size_t i = 0; i < sw.size(); ++i) {
+        Request* r = sw[i];
+        if (!r->st) hipCheck(hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        nks[i] = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
+        master_launch(r->m, r->iv.data() + r->next, nks[i], r->st);
+      }
+      for (size_t i = 0; i < sw.size(); ++i) {
+        Request* r = sw[i];
+        const int nk = nks[i], no = (int)r->sys->h->O.size();
         std::vector<double> rows((size_t)r->n * nk * (2 + no));
-        int rc = qt_master_advance(r->m, r->iv.data() + r->next, nk, rows.data(), nullptr, (qt_stream)s->sweepStream);
-        if (rc) throw Fail(rc, g_err);
+        master_finish(r->m, r->iv.data() + r->next, nk, rows.data(), nullptr, r->st);
         for (int k = 0; k < r->n; ++k)
           r->prow[k].insert(r->prow[k].end(), rows.begin() + (size_t)k * nk * (2 + no),
                             rows.begin() + (size_t)(k + 1) * nk * (2 + no));
@@ -1715,6 +1746,8 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
           s->kernelMs += ms;
           qt_master_destroy(r->m);
           r->m = nullptr;
+          if (r->st) hipCheck(hipStreamDestroy(r->st), "hipStreamDestroy");
+          r->st = nullptr;
           for (auto& pr : r->prow) r->rows.insert(r->rows.end(), pr.begin(), pr.end());  // point-major
           r->prow.clear();
         } else {
