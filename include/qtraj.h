@@ -265,8 +265,8 @@ int qt_master_kernel_ms(qt_master* m, double* total_ms, int* launches);
 
 /* A parameter-sweep solve for the request loop: n_points Master-equation points (systems[k], the
  * same dimension and jump structure, all from rho0_host, d*d complex row-major; parameters as
- * qt_master_create) advanced to T in stream intervals of Dt, a chunk per qt_server_step, on a
- * second stream beside the step's mixed MCWF launch.  Admitted by slots and memory like the
+ * qt_master_create) advanced to T in stream intervals of Dt, a chunk per qt_server_step, on its
+ * own stream beside the step's mixed MCWF launch.  Admitted by slots and memory like the
  * ensembles.  qt_server_result's rows are point-major: [point][interval][2 + n_avr], as
  * qt_master_advance writes them.  No counterpart in the reference (its Master runs one point
  * per process, Master.h:50-208; sweeps are separate runs). */
