@@ -17,12 +17,17 @@ LIBPATH = os.path.join(_HERE, "libqtraj.so")
 
 QT_FP64, QT_FP32_DERIV = 0, 1
 QT_CONTROLLER_ODEINT, QT_CONTROLLER_GSL = 0, 1
+QT_OK, QT_EINVAL, QT_EDEVICE, QT_ENOMEM, QT_ENUMERIC, QT_ENOTSUP = 0, 1, 2, 3, 4, 5
 
 _lib = None
 
 
 class QtError(RuntimeError):
-    pass
+    """An error of the C ABI: .code is the status (QT_EINVAL, QT_ENUMERIC, ...)."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 class SysInfo(C.Structure):
@@ -56,7 +61,8 @@ class ServerParams(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("finished", C.c_longlong), ("tries", C.c_longlong),
-                ("kernel_ms", C.c_double), ("active", C.c_int), ("waiting", C.c_int), ("bytes_in_use", C.c_size_t)]
+                ("kernel_ms", C.c_double), ("active", C.c_int), ("waiting", C.c_int), ("bytes_in_use", C.c_size_t),
+                ("failed", C.c_longlong)]
 
 
 class TrajState(C.Structure):
@@ -131,7 +137,7 @@ def lib():
 
 def check(rc):
     if rc:
-        raise QtError(lib().qt_last_error().decode())
+        raise QtError(lib().qt_last_error().decode(), rc)
 
 
 def _cbuf(a):
@@ -547,18 +553,25 @@ class Server:
             self.h = None
 
     def submit(self, system, n_traj, psi0, T, Dt=0.1, seed=1001):
+        if np.shape(psi0) != (system.dim,):
+            raise ValueError(f"psi0 must have shape ({system.dim},), got {np.shape(psi0)}")
         psi0, p = _cbuf(psi0)
         rid = C.c_longlong()
         check(lib().qt_server_submit(self.h, system.h, int(n_traj), p, float(T), float(Dt), int(seed), C.byref(rid)))
         self._systems[rid.value] = system
         return rid.value
 
-    def submit_sweep(self, systems, rho0, T, Dt=0.1, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0):
+    def submit_sweep(self, systems, rho0, T, Dt=0.1, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0,
+                     autostop=None):
         """A Master-equation parameter sweep (one point per system) as a request; its result rows
-        are point-major, shape (points * intervals, 2 + n_avr)."""
+        are point-major, shape (points * intervals, 2 + n_avr).  autostop as for MasterSweep."""
+        d = systems[0].dim
+        if np.shape(rho0) != (d, d):
+            raise ValueError(f"rho0 must have shape ({d}, {d}), got {np.shape(rho0)}")
         rho = np.ascontiguousarray(np.asarray(rho0, dtype=np.complex128))
         arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
-        mp = MasterParams(epsRel, epsAbs, dtInit, ss_tol, 0.0, 0.0, 0)
+        a_eps, a_abs, a_rep = autostop if autostop else (0.0, 0.0, 0)
+        mp = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep)
         rid = C.c_longlong()
         check(lib().qt_server_submit_sweep(self.h, arr, len(systems), rho.ctypes.data, float(T), float(Dt),
                                            C.byref(mp), C.byref(rid)))
@@ -573,12 +586,17 @@ class Server:
         return a.value, w.value
 
     def result(self, rid):
-        """(rows, latency_s, queue_wait_s, tries) of a finished request, or None if not finished."""
+        """(rows, latency_s, queue_wait_s, tries) of a finished request, or None if not finished;
+        raises QtError (with the request's error code) for a request that failed."""
         system = self._systems[rid]
         n = C.c_int()
         lat, wait, tries = C.c_double(), C.c_double(), C.c_longlong()
-        if lib().qt_server_result(self.h, rid, None, 0, C.byref(n), C.byref(lat), C.byref(wait), C.byref(tries)):
-            return None
+        rc = lib().qt_server_result(self.h, rid, None, 0, C.byref(n), C.byref(lat), C.byref(wait), C.byref(tries))
+        if rc:
+            msg = lib().qt_last_error().decode()
+            if msg == "request not finished":
+                return None
+            raise QtError(msg, rc)
         rows = np.zeros((n.value, 2 + system.n_avr))
         check(lib().qt_server_result(self.h, rid, rows.ctypes.data, n.value, C.byref(n), C.byref(lat), C.byref(wait),
                                      C.byref(tries)))

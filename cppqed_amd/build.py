@@ -17,8 +17,9 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC"]
 
-SOURCES = [("qt_system.cpp", "cxx"), ("qt_capi.cpp", "hip"), ("qt_mcwf.hip", "hip"), ("qt_subgroup.hip", "hip"),
-           ("qt_master.hip", "hip")]
+# heaviest first: the kernel templates are instantiated in several translation units, compiled in parallel
+SOURCES = [("qt_batch.hip", "hip"), ("qt_batch_td.hip", "hip"), ("qt_batch_gsl.hip", "hip"), ("qt_mixed.hip", "hip"), ("qt_aux.hip", "hip"),
+           ("qt_subgroup.hip", "hip"), ("qt_master.hip", "hip"), ("qt_capi.cpp", "hip"), ("qt_system.cpp", "cxx")]
 
 
 def _run(cmd):
@@ -29,29 +30,42 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def _newest_header():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    hs.append(os.path.join(HERE, "..", "include", "qtraj.h"))
+    return max(os.path.getmtime(h) for h in hs if os.path.exists(h))
+
+
+def build(verbose=False, incremental=False):
+    """Compiles every source (incremental=True: only objects older than their source or a header)
+    and links the library."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     jobs = []
+    hdr = _newest_header()
+    objs = []
     for src, kind in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src + ".o")
+        objs.append(obj)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(path), hdr):
+            continue
         if kind == "cxx":
             cmd = ["g++"] + CXXFLAGS + ["-c", path, "-o", obj]
         else:
             cmd = ["hipcc"] + HIPFLAGS + ["-c", path, "-o", obj]
         jobs.append((cmd, obj))
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1))) as ex:
         for out in ex.map(lambda j: _run(j[0]), jobs):
             if verbose and out.strip():
                 print(out)
     tmp = OUT + ".tmp"
-    _run(["hipcc", "-shared", f"--offload-arch={ARCH}", "-o", tmp] + [o for _, o in jobs])
+    _run(["hipcc", "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, OUT)
     return OUT
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

@@ -397,6 +397,26 @@ struct qt_master {
 
 namespace {
 
+// what qt_master_create refuses about its parameters and the sweep's systems
+void check_master(qt_sys* const* systems, int n_points, const qt_master_params* p) {
+  if (!systems || !p || n_points <= 0) throw Fail(QT_EINVAL, "bad arguments");
+  for (int k = 0; k < n_points; ++k) if (!systems[k]) throw Fail(QT_EINVAL, "null system");
+  if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
+  if (p->autostop_rep < 0 || (p->autostop_rep > 0 && !(p->autostop_eps > 0.)))
+    throw Fail(QT_EINVAL, "autostop needs a positive epsilon and a non-negative repetition");
+  const qt::System& s0 = *systems[0]->h;
+  if (p->autostop_rep > 0 && (int)s0.O.size() > qt::kMaxAutostopAvr)
+    throw Fail(QT_ENOTSUP, "autostop watches at most 64 averages");
+  if (!qt::master_supported((int)(s0.dim + 15) / 16)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
+  for (int k = 0; k < n_points; ++k) {
+    const qt::System& s = *systems[k]->h;
+    if (s.dim != s0.dim || s.J.size() != s0.J.size())
+      throw Fail(QT_EINVAL, "sweep points must share dimension and jump channels");
+    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the Master sweep runs the Schroedinger picture only");
+  }
+}
+
+
 // rates of a host state through the host ELL tables: ||J_m psi||^2
 std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& psi) {
   std::vector<double> r;
@@ -522,6 +542,20 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     check_params(p);
     const qt::System& s = *sys->h;
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
+    const long d = s.dim;
+    std::vector<qt::cd> psi(d);
+    for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
+    double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
+    // constructor check (MCWF_Trajectory.h:114-117): dtTry <= dpLimit / total rate at t = 0; the
+    // rates pass LiouvilleanAveragedCommon::average's check (LiouvilleanAveragedCommon.h:82-93).
+    // Host tables only: refused before the device is touched.
+    double dtTry = dtInit;
+    if (!s.J.empty()) {
+      auto r = host_rates(s, psi);
+      double tot = 0.; for (double x : r) tot += x;
+      if (!std::isfinite(tot)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+      dtTry = std::min(dtTry, p->dpLimit / tot);
+    }
     sys->checkPictureSupported();
     if (sys->interactionPicture() && p->layout_group_lanes > 0)
       throw Fail(QT_ENOTSUP, "sub-wave layouts run the Schroedinger picture only");
@@ -550,19 +584,8 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
           throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
       }
     }
-    const long d = s.dim;
-    std::vector<qt::cd> psi(d);
-    for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
     if (p->controller == QT_CONTROLLER_GSL && useSub)
       throw Fail(QT_ENOTSUP, "the GSL controller runs in the one-wavefront engine (dimensions up to 256)");
-    double dtInit = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
-    // constructor check (MCWF_Trajectory.h:114-117): dtTry <= dpLimit / total rate at t = 0
-    double dtTry = dtInit;
-    if (!s.J.empty()) {
-      auto r = host_rates(s, psi);
-      double tot = 0.; for (double x : r) tot += x;
-      dtTry = std::min(dtTry, p->dpLimit / tot);
-    }
     e->psi.alloc((size_t)n_traj * d);
     std::vector<double2> rep((size_t)n_traj * d);
     for (int k = 0; k < n_traj; ++k)
@@ -678,7 +701,11 @@ static void ens_rows(qt_ens* e, double tStart, const double* intervals, int n_in
   int flag = 0;
   hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
   hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-  if (flag) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+  // the reference throws at the first non-finite rate (LiouvilleanAveragedCommon.h:82-93, via
+  // Liouvillean::rates) or jump norm (MCWF_Trajectory.h:338), and at non-finite streamed averages
+  if (flag & 2) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+  if (flag & 1) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+  for (double v : m) if (!std::isfinite(v)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
   if (rows) {
     double tt = tStart;
     for (int k = 0; k < n_int; ++k) {
@@ -716,6 +743,7 @@ int qt_mcwf_reset(qt_ens* e, qt_stream stream) {
     hipCheck(hipMemsetAsync(e->dtDid.p, 0, n * sizeof(double), st), "reset dtDid");
     hipCheck(hipMemsetAsync(e->ord.p, 0, n * sizeof(unsigned long long), st), "reset ordinal");
     hipCheck(hipMemsetAsync(e->cnt.p, 0, n * qt::kCounters * sizeof(long long), st), "reset counters");
+    hipCheck(qt::launch_fill_int(e->errflag.p, 1, 0, st), "reset error flag");
     e->time = 0.;
   });
 }
@@ -851,6 +879,9 @@ int qt_mcwf_set_states(qt_ens* e, int first, int count, const qt_traj_state* st,
                        hipMemcpyHostToDevice), "H2D");
     if (cap && jt) hipCheck(hipMemcpy(e->jt.p + (size_t)first * cap, jt, n * cap * sizeof(double), hipMemcpyHostToDevice), "H2D");
     if (cap && jch) hipCheck(hipMemcpy(e->jch.p + (size_t)first * cap, jch, n * cap * sizeof(int), hipMemcpyHostToDevice), "H2D");
+    // a resumed ensemble starts without the failure of an earlier launch
+    const int zero = 0;
+    hipCheck(hipMemcpy(e->errflag.p, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D");
     if (whole) e->time = t[0];
   });
 }
@@ -1138,24 +1169,16 @@ extern "C" {
 int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_params* p, const double* rho0,
                      qt_master** out) {
   return guard([&] {
-    if (!systems || !p || !rho0 || !out || n_points <= 0) throw Fail(QT_EINVAL, "bad arguments");
-    if (!(p->epsRel > 0) || !(p->epsAbs >= 0)) throw Fail(QT_EINVAL, "tolerances must be positive");
+    if (!rho0 || !out) throw Fail(QT_EINVAL, "bad arguments");
+    check_master(systems, n_points, p);
     const qt::System& s0 = *systems[0]->h;
-    if (p->autostop_rep < 0 || (p->autostop_rep > 0 && !(p->autostop_eps > 0.)))
-      throw Fail(QT_EINVAL, "autostop needs a positive epsilon and a non-negative repetition");
-    if (p->autostop_rep > 0 && (int)s0.O.size() > qt::kMaxAutostopAvr)
-      throw Fail(QT_ENOTSUP, "autostop watches at most 64 averages");
     const int d = (int)s0.dim;
     const int NB = (d + 15) / 16, DP = 16 * NB, KC = DP / 4, NN = DP * DP;
     if (!qt::master_supported(NB)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
     const int nj = (int)s0.J.size();
     int wj = 1;
-    for (int k = 0; k < n_points; ++k) {
-      const qt::System& s = *systems[k]->h;
-      if (s.dim != d || (int)s.J.size() != nj) throw Fail(QT_EINVAL, "sweep points must share dimension and jump channels");
-      if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the Master sweep runs the Schroedinger picture only");
-      for (auto& j : s.J) wj = std::max(wj, j.width);
-    }
+    for (int k = 0; k < n_points; ++k)
+      for (auto& j : systems[k]->h->J) wj = std::max(wj, j.width);
     std::unique_ptr<qt_master> m(new qt_master);
     m->n = n_points; m->d = d; m->nb = NB; m->p = *p;
     m->sys0 = systems[0];
@@ -1418,6 +1441,7 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     if (nj) {
       auto r = host_rates(s, psi);
       double tot = 0.; for (double v : r) tot += v;
+      if (!std::isfinite(tot)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");  // LiouvilleanAveragedCommon.h:82-93
       dtTry = std::min(dtTry, p->dpLimit / tot);
     }
     unsigned long long ord = 0;
@@ -1533,6 +1557,9 @@ struct Request {
   qt_master* m = nullptr;
   hipStream_t st = nullptr;               // its own non-blocking stream while admitted
   std::vector<std::vector<double>> prow;  // per point, the streamed rows so far
+  bool admitted = false;
+  int status = QT_OK;                     // a failed request keeps its error code and message
+  std::string error;
   ~Request() {
     if (ens) qt_mcwf_destroy(ens);
     if (m) qt_master_destroy(m);
@@ -1562,8 +1589,24 @@ struct qt_server_s {
   std::map<long long, std::unique_ptr<Request>> all;
   size_t used = 0;
   qt_batch_s batch;
-  long long steps = 0, tries = 0, finished = 0;
+  long long steps = 0, tries = 0, finished = 0, failed = 0;
   double kernelMs = 0.;
+
+  // Retires r with an error: its device work is waited for and its memory freed; the loop and
+  // the other requests go on.  r must not be in `waiting` or `active` afterwards (the caller's job).
+  void fail(Request* r, int code, const std::string& msg) {
+    if (r->st) (void)hipStreamSynchronize(r->st);
+    if (r->ens) { qt_mcwf_destroy(r->ens); r->ens = nullptr; }
+    if (r->m) { qt_master_destroy(r->m); r->m = nullptr; }
+    if (r->st) { (void)hipStreamDestroy(r->st); r->st = nullptr; }
+    if (r->admitted) used -= r->bytes;
+    r->admitted = false;
+    r->status = code ? code : QT_EDEVICE;
+    r->error = msg;
+    r->rows.clear(); r->prow.clear();
+    r->tDone = now_s();
+    ++failed;
+  }
 };
 
 extern "C" {
@@ -1623,15 +1666,9 @@ int qt_server_submit_sweep(qt_server* s, qt_sys* const* systems, int n_points, c
                            double Dt, const qt_master_params* p, long long* id) {
   return guard([&] {
     if (!s || !systems || !rho0 || !p || n_points <= 0 || !(T > 0) || !(Dt > 0)) throw Fail(QT_EINVAL, "bad arguments");
-    for (int k = 0; k < n_points; ++k) if (!systems[k]) throw Fail(QT_EINVAL, "null system");
-    const qt::System& h = *systems[0]->h;
     // what qt_master_create would refuse is refused here, not at admission inside a step
-    for (int k = 0; k < n_points; ++k) {
-      const qt::System& hk = *systems[k]->h;
-      if (hk.dim != h.dim || hk.J.size() != h.J.size()) throw Fail(QT_EINVAL, "sweep points differ in dimension or jumps");
-      if (hk.H.timeDependent() || hk.hasEx) throw Fail(QT_ENOTSUP, "the request loop runs the Schroedinger picture only");
-    }
-    if (h.dim > 96) throw Fail(QT_ENOTSUP, "Master sweeps run dimensions up to 96");
+    check_master(systems, n_points, p);
+    const qt::System& h = *systems[0]->h;
     std::unique_ptr<Request> r(new Request);
     r->id = s->nextId++;
     r->sweep = true;
@@ -1657,25 +1694,33 @@ int qt_server_submit_sweep(qt_server* s, qt_sys* const* systems, int n_points, c
 // One continuous-batching iteration: admit waiting requests in arrival order while slots and
 // the memory budget allow (a request that does not fit lets later, smaller ones pass), advance
 // every admitted request by its next chunk of stream intervals in ONE mixed launch, append the
-// streamed rows, retire finished requests and free their memory.
+// streamed rows, retire finished requests and free their memory.  A request that fails (at
+// admission, or a non-finite state in its part of a launch) is retired with its error; the
+// others go on.
 int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
   return guard([&] {
     if (!s) throw Fail(QT_EINVAL, "null argument");
+    auto code_of = [](const std::exception& ex) {
+      const Fail* f = dynamic_cast<const Fail*>(&ex);
+      return f ? f->code : QT_EINVAL;
+    };
     for (auto it = s->waiting.begin(); it != s->waiting.end();) {
       Request* r = *it;
       if ((int)s->active.size() >= s->P.max_active) break;
       if (s->used + r->bytes > s->P.hbm_budget) { ++it; continue; }
       int rc = r->sweep ? qt_master_create(r->systems.data(), r->n, &r->mp, r->rho0.data(), &r->m)
                         : qt_mcwf_create(r->sys, r->n, &r->p, r->psi0.data(), &r->ens);
-      if (rc) throw Fail(rc, g_err);
+      it = s->waiting.erase(it);
+      if (rc) { s->fail(r, rc, g_err); continue; }
       r->tAdmit = now_s();
+      r->admitted = true;
       s->used += r->bytes;
       s->active.push_back(r);
-      it = s->waiting.erase(it);
     }
     if (!s->active.empty()) {
       std::vector<Request*> ens, sw;
       for (auto* r : s->active) (r->sweep ? sw : ens).push_back(r);
+      std::vector<Request*> dead;  // failed during this step
       const size_t na = ens.size();
       std::vector<const double*> ivp(na);
       std::vector<int> ni(na);
@@ -1687,79 +1732,117 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
         t0[m] = r->ens->time;
       }
       // the ensembles in one mixed launch (asynchronous), the sweeps beside it on their own stream
+      bool launched = false;
       if (na) {
-        s->batch.mem.clear();
-        for (auto* r : ens) s->batch.mem.push_back(r->ens);
-        s->batch.choose(-1, nullptr);
-        batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
-      }
-      // every sweep launched on its own non-blocking stream before anything is waited for
-      std::vector<int> nks(sw.size());
-      for (size_t i = 0; i < sw.size(); ++i) {
-        Request* r = sw[i];
-        if (!r->st) hipCheck(hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-        nks[i] = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
-        master_launch(r->m, r->iv.data() + r->next, nks[i], r->st);
-      }
-      for (size_t i = 0; i < sw.size(); ++i) {
-        Request* r = sw[i];
-        const int nk = nks[i], no = (int)r->sys->h->O.size();
-        std::vector<double> rows((size_t)r->n * nk * (2 + no));
-        master_finish(r->m, r->iv.data() + r->next, nk, rows.data(), nullptr, r->st);
-        for (int k = 0; k < r->n; ++k)
-          r->prow[k].insert(r->prow[k].end(), rows.begin() + (size_t)k * nk * (2 + no),
-                            rows.begin() + (size_t)(k + 1) * nk * (2 + no));
-        r->next += nk;
-      }
-      if (na) {
-        batch_check(&s->batch, nullptr);
-        for (size_t m = 0; m < na; ++m) {
-          Request* r = ens[m];
-          const int no = (int)r->sys->h->O.size();
-          std::vector<double> rows((size_t)ni[m] * (2 + no));
-          for (int k = 0; k < ni[m]; ++k) r->ens->time += ivp[m][k];
-          ens_rows(r->ens, t0[m], ivp[m], ni[m], rows.data(), nullptr);
-          r->rows.insert(r->rows.end(), rows.begin(), rows.end());
-          r->next += ni[m];
+        try {
+          s->batch.mem.clear();
+          for (auto* r : ens) s->batch.mem.push_back(r->ens);
+          s->batch.choose(-1, nullptr);
+          batch_launch(&s->batch, ivp.data(), ni.data(), nullptr);
+          launched = true;
+        } catch (std::exception& ex) {
+          (void)hipDeviceSynchronize();
+          for (auto* r : ens) { s->fail(r, code_of(ex), ex.what()); dead.push_back(r); }
         }
       }
-      if (na) {
-        float ms = 0.f;
-        hipCheck(hipEventElapsedTime(&ms, s->batch.e0, s->batch.e1), "hipEventElapsedTime");
-        s->kernelMs += ms;
+      // every sweep launched on its own non-blocking stream before anything is waited for
+      std::vector<int> nks(sw.size(), 0);
+      for (size_t i = 0; i < sw.size(); ++i) {
+        Request* r = sw[i];
+        try {
+          if (!r->st) hipCheck(hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+          nks[i] = (int)std::min<size_t>(s->P.chunk_intervals, r->iv.size() - r->next);
+          master_launch(r->m, r->iv.data() + r->next, nks[i], r->st);
+        } catch (std::exception& ex) {
+          s->fail(r, code_of(ex), ex.what());
+          dead.push_back(r);
+          nks[i] = 0;
+        }
+      }
+      for (size_t i = 0; i < sw.size(); ++i) {
+        Request* r = sw[i];
+        if (!nks[i]) continue;
+        try {
+          const int nk = nks[i], no = (int)r->sys->h->O.size();
+          std::vector<double> rows((size_t)r->n * nk * (2 + no));
+          master_finish(r->m, r->iv.data() + r->next, nk, rows.data(), nullptr, r->st);
+          for (double v : rows) if (!std::isfinite(v)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+          for (int k = 0; k < r->n; ++k)
+            r->prow[k].insert(r->prow[k].end(), rows.begin() + (size_t)k * nk * (2 + no),
+                              rows.begin() + (size_t)(k + 1) * nk * (2 + no));
+          r->next += nk;
+        } catch (std::exception& ex) {
+          s->fail(r, code_of(ex), ex.what());
+          dead.push_back(r);
+        }
+      }
+      if (launched) {
+        bool ok = true;
+        try {
+          batch_check(&s->batch, nullptr);
+        } catch (std::exception& ex) {  // a team barrier of the launch timed out: every member is suspect
+          for (auto* r : ens) { s->fail(r, code_of(ex), ex.what()); dead.push_back(r); }
+          ok = false;
+        }
+        for (size_t m = 0; ok && m < na; ++m) {
+          Request* r = ens[m];
+          try {
+            const int no = (int)r->sys->h->O.size();
+            std::vector<double> rows((size_t)ni[m] * (2 + no));
+            for (int k = 0; k < ni[m]; ++k) r->ens->time += ivp[m][k];
+            ens_rows(r->ens, t0[m], ivp[m], ni[m], rows.data(), nullptr);
+            r->rows.insert(r->rows.end(), rows.begin(), rows.end());
+            r->next += ni[m];
+          } catch (std::exception& ex) {
+            s->fail(r, code_of(ex), ex.what());
+            dead.push_back(r);
+          }
+        }
+        if (ok) {
+          float ms = 0.f;
+          hipCheck(hipEventElapsedTime(&ms, s->batch.e0, s->batch.e1), "hipEventElapsedTime");
+          s->kernelMs += ms;
+        }
       }
       ++s->steps;
       // retire
       std::vector<Request*> keep;
       for (auto* r : s->active) {
+        if (std::find(dead.begin(), dead.end(), r) != dead.end()) continue;
         if (r->next < r->iv.size()) { keep.push_back(r); continue; }
-        if (r->sweep) {
-          std::vector<int> done(r->n);
-          std::vector<double> tss(r->n), tt(r->n);
-          std::vector<long long> c((size_t)r->n * qt::kCounters);
-          int rc = qt_master_status(r->m, done.data(), tss.data(), tt.data(), c.data());
-          if (rc) throw Fail(rc, g_err);
-          r->tries = 0;
-          for (int k = 0; k < r->n; ++k) r->tries += c[(size_t)k * qt::kCounters + qt::kTries];
-          double ms = 0.; int nl = 0;
-          qt_master_kernel_ms(r->m, &ms, &nl);
-          s->kernelMs += ms;
-          qt_master_destroy(r->m);
-          r->m = nullptr;
-          if (r->st) hipCheck(hipStreamDestroy(r->st), "hipStreamDestroy");
-          r->st = nullptr;
-          for (auto& pr : r->prow) r->rows.insert(r->rows.end(), pr.begin(), pr.end());  // point-major
-          r->prow.clear();
-        } else {
-          long long tot[qt::kCounters];
-          int rc = qt_mcwf_counters(r->ens, nullptr, tot);
-          if (rc) throw Fail(rc, g_err);
-          r->tries = tot[qt::kTries];
-          qt_mcwf_destroy(r->ens);
-          r->ens = nullptr;
+        try {
+          if (r->sweep) {
+            std::vector<int> done(r->n);
+            std::vector<double> tss(r->n), tt(r->n);
+            std::vector<long long> c((size_t)r->n * qt::kCounters);
+            int rc = qt_master_status(r->m, done.data(), tss.data(), tt.data(), c.data());
+            if (rc) throw Fail(rc, g_err);
+            r->tries = 0;
+            for (int k = 0; k < r->n; ++k) r->tries += c[(size_t)k * qt::kCounters + qt::kTries];
+            double ms = 0.; int nl = 0;
+            qt_master_kernel_ms(r->m, &ms, &nl);
+            s->kernelMs += ms;
+            qt_master_destroy(r->m);
+            r->m = nullptr;
+            if (r->st) hipCheck(hipStreamDestroy(r->st), "hipStreamDestroy");
+            r->st = nullptr;
+            for (auto& pr : r->prow) r->rows.insert(r->rows.end(), pr.begin(), pr.end());  // point-major
+            r->prow.clear();
+          } else {
+            long long tot[qt::kCounters];
+            int rc = qt_mcwf_counters(r->ens, nullptr, tot);
+            if (rc) throw Fail(rc, g_err);
+            r->tries = tot[qt::kTries];
+            qt_mcwf_destroy(r->ens);
+            r->ens = nullptr;
+          }
+        } catch (std::exception& ex) {
+          s->fail(r, code_of(ex), ex.what());
+          continue;
         }
         s->tries += r->tries;
         s->used -= r->bytes;
+        r->admitted = false;
         r->tDone = now_s();
         ++s->finished;
       }
@@ -1778,6 +1861,11 @@ int qt_server_result(qt_server* s, long long id, double* rows, int cap_rows, int
     if (it == s->all.end()) throw Fail(QT_EINVAL, "unknown request id");
     Request* r = it->second.get();
     if (r->tDone == 0.) throw Fail(QT_EINVAL, "request not finished");
+    if (latency) *latency = r->tDone - r->tSubmit;
+    if (r->status != QT_OK) {
+      if (n_rows) *n_rows = 0;
+      throw Fail(r->status, "request " + std::to_string(id) + " failed: " + r->error);
+    }
     const int w = 2 + (int)r->sys->h->O.size();
     const int n = (int)(r->rows.size() / w);
     if (n_rows) *n_rows = n;
@@ -1802,6 +1890,7 @@ int qt_server_stats(const qt_server* s, qt_server_stats_t* st) {
   if (!s || !st) return QT_EINVAL;
   st->steps = s->steps;
   st->finished = s->finished;
+  st->failed = s->failed;
   st->tries = s->tries;
   st->kernel_ms = s->kernelMs;
   st->active = (int)s->active.size();

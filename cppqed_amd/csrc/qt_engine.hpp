@@ -1,4 +1,7 @@
-// qt_mcwf.hip -- MCWF kernels for gfx950 (CDNA4).
+// qt_engine.hpp -- MCWF trajectory engines and persistent kernels (templates) for gfx950 (CDNA4).
+//
+// Included by the translation units that instantiate them (qt_batch*.hip, qt_mixed.hip,
+// qt_aux.hip), which are compiled in parallel.
 //
 // Batched persistent kernel: one work group ("group") owns one trajectory at a
 // time and advances it through every requested stream interval.  A group is
@@ -16,6 +19,8 @@
 //
 // Work distribution: a global atomic counter hands trajectories to groups, so
 // uneven trajectory costs (jumps, step-backs, rejected tries) balance out.
+
+#pragma once
 
 #include "qt_device.hpp"
 #include "qt_kernels.hpp"
@@ -648,6 +653,9 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
         for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
           double totalRate = 0.;
           for (int m = 0; m < nj; ++m) totalRate += rts[m];
+          // Liouvillean::rates goes through LiouvilleanAveragedCommon::average, which refuses
+          // non-finite values (LiouvilleanAveragedCommon.h:82-93): flag the launch
+          if (!isfinite(totalRate) && lane == 0) atomicOr(E.err_flag, 2);
           const double sugg = E.dpLimit / totalRate;
           if (totalRate * dtDid > E.overshootTol * E.dpLimit) {
             t = tCache; dtDid = dtDidCache; dtTry = sugg;
@@ -669,7 +677,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
         if (random < 0.) {
           --no;
           const double nf = sqrt(rts[no]);
-          if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
+          if (!isfinite(nf) && lane == 0) atomicOr(E.err_flag, 1);
           const double inv = 1.0 / nf;
           double2 y[ROWS];
 #pragma unroll
@@ -862,6 +870,9 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
         for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
           double totalRate = 0.;
           for (int m = 0; m < nj; ++m) totalRate += rts[m];
+          // Liouvillean::rates goes through LiouvilleanAveragedCommon::average, which refuses
+          // non-finite values (LiouvilleanAveragedCommon.h:82-93): flag the launch
+          if (!isfinite(totalRate) && lane == 0) atomicOr(E.err_flag, 2);
           const double sugg = E.dpLimit / totalRate;
           if (totalRate * dtDid > E.overshootTol * E.dpLimit) {
             t = tCache; dtDid = dtDidCache; dtTry = sugg;
@@ -887,7 +898,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
         if (random < 0.) {
           --no;
           const double nf = sqrt(rts[no]);
-          if (!isfinite(nf) && lane == 0) atomicExch(E.err_flag, 1);
+          if (!isfinite(nf) && lane == 0) atomicOr(E.err_flag, 1);
           const double inv = jumpScale / nf;  // ps holds psi before the renorm
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
@@ -1116,158 +1127,9 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   }
 }
 
-// jump draws for the next `cap` MCWF steps of every trajectory, one Philox evaluation per lane:
-// draws[traj][i] = jump_uniform(seed, traj0 + traj, ordinal[traj] + i)
-__global__ void jump_draws_kernel(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
-                                  int n_traj, int cap, double* draws) {
-  const size_t total = (size_t)n_traj * cap;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    const int traj = (int)(i / cap), k = (int)(i % cap);
-    draws[i] = jump_uniform(seed, traj0 + (unsigned long long)traj, ordinal[traj] + (unsigned long long)k);
-  }
-}
-
-hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
-                             int n_traj, int cap, double* draws, hipStream_t st) {
-  const size_t total = (size_t)n_traj * cap;
-  const int grid = (int)std::min<size_t>((total + 255) / 256, 16384);
-  hipLaunchKernelGGL(jump_draws_kernel, dim3(grid), dim3(256), 0, st, ordinal, seed, traj0, n_traj, cap, draws);
-  return hipGetLastError();
-}
-
-// ensemble mean over trajectories of the streamed raw averages (deterministic order):
-// out[k][q] = (1/N) sum_traj obs[traj][k][q], out_dt[k] = (1/N) sum dtDid
-__global__ void ensemble_mean_kernel(const double* obs, const double* dtd, int n_traj, int n_int, int no,
-                                     double* out, double* out_dt) {
-  const int k = blockIdx.x;
-  const int q = blockIdx.y;  // q == no: dtDid
-  __shared__ double red[256];
-  double s = 0.;
-  for (int i = threadIdx.x; i < n_traj; i += blockDim.x)
-    s += q < no ? obs[((size_t)i * n_int + k) * no + q] : dtd[(size_t)i * n_int + k];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (q < no) out[k * no + q] = red[0] / (double)n_traj;
-    else out_dt[k] = red[0] / (double)n_traj;
-  }
-}
-
 // ---------------------------------------------------------------------------
-// plain form: one trajectory, one launch per RKCK try / per post-step phase
+// compiled variants
 // ---------------------------------------------------------------------------
-
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_try_kernel(SysDev S, PlainTry P) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS], xn[ROWS], k1[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? P.x[r * G + threadIdx.x] : make_double2(0., 0.);
-  long long derivs = 0;
-  eng.derivative(x, k1);  // controlled_runge_kutta::try_step_v1 recomputes dxdt on every try
-  const double err = sqrt(eng.tryStep(x, k1, P.dt, P.epsAbs, P.epsRel, xn, derivs));  // tryStep: squared norm
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) P.xn[r * G + threadIdx.x] = xn[r];
-  if (threadIdx.x == 0) *P.err = err;
-}
-
-// renormalise psi in place and compute the jump rates: out[0] = norm, out[1..nj] = rates
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_post_kernel(SysDev S, double2* psi, double* out) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-  double s = 0.;
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) { x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.); s += cabs2(x[r]); }
-  const double nrm = sqrt(eng.gsum(s));
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) { x[r].x /= nrm; x[r].y /= nrm; if (eng.valid(r)) psi[r * G + threadIdx.x] = x[r]; }
-  eng.stagePsi(x);
-  double rt[kMaxJumps];
-  eng.rates(rt);
-  if (threadIdx.x == 0) { out[0] = nrm; for (int m = 0; m < S.nj; ++m) out[1 + m] = rt[m]; }
-}
-
-// psi <- J_m psi / nf
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_jump_kernel(SysDev S, double2* psi, int m, double nf) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
-  eng.stagePsi(x);
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) { double2 y = eng.jumpRow(m, r); psi[r * G + threadIdx.x] = make_double2(y.x / nf, y.y / nf); }
-}
-
-// observables of one state: out[q]
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_obs_kernel(SysDev S, const double2* psi, double* out) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
-  eng.stagePsi(x);
-  for (int q = 0; q < S.no; ++q) { double v = eng.observable(q, x); if (threadIdx.x == 0) out[q] = v; }
-}
-
-// dpsidt += (H_nH/i)(t - t0) psi for a batch of state vectors, one group per vector (qt_apply_heff)
-template <int ROWS, int NW, int WH, bool F32, bool TD>
-__global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const double2* psi, double2* dpsidt, int n,
-                                                             double tau) {
-  using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
-  using T = typename Eng::T;
-  constexpr int G = Eng::G;
-  __shared__ T xs[ROWS * G];
-  __shared__ double2 ps[1];
-  __shared__ double red[NW > 1 ? NW : 1];
-  __shared__ T ph[kMaxFreqs];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x, ph);
-  for (int v = blockIdx.x; v < n; v += gridDim.x) {
-    double2 x[ROWS];
-    T k[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[(size_t)v * S.dim + r * G + threadIdx.x] : make_double2(0., 0.);
-    eng.derivative(x, k, tau);
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r)
-      if (eng.valid(r)) {
-        double2& o = dpsidt[(size_t)v * S.dim + r * G + threadIdx.x];
-        const double2 kd = to_d(k[r]);
-        o.x += kd.x; o.y += kd.y;
-      }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// launch table
-// ---------------------------------------------------------------------------
-
-struct Layout { int rows, nw, wh; };
 
 // Every (rows, waves, ELL width) combination compiled into the library.
 #define QT_LAYOUTS(X) \
@@ -1298,183 +1160,50 @@ struct Layout { int rows, nw, wh; };
 #define QT_BATCH_GSL(X) \
   X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1)
 
-bool layout_supported(int rows, int nw, int wh) {
-#define QT_CASE(R, N, W) if (rows == R && nw == N && wh == W) return true;
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return false;
-}
-
-bool batch_supported(int rows, int nw, int wh, int minw, bool td, bool gsl) {
-#define QT_CASE(R, N, W, M) if (rows == R && nw == N && wh == W && minw == M) return true;
-  if (gsl) { QT_BATCH_GSL(QT_CASE) }
-  else if (td) { QT_BATCH_TD(QT_CASE) }
-  else { QT_BATCH(QT_CASE) }
-#undef QT_CASE
-  return false;
-}
-
-// resident work groups of a batched variant per compute unit (registers and LDS permitting)
-int batch_blocks_per_cu(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl) {
-  int n = 0;
-#define QT_CASE_T(R, N, W, M, TDV, GV)                                                                               \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                               \
+// One launch table over a list of batched variants: NAME##_blocks (resident work groups per
+// compute unit, registers and LDS permitting) and NAME##_launch.  Each list is instantiated in
+// one translation unit.
+#define QT_BATCH_TABLE(NAME, LIST, TDV, GV)                                                                       \
+  int NAME##_blocks(int rows, int nw, int wh, int minw, bool f32) {                                               \
+    int n = 0;                                                                                                    \
+    QT_TABLE_CASES_B(LIST, TDV, GV)                                                                               \
+    return 0;                                                                                                     \
+  }                                                                                                               \
+  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E, \
+                           hipStream_t st) {                                                                      \
+    QT_TABLE_CASES_L(LIST, TDV, GV)                                                                               \
+    return hipErrorInvalidValue;                                                                                  \
+  }
+#define QT_TABLE_CASES_B(LIST, TDV, GV) LIST(QT_TABLE_B_##TDV##_##GV)
+#define QT_TABLE_CASES_L(LIST, TDV, GV) LIST(QT_TABLE_L_##TDV##_##GV)
+#define QT_TABLE_B(R, N, W, M, TDV, GV)                                                                           \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                             \
     if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M, TDV, GV>, 64 * N, 0); \
     else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M, TDV, GV>, 64 * N, 0);    \
-    return n;                                                                                                       \
+    return n;                                                                                                     \
   }
-#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false, false)
-#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true, false)
-#define QT_CASE_G(R, N, W, M) QT_CASE_T(R, N, W, M, false, true)
-#define QT_CASE_GTD(R, N, W, M) QT_CASE_T(R, N, W, M, true, true)
-  if (gsl && td) { QT_BATCH_GSL(QT_CASE_GTD) }
-  else if (gsl) { QT_BATCH_GSL(QT_CASE_G) }
-  else if (td) { QT_BATCH_TD(QT_CASE_TD) }
-  else { QT_BATCH(QT_CASE) }
-#undef QT_CASE
-#undef QT_CASE_TD
-#undef QT_CASE_G
-#undef QT_CASE_GTD
-#undef QT_CASE_T
-  return 0;
-}
-
-hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl, int grid, const SysDev& S,
-                        const EnsDev& E, hipStream_t st) {
-#define QT_CASE_T(R, N, W, M, TDV, GV)                                                                              \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                               \
+#define QT_TABLE_L(R, N, W, M, TDV, GV)                                                                           \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                             \
     if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E); \
     else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
-    return hipGetLastError();                                                                                       \
+    return hipGetLastError();                                                                                     \
   }
-#define QT_CASE(R, N, W, M) QT_CASE_T(R, N, W, M, false, false)
-#define QT_CASE_TD(R, N, W, M) QT_CASE_T(R, N, W, M, true, false)
-#define QT_CASE_G(R, N, W, M) QT_CASE_T(R, N, W, M, false, true)
-#define QT_CASE_GTD(R, N, W, M) QT_CASE_T(R, N, W, M, true, true)
-  if (gsl && td) { QT_BATCH_GSL(QT_CASE_GTD) }
-  else if (gsl) { QT_BATCH_GSL(QT_CASE_G) }
-  else if (td) { QT_BATCH_TD(QT_CASE_TD) }
-  else { QT_BATCH(QT_CASE) }
-#undef QT_CASE
-#undef QT_CASE_TD
-#undef QT_CASE_G
-#undef QT_CASE_GTD
-#undef QT_CASE_T
-  return hipErrorInvalidValue;
-}
+#define QT_TABLE_B_false_false(R, N, W, M) QT_TABLE_B(R, N, W, M, false, false)
+#define QT_TABLE_B_true_false(R, N, W, M) QT_TABLE_B(R, N, W, M, true, false)
+#define QT_TABLE_B_false_true(R, N, W, M) QT_TABLE_B(R, N, W, M, false, true)
+#define QT_TABLE_B_true_true(R, N, W, M) QT_TABLE_B(R, N, W, M, true, true)
+#define QT_TABLE_L_false_false(R, N, W, M) QT_TABLE_L(R, N, W, M, false, false)
+#define QT_TABLE_L_true_false(R, N, W, M) QT_TABLE_L(R, N, W, M, true, false)
+#define QT_TABLE_L_false_true(R, N, W, M) QT_TABLE_L(R, N, W, M, false, true)
+#define QT_TABLE_L_true_true(R, N, W, M) QT_TABLE_L(R, N, W, M, true, true)
 
-// Mixed-batch variants: (waves per work group, rows per lane of the work-group team, its ELL
-// register width, rows per lane of the wavefront team, its ELL register width).
-#define QT_MIXED(X) \
-  X(16, 4, 16, 1, 8) X(16, 4, 16, 2, 8) X(16, 2, 16, 1, 8) X(16, 1, 16, 1, 8)
-
-bool mixed_supported(int nwb, int rb, int whb, int rw, int whw) {
-#define QT_CASE(A, B, C, D, F) if (nwb == A && rb == B && whb == C && rw == D && whw == F) return true;
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return false;
-}
-
-// compiled mixed variants as rows of 5 ints; returns their number
-int mixed_variants(int* table, int cap) {
-  int n = 0;
-#define QT_CASE(A, B, C, D, F) \
-  if (n < cap && table) { int* r = table + 5 * n; r[0] = A; r[1] = B; r[2] = C; r[3] = D; r[4] = F; } ++n;
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return n;
-}
-
-int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32) {
-  int n = 0;
-#define QT_CASE(A, B, C, D, F)                                                                                      \
-  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                     \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, true>, 64 * A, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, false>, 64 * A, 0);   \
-    return n;                                                                                                       \
-  }
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return 0;
-}
-
-hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st) {
-#define QT_CASE(A, B, C, D, F)                                                                                          \
-  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                         \
-    if (f32) hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, true>), dim3(grid), dim3(64 * A), 0, st, M);          \
-    else hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, false>), dim3(grid), dim3(64 * A), 0, st, M);             \
-    return hipGetLastError();                                                                                           \
-  }
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-// p[0..n) = v  (a kernel rather than a memset node, so that the launch sequence can be captured)
-__global__ void fill_int_kernel(int* p, int n, int v) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
-
-hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st) {
-  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, p, n, v);
-  return hipGetLastError();
-}
-
-hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
-                                double* out_dt, hipStream_t st) {
-  hipLaunchKernelGGL(ensemble_mean_kernel, dim3(n_int, no + 1), dim3(256), 0, st, obs, dtd, n_traj, n_int, no, out, out_dt);
-  return hipGetLastError();
-}
-
-hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_try_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, P); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_post_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, out); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_jump_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, m, nf); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_obs_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, out); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const SysDev& S, const double2* psi,
-                             double2* dpsidt, int n, double tau, hipStream_t st) {
-  const int grid = n < 4096 ? n : 4096;
-#define QT_CASE(R, N, W)                                                                                      \
-  if (rows == R && nw == N && wh == W) {                                                                      \
-    if (td) {                                                                                                 \
-      if (f32) hipLaunchKernelGGL((apply_heff_kernel<R, N, W, true, true>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau); \
-      else hipLaunchKernelGGL((apply_heff_kernel<R, N, W, false, true>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau);    \
-    } else {                                                                                                  \
-      if (f32) hipLaunchKernelGGL((apply_heff_kernel<R, N, W, true, false>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau); \
-      else hipLaunchKernelGGL((apply_heff_kernel<R, N, W, false, false>), dim3(grid), dim3(64 * N), 0, st, S, psi, dpsidt, n, tau);    \
-    }                                                                                                         \
-    return hipGetLastError();                                                                                 \
-  }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
+#define QT_TABLE_DECL(NAME)                                                                                        \
+  int NAME##_blocks(int rows, int nw, int wh, int minw, bool f32);                                                 \
+  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E, \
+                           hipStream_t st);
+QT_TABLE_DECL(batch_sch)
+QT_TABLE_DECL(batch_td)
+QT_TABLE_DECL(batch_gsl)
+QT_TABLE_DECL(batch_gsl_td)
 
 }  // namespace qt

@@ -72,7 +72,7 @@ struct EnsDev {
   double* dtdid_out;            // [n_traj][n_intervals]
   int* work;                    // work-queue head (trajectories are claimed in order)
   int gsl;                      // 1: GSL's controller (kernel variant chosen on the host), 0: odeint's
-  int* err_flag;                // set to 1 when a trajectory hits a non-finite jump norm
+  int* err_flag;                // bit 0: a non-finite jump norm; bit 1: non-finite jump rates
   const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals ordinal[traj] + i (may be null)
   int draw_cap;
 };

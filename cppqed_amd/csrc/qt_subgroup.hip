@@ -298,6 +298,7 @@ __global__ void __launch_bounds__(64) mcwf_sub_kernel(SysDev S, EnsDev E) {
       __syncthreads();
       double totalRate = 0.;
       for (int m = 0; m < nj; ++m) totalRate += rts[g][m];
+      if (accepted && !isfinite(totalRate) && j == 0) atomicOr(E.err_flag, 2);  // LiouvilleanAveragedCommon.h:82-93
       const double sugg = E.dpLimit / totalRate;
       const bool stepBack = accepted && totalRate * dtDid > E.overshootTol * E.dpLimit;
       if (stepBack) {
@@ -324,7 +325,7 @@ __global__ void __launch_bounds__(64) mcwf_sub_kernel(SysDev S, EnsDev E) {
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) y[r] = jumpRow(no, r);
         if (jump) {
-          if (!isfinite(nf) && j == 0) atomicExch(E.err_flag, 1);
+          if (!isfinite(nf) && j == 0) atomicOr(E.err_flag, 1);
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) x[r] = make_double2(y[r].x / nf, y[r].y / nf);
           if (j == 0) {

@@ -210,6 +210,7 @@ typedef struct {
   double kernel_ms;
   int active, waiting;
   size_t bytes_in_use;
+  long long failed;        /* requests retired with an error (qt_server_result returns it) */
 } qt_server_stats_t;
 
 int qt_server_create(const qt_server_params* p, qt_server** out);
@@ -219,10 +220,15 @@ int qt_server_destroy(qt_server* s);
  * the request.  Returns the request id. */
 int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0_host, double T, double Dt,
                      unsigned long long seed, long long* id);
-/* One iteration of the loop (admit, one launch, retire); reports active and waiting counts. */
+/* One iteration of the loop (admit, one launch, retire); reports active and waiting counts.
+ * A request that fails -- at admission (qt_mcwf_create / qt_master_create refuse it, e.g. a
+ * non-finite initial rate) or in its part of a launch (a non-finite rate, jump norm or streamed
+ * average, the reference's "Infinite detected ..." exceptions) -- is retired with that error and
+ * its memory freed; the other requests and the loop go on. */
 int qt_server_step(qt_server* s, int* n_active, int* n_waiting);
 /* Rows (time, mean dtDid, processed averages) of a finished request, its latency (submit to
- * retirement, seconds), its wait before admission and its RKCK tries. */
+ * retirement, seconds), its wait before admission and its RKCK tries.  For a failed request it
+ * returns the request's error code and message (latency still reported, *n_rows = 0). */
 int qt_server_result(qt_server* s, long long id, double* rows, int cap_rows, int* n_rows, double* latency,
                      double* queue_wait, long long* tries);
 /* Forgets a finished request. */

@@ -853,7 +853,9 @@ std::vector<double> rates(System& s, const cd* psi) {
       r.push_back(acc);
     }
   }
-  for (double x : r) if (!std::isfinite(x)) throw std::runtime_error("Non-finite rate");
+  // Liouvillean::rates passes LiouvilleanAveragedCommon::average, which refuses non-finite values
+  // (LiouvilleanAveragedCommon.h:82-93)
+  for (double x : r) if (!std::isfinite(x)) throw std::runtime_error("Infinite detected in averages");
   return r;
 }
 

@@ -99,3 +99,31 @@ def test_large_states_with_offset_gathers_match_oracle(name):
         assert np.linalg.norm(psi - res["psi"]) <= 1e-9 * np.linalg.norm(res["psi"]), k
         for c in (0, 1, 2, 4, 5, 7):
             assert per[k][c] == res["counters"][c], (k, c, per[k][c], res["counters"][c])
+
+
+def test_non_finite_state_is_reported():
+    """The reference's numeric-failure exceptions.  A non-finite initial state is refused when the
+    ensemble is made (its rates: LiouvilleanAveragedCommon.h:82-93 via the MCWF_Trajectory
+    constructor, MCWF_Trajectory.h:114-117).  A non-finite state loaded into one trajectory (a
+    host-forced checkpoint) makes the next advance fail with QT_ENUMERIC "Infinite detected in
+    averages" (the rates of that trajectory's first step) -- the kernel itself finishes: with
+    non-finite rates no jump fires and the controller only grows dt.  A reset clears the failure,
+    and the ensemble then runs as new."""
+    print("[gpu] non-finite state", flush=True)
+    s = Q.System(JC60)
+    bad = ground(60)
+    bad[5] = np.nan   # a photon-number state the mode's jump operator reads
+    with pytest.raises(Q.QtError) as ex:
+        Q.Ensemble(s, 4, bad)
+    assert ex.value.code == Q.QT_ENUMERIC and "Infinite detected in averages" in str(ex.value)
+    e = Q.Ensemble(s, 8, ground(60), Q.mcwf_pars(seed=3))
+    ref = e.advance(np.full(5, 0.1))
+    e.reset()
+    st, psi, jt, jc = e.get_states()
+    psi[3, 7] = np.inf
+    e.set_states(st, psi, jt, jc)
+    with pytest.raises(Q.QtError) as ex:
+        e.advance(np.full(5, 0.1))
+    assert ex.value.code == Q.QT_ENUMERIC and "Infinite detected in averages" in str(ex.value)
+    e.reset()
+    np.testing.assert_array_equal(e.advance(np.full(5, 0.1)), ref)

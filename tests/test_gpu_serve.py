@@ -138,4 +138,55 @@ def test_sweep_request_refusals():
     rho20[0, 0] = 1
     with pytest.raises(Q.QtError):
         srv.submit_sweep([ip], rho20, T=1.0)                 # interaction picture
+    # the parameters qt_master_create refuses are refused at submission too
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([a], rho, T=1.0, epsRel=0.0)
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([a], rho, T=1.0, autostop=(1e-3, 1e-6, -1))
+    with pytest.raises(Q.QtError):
+        srv.submit_sweep([a], rho, T=1.0, autostop=(0.0, 1e-6, 5))
+    with pytest.raises(ValueError):
+        srv.submit_sweep([a], rho20, T=1.0)                  # rho0 of the wrong shape
+    with pytest.raises(ValueError):
+        srv.submit(a, 16, ground(20), T=1.0)                 # psi0 of the wrong size
     assert srv.step() == (0, 0)
+
+
+def test_failed_requests_are_retired_alone():
+    """A request that fails is retired with the reference's error and the others are unaffected:
+    one refused at admission (a non-finite initial rate: LiouvilleanAveragedCommon.h:82-93 throws
+    when the MCWF_Trajectory constructor computes the rates, MCWF_Trajectory.h:114-117), one that
+    fails inside the shared mixed launch (a system without jumps, so nothing is checked before the
+    run, whose streamed averages come out non-finite: "Infinite detected in averages").  The
+    neighbour's rows equal its rows alone, bit for bit, and the loop keeps serving."""
+    print("[gpu] serve failure containment", flush=True)
+    jc = Q.System(JC60)
+    nojump = Q.System("free mode cutoff=8 delta=-1 kappa=0 eta=1")
+    bad_psi = ground(60).copy()
+    bad_psi[1] = np.nan
+    srv = Q.Server(chunk_intervals=5)
+    rid = srv.submit(jc, 128, ground(60), T=1.0, seed=9)
+    alone = _drain(srv, [rid])[rid][0]
+    srv = Q.Server(chunk_intervals=5)
+    ids = [srv.submit(jc, 64, bad_psi, T=1.0, seed=8)]
+    nan8 = ground(8).copy()
+    nan8[2] = np.inf
+    ids.append(srv.submit(nojump, 32, nan8, T=1.0, seed=7))
+    rid = srv.submit(jc, 128, ground(60), T=1.0, seed=9)
+    for _ in range(100):
+        a, w = srv.step()
+        if a == 0 and w == 0:
+            break
+    for r in ids:
+        with pytest.raises(Q.QtError) as ex:
+            srv.result(r)
+        assert ex.value.code == Q.QT_ENUMERIC
+        assert "Infinite detected in averages" in str(ex.value)
+    np.testing.assert_array_equal(srv.result(rid)[0], alone)
+    st = srv.stats()
+    assert st["failed"] == 2 and st["finished"] == 1 and st["bytes_in_use"] == 0
+    # the loop goes on serving
+    rid2 = srv.submit(jc, 128, ground(60), T=1.0, seed=9)
+    np.testing.assert_array_equal(_drain(srv, [rid2])[rid2][0], alone)
+    for r in ids + [rid, rid2]:
+        srv.release(r)

@@ -291,3 +291,17 @@ def test_gsl_controller_accepts_every_step():
     res = O.mcwf_run(s, ground(2), O.params(controller="gsl", rng="philox"), O.run_spec("Dt", Dt=0.1, T=1.0))
     assert res["counters"][2] == 0
     assert res["counters"][4] == res["counters"][1]
+
+
+def test_non_finite_rates_raise_the_reference_error():
+    """A non-finite state: the rates pass LiouvilleanAveragedCommon::average, which throws
+    "Infinite detected in averages" (LiouvilleanAveragedCommon.h:82-93); the oracle and the
+    product's ensemble set-up (host tables, before the device is touched) both refuse it."""
+    import cppqed_amd as Q
+    bad = ground(60)
+    bad[5] = np.nan
+    with pytest.raises(RuntimeError, match="Infinite detected in averages"):
+        O.mcwf_run(O.System(JC60), bad, O.params(rng="philox"), O.run_spec("Dt", Dt=0.1, T=0.2))
+    with pytest.raises(Q.QtError, match="Infinite detected in averages") as ex:
+        Q.Ensemble(Q.System(JC60), 4, bad)
+    assert ex.value.code == Q.QT_ENUMERIC
