@@ -23,26 +23,35 @@ hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long lo
   return hipGetLastError();
 }
 
-// ensemble mean over trajectories of the streamed raw averages (deterministic order):
-// out[k][q] = (1/N) sum_traj obs[traj][k][q], out_dt[k] = (1/N) sum dtDid
-__global__ void ensemble_mean_kernel(const double* obs, const double* dtd, int n_traj, int n_int, int no,
-                                     double* out, double* out_dt) {
-  const int k = blockIdx.x;
-  const int q = blockIdx.y;  // q == no: dtDid
-  __shared__ double red[256];
-  double s = 0.;
-  for (int i = threadIdx.x; i < n_traj; i += blockDim.x)
-    s += q < no ? obs[((size_t)i * n_int + k) * no + q] : dtd[(size_t)i * n_int + k];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+// The streamed rows of a launch from its fixed-point stream sums (AverageTrajectories and
+// DensityOperatorStreamer, MCWF_Trajectory.h:406-420; the row format of Trajectory.h:428-437):
+// rows[k] = (t_start + intervals[0] + ... + intervals[k], mean dtDid, processed mean averages).
+// One thread per stream point; the time is summed in the host's order, the processing runs in
+// place in average order (Averaged::process), as the host form does.
+__global__ void stream_rows_kernel(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
+                                   const double* intervals, const int* pvar, const int* psqrt, double* rows) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_int) return;
+  double tt = t_start;
+  for (int j = 0; j <= k; ++j) tt += intervals[j];
+  double* r = rows + (size_t)k * (2 + no);
+  r[0] = tt;
+  for (int q = 0; q <= no; ++q) {
+    unsigned long long H = 0, L = 0;
+    for (int st = 0; st < kAccStripes; ++st) {
+      const unsigned long long* a = acc + (((size_t)st * n_int + k) * (no + 1) + q) * 2;
+      H += a[0];
+      L += a[1];
+    }
+    const double m = acc_mean(H, L, n_traj);
+    if (q == no) r[1] = m;
+    else r[2 + q] = m;
   }
-  if (threadIdx.x == 0) {
-    if (q < no) out[k * no + q] = red[0] / (double)n_traj;
-    else out_dt[k] = red[0] / (double)n_traj;
-  }
+  for (int q = 0; q < no; ++q)
+    if (pvar[q] >= 0) {
+      r[2 + q] -= r[2 + pvar[q]] * r[2 + pvar[q]];
+      if (psqrt[q]) r[2 + q] = sqrt(r[2 + q]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -162,9 +171,21 @@ hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
-                                double* out_dt, hipStream_t st) {
-  hipLaunchKernelGGL(ensemble_mean_kernel, dim3(n_int, no + 1), dim3(256), 0, st, obs, dtd, n_traj, n_int, no, out, out_dt);
+// p[0..n) = 0 for 64-bit words
+__global__ void zero_u64_kernel(unsigned long long* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t st) {
+  const int grid = (int)std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
+                              const double* intervals, const int* pvar, const int* psqrt, double* rows, hipStream_t st) {
+  hipLaunchKernelGGL(stream_rows_kernel, dim3((n_int + 63) / 64), dim3(64), 0, st, acc, n_int, no, n_traj, t_start,
+                     intervals, pvar, psqrt, rows);
   return hipGetLastError();
 }
 

@@ -35,8 +35,9 @@ int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32);
 int mixed_variants(int* table, int cap);
 hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
-hipError_t launch_ensemble_mean(const double* obs, const double* dtd, int n_traj, int n_int, int no, double* out,
-                                double* out_dt, hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
+                              const double* intervals, const int* pvar, const int* psqrt, double* rows, hipStream_t st);
 hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st);
 hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st);
 hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st);
@@ -211,7 +212,7 @@ struct qt_sys {
     if (h->H.timeDependent() && nfreq > qt::kMaxFreqs)
       throw Fail(QT_ENOTSUP, "interaction picture with more than 32 distinct frequencies");
   }
-  DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx, hoff;
+  DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx, hoff, pvar, psqrt;
   DevBuf<double2> hval, jval, oval, freq, exz;
   int nfreq = 0;          // distinct interaction-picture frequencies of H (0: Schroedinger picture)
   DevBuf<float2> hval32;
@@ -286,6 +287,11 @@ struct qt_sys {
     }
     if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); im.push_back(0); ow.push_back(0); }
     ocol.upload(c); oval.upload(v); oimag.upload(im); owid.upload(ow);
+    {  // Averaged::process rules of the streamed averages
+      std::vector<int> pv(std::max<size_t>(1, s.O.size()), -1), ps(std::max<size_t>(1, s.O.size()), 0);
+      for (size_t q = 0; q < s.processVarOf.size() && q < s.O.size(); ++q) { pv[q] = s.processVarOf[q]; ps[q] = s.processSqrt[q]; }
+      pvar.upload(pv); psqrt.upload(ps);
+    }
     // interaction picture: distinct entry frequencies (index 0 = 0) and per-entry indices,
     // column-major like the values; the Exact diagonal's exponents per basis state
     if (s.H.timeDependent()) {
@@ -319,6 +325,7 @@ struct qt_sys {
     dev.hoff = offs.empty() ? nullptr : hoff.p;
     dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
     dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p; dev.owid = owid.p;
+    dev.pvar = pvar.p; dev.psqrt = psqrt.p;
     uploaded = true;
   }
 };
@@ -330,7 +337,8 @@ struct qt_ens {
   LayoutChoice lay{};
   double time = 0.;           // common time of all trajectories (they all stop at each endTime)
   DevBuf<double2> psi, psiInit;
-  DevBuf<double> t, t0, dtDid, dtTry, dtTryInit, obs, dtd, mean, meanDt, intervals, jt, draws;
+  DevBuf<double> t, t0, dtDid, dtTry, dtTryInit, intervals, jt, draws, lastObs, rowsDev;
+  DevBuf<unsigned long long> acc;   // stream sums of the last launch (acc_add)
   DevBuf<unsigned long long> ord;
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
@@ -416,6 +424,16 @@ void check_master(qt_sys* const* systems, int n_points, const qt_master_params* 
   }
 }
 
+
+// The reference throws at the first non-finite rate (LiouvilleanAveragedCommon.h:82-93, via
+// Liouvillean::rates) or jump norm (MCWF_Trajectory.h:338), and at non-finite streamed averages;
+// the device flags them per launch (bits 0-1), and a streamed average too large for the
+// fixed-point stream sums (bit 2).
+void ens_check_flag(int flag) {
+  if (flag & 2) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+  if (flag & 1) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
+  if (flag & 4) throw Fail(QT_ENUMERIC, "a streamed average exceeds the range of the ensemble sums (2^43 / n_traj)");
+}
 
 // rates of a host state through the host ELL tables: ||J_m psi||^2
 std::vector<double> host_rates(const qt::System& s, const std::vector<qt::cd>& psi) {
@@ -539,6 +557,7 @@ int qt_apply_heff(qt_sys* sys, double t, double t0, const double* psi, double* d
 int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const double* psi0, qt_ens** out) {
   return guard([&] {
     if (!sys || !psi0 || !out || n_traj <= 0) throw Fail(QT_EINVAL, "bad arguments");
+    if (n_traj >= (1 << 23)) throw Fail(QT_ENOTSUP, "at most 2^23 - 1 trajectories per ensemble (range of the stream sums)");
     check_params(p);
     const qt::System& s = *sys->h;
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
@@ -624,12 +643,12 @@ static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hip
   for (int k = 0; k < n_int; ++k) if (!(intervals[k] >= 0) || !std::isfinite(intervals[k])) throw Fail(QT_EINVAL, "intervals must be finite and non-negative");
   e->intervals.alloc(n_int);
   hipCheck(hipMemcpyAsync(e->intervals.p, intervals, n_int * sizeof(double), hipMemcpyHostToDevice, st), "intervals H2D");
-  const int no = std::max(1, (int)s.O.size());
-  e->obs.alloc((size_t)e->n * n_int * no);
-  e->dtd.alloc((size_t)e->n * n_int);
+  const int no = (int)s.O.size();
+  e->acc.alloc((size_t)qt::kAccStripes * n_int * (no + 1) * 2);
+  e->lastObs.alloc((size_t)e->n * std::max(1, no));
   qt::EnsDev E = e->view();
   E.intervals = e->intervals.p; E.n_intervals = n_int;
-  E.obs_out = e->obs.p; E.dtdid_out = e->dtd.p;
+  E.acc = e->acc.p; E.acc_max = std::ldexp(1.0, 43) / e->n; E.last_obs = e->lastObs.p;
   // jump draws for the launch, generated in bulk (one Philox per lane); trajectories that take
   // more steps than the budget evaluate Philox in the kernel
   if (!s.J.empty() && bulkDraws) {
@@ -641,6 +660,11 @@ static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hip
     E.draws = e->draws.p; E.draw_cap = cap;
   }
   return E;
+}
+
+// 64-bit words of a launch's stream sums
+static size_t ens_acc_words(const qt_ens* e, int n_int) {
+  return (size_t)qt::kAccStripes * n_int * (e->sys->h->O.size() + 1) * 2;
 }
 
 static void ens_kernel(qt_ens* e, const qt::EnsDev& E, hipStream_t st) {
@@ -665,6 +689,7 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
       hipCheck(hipStreamBeginCapture(e->capStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       if (E.draws)
         hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, E.draw_cap, e->draws.p, e->capStream), "jump draws");
+      hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, n_int), e->capStream), "zero stream sums");
       hipCheck(qt::launch_fill_int(e->work.p, 1, 0, e->capStream), "reset work queue");
       ens_kernel(e, E, e->capStream);
       hipCheck(hipStreamEndCapture(e->capStream, &e->graph), "hipStreamEndCapture");
@@ -678,6 +703,7 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
     ++e->graphLaunches;
   } else {
     qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0);
+    hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, n_int), st), "zero stream sums");
     hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
     if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
     ens_kernel(e, E, st);
@@ -687,35 +713,24 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
   for (int k = 0; k < n_int; ++k) e->time += intervals[k];
 }
 
-// Ensemble means of the last launch's stream outputs -> processed rows (time, dtDid, averages)
-// on the host; synchronises the stream.  Throws on a non-finite jump norm.
+// The ensemble rows of the last launch (time, mean dtDid, processed averages) from its stream
+// sums, made on the device and copied to the host; synchronises the stream.  Throws the
+// reference's exception for a non-finite rate, jump norm or streamed average.
 static void ens_rows(qt_ens* e, double tStart, const double* intervals, int n_int, double* rows, hipStream_t st) {
   const qt::System& s = *e->sys->h;
   const int no = (int)s.O.size();
-  e->mean.alloc((size_t)n_int * std::max(1, no));
-  e->meanDt.alloc(n_int);
-  hipCheck(qt::launch_ensemble_mean(e->obs.p, e->dtd.p, e->n, n_int, no, e->mean.p, e->meanDt.p, st), "mean launch");
-  std::vector<double> m((size_t)n_int * std::max(1, no)), md(n_int);
-  hipCheck(hipMemcpyAsync(m.data(), e->mean.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
-  hipCheck(hipMemcpyAsync(md.data(), e->meanDt.p, md.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+  (void)intervals;  // on the device since ens_prepare
+  e->rowsDev.alloc((size_t)n_int * (2 + no));
+  hipCheck(qt::launch_stream_rows(e->acc.p, n_int, no, e->n, tStart, e->intervals.p, e->sys->dev.pvar, e->sys->dev.psqrt,
+                                  e->rowsDev.p, st), "stream rows launch");
+  std::vector<double> r((size_t)n_int * (2 + no));
+  hipCheck(hipMemcpyAsync(r.data(), e->rowsDev.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
   int flag = 0;
   hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
   hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-  // the reference throws at the first non-finite rate (LiouvilleanAveragedCommon.h:82-93, via
-  // Liouvillean::rates) or jump norm (MCWF_Trajectory.h:338), and at non-finite streamed averages
-  if (flag & 2) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
-  if (flag & 1) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
-  for (double v : m) if (!std::isfinite(v)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
-  if (rows) {
-    double tt = tStart;
-    for (int k = 0; k < n_int; ++k) {
-      tt += intervals[k];
-      double* r = rows + (size_t)k * (2 + no);
-      r[0] = tt; r[1] = md[k];
-      for (int q = 0; q < no; ++q) r[2 + q] = m[(size_t)k * no + q];
-      process_rows(s, r + 2);
-    }
-  }
+  ens_check_flag(flag);
+  for (double v : r) if (std::isnan(v)) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+  if (rows) std::memcpy(rows, r.data(), r.size() * sizeof(double));
 }
 
 int qt_mcwf_advance(qt_ens* e, const double* intervals, int n_int, double* rows, qt_stream stream) {
@@ -773,10 +788,7 @@ int qt_mcwf_traj_averages(qt_ens* e, double* out) {
     const qt::System& s = *e->sys->h;
     const int no = (int)s.O.size();
     if (!e->lastIntervals) throw Fail(QT_EINVAL, "no interval advanced yet");
-    std::vector<double> all((size_t)e->n * e->lastIntervals * std::max(1, no));
-    hipCheck(hipMemcpy(all.data(), e->obs.p, all.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-    for (int i = 0; i < e->n; ++i)
-      for (int q = 0; q < no; ++q) out[(size_t)i * no + q] = all[((size_t)i * e->lastIntervals + e->lastIntervals - 1) * no + q];
+    if (no) hipCheck(hipMemcpy(out, e->lastObs.p, (size_t)e->n * no * sizeof(double), hipMemcpyDeviceToHost), "D2H");
   });
 }
 
@@ -1051,6 +1063,7 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
     qt_ens* e = b->mem[m];
     if (!intervals[m]) throw Fail(QT_EINVAL, "null interval list");
     M.E[m] = ens_prepare(e, intervals[m], n_int[m], st, true);
+    hipCheck(qt::launch_zero_u64(M.E[m].acc, ens_acc_words(e, n_int[m]), st), "zero stream sums");
     M.S[m] = e->sys->dev;
     M.team[m] = b->team[m];
     M.first[m] = items[b->team[m]];
@@ -1574,7 +1587,7 @@ size_t request_bytes(const qt::System& s, int n, int chunk, int jcap, double spa
   size_t b = 2 * (size_t)n * d * 16;                  // psi, psiInit
   b += (size_t)n * (5 * 8 + qt::kCounters * 8);        // t, dtDid, dtTry, dtTryInit, ordinal, counters
   b += (size_t)n * jcap * 12;                          // jump log
-  b += (size_t)n * chunk * (no + 1) * 8;               // stream outputs
+  b += (size_t)qt::kAccStripes * chunk * (no + 1) * 16 + (size_t)n * no * 8 + (size_t)chunk * (no + 2) * 8;  // stream sums, rows
   b += s.J.empty() ? 0 : (size_t)n * cap * 8;          // jump draws
   return b;
 }

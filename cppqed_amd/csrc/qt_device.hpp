@@ -233,6 +233,31 @@ __device__ __forceinline__ float2 gld2(const float2* p, unsigned i) {
   return make_float2(q[2 * i], q[2 * i + 1]);
 }
 
+// Stream averages are summed over the trajectories of an ensemble as fixed-point integers:
+// v = H 2^-20 + L 2^-60 with H = floor(v 2^20) and 0 <= L <= 2^40 (the remainder, exact in
+// double, rounded to 2^-61).  Integer addition is associative, so the sums -- and the ensemble
+// means made from them -- do not depend on the order in which trajectories reach a stream point
+// (bit-identical for any layout, batching or mix).  |v| < vmax keeps the H sum within 2^63
+// (vmax = 2^43 / n_traj); the L sums stay below 2^63 for up to 2^23 trajectories.  A value out
+// of range sets bit 1 (non-finite) or bit 2 (too large) of the launch's error flag instead.
+__device__ __forceinline__ void acc_add(unsigned long long* p, double v, double vmax, int* err) {
+  if (!(fabs(v) < vmax)) {
+    atomicOr(err, isfinite(v) ? 4 : 2);
+    return;
+  }
+  const double h = floor(v * 0x1p20);
+  const double r = v - h * 0x1p-20;  // exact: h 2^-20 is a double within 2^-20 of v
+  const unsigned long long H = (unsigned long long)(long long)h;
+  const unsigned long long L = (unsigned long long)rint(r * 0x1p60);
+  __hip_atomic_fetch_add(p, H, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(p + 1, L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the mean of n values accumulated by acc_add (sums of H and of L, two's complement)
+__device__ __forceinline__ double acc_mean(unsigned long long H, unsigned long long L, int n) {
+  return ((double)(long long)H * 0x1p-20 + (double)L * 0x1p-60) / (double)n;
+}
+
 // Cash-Karp RK4(5) tableau (as in Boost.Odeint's runge_kutta_cash_karp54)
 struct CK {
   static constexpr double c2 = 1. / 5, c3 = 3. / 10, c4 = 3. / 5, c5 = 1., c6 = 7. / 8;

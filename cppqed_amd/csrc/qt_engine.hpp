@@ -701,11 +701,16 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
       }
       bump(kMcwfSteps, 1);
     }
+    unsigned long long* accRow = E.acc + ((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) * 2;
+    const bool lastPoint = k == E.n_intervals - 1;
     for (int q = 0; q < S.no; ++q) {
       const double v = eng.observableX(q);
-      if (lane == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = v;
+      if (lane == 0) {
+        acc_add(accRow + 2 * q, v, E.acc_max, E.err_flag);
+        if (lastPoint) E.last_obs[(size_t)traj * S.no + q] = v;
+      }
     }
-    if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+    if (lane == 0) acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
   }
 
 #pragma unroll
@@ -919,16 +924,21 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     }
     // stream point: raw linear averages of the normalised state
     eng.stagePsi(x);
-    double* obsRow = E.obs_out + ((size_t)traj * E.n_intervals + k) * S.no;
+    unsigned long long* accRow = E.acc + ((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) * 2;
+    const bool lastPoint = k == E.n_intervals - 1;
+    auto store = [&](int q, double v) {
+      acc_add(accRow + 2 * q, v, E.acc_max, E.err_flag);
+      if (lastPoint) E.last_obs[(size_t)traj * S.no + q] = v;
+    };
     if (NW == 1 && eng.ored) {
-      eng.observablesBatched(x, S.no, [&](int q, double v) { obsRow[q] = v; });
+      eng.observablesBatched(x, S.no, store);
     } else {
       for (int q = 0; q < S.no; ++q) {
         const double v = eng.observable(q, x);
-        if (lane == 0) obsRow[q] = v;
+        if (lane == 0) store(q, v);
       }
     }
-    if (lane == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+    if (lane == 0) acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
   }
 
 #pragma unroll

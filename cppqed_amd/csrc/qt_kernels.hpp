@@ -9,6 +9,7 @@ namespace qt {
 constexpr int kMaxJumps = 8;      // jump channels handled by the batched kernel
 constexpr int kCounters = 8;      // per-trajectory counters, see Counter
 constexpr int kMaxFreqs = 32;     // distinct interaction-picture frequencies of one H_nH/i
+constexpr int kAccStripes = 16;   // stream accumulators are striped by trajectory index (less contention)
 
 enum Counter {
   kMcwfSteps = 0,   // MCWF_Trajectory::step calls
@@ -42,6 +43,10 @@ struct SysDev {
   const double2* freq;     // [nfreq], freq[0] = 0
   const int* hfidx;        // [wh][dim]
   const double2* exz;      // [dim] (null without an Exact part)
+  // processing of the streamed averages (Averaged::process): average q becomes
+  // av[q] - av[pvar[q]]^2 when pvar[q] >= 0, then its square root when psqrt[q]
+  const int* pvar;         // [no]
+  const int* psqrt;        // [no]
   // constant-offset form of the H table (Schroedinger picture): when non-null, slot w of every
   // row holds the entry at column row + hoff[w] (zero where the row has none, its hcol clamped
   // into range), so a gather may compute the column instead of loading it
@@ -68,8 +73,10 @@ struct EnsDev {
   // per launch
   const double* intervals;      // advance lengths (Trajectory.h:43-50 semantics)
   int n_intervals;
-  double* obs_out;              // [n_traj][n_intervals][no]
-  double* dtdid_out;            // [n_traj][n_intervals]
+  unsigned long long* acc;      // [kAccStripes][n_intervals][no + 1][2]: fixed-point sums over the
+                                // trajectories of each stream point's raw averages and dtDid (acc_add)
+  double acc_max;               // |value| bound that keeps the sums in range: 2^43 / n_traj
+  double* last_obs;             // [n_traj][no]: raw averages at the launch's last stream point
   int* work;                    // work-queue head (trajectories are claimed in order)
   int gsl;                      // 1: GSL's controller (kernel variant chosen on the host), 0: odeint's
   int* err_flag;                // bit 0: a non-finite jump norm; bit 1: non-finite jump rates

@@ -172,11 +172,17 @@ __global__ void __launch_bounds__(64) mcwf_sub_kernel(SysDev S, EnsDev E) {
           s += im ? (x[r].x * acc.y - x[r].y * acc.x) : (x[r].x * acc.x + x[r].y * acc.y);
         }
         s = gsum<G>(s);
-        if (atEnd && j == 0) E.obs_out[((size_t)traj * E.n_intervals + k) * S.no + q] = s;
+        if (atEnd && j == 0) {
+          acc_add(E.acc + (((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) + q) * 2, s, E.acc_max,
+                  E.err_flag);
+          if (k == E.n_intervals - 1) E.last_obs[(size_t)traj * S.no + q] = s;
+        }
       }
       bool finished = false;
       if (atEnd) {
-        if (j == 0) E.dtdid_out[(size_t)traj * E.n_intervals + k] = dtDid;
+        if (j == 0)
+          acc_add(E.acc + (((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) + S.no) * 2, dtDid,
+                  E.acc_max, E.err_flag);
         ++k;
         if (k == E.n_intervals) finished = true;
         else endTime = t + E.intervals[k];
