@@ -97,6 +97,7 @@ def lib():
         l.qt_mcwf_destroy.argtypes = [vp]
         l.qt_mcwf_advance.argtypes = [vp, vp, ip, vp, vp]
         l.qt_mcwf_advance_async.argtypes = [vp, vp, ip, vp]
+        l.qt_mcwf_advance_noalloc.argtypes = [vp, vp, ip, vp, vp]
         l.qt_mcwf_reset.argtypes = [vp, vp]
         l.qt_mcwf_set_graph.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         l.qt_mcwf_traj_averages.argtypes = [vp, vp]
@@ -338,6 +339,14 @@ class Ensemble:
 
     def reset(self, stream=None):
         check(lib().qt_mcwf_reset(self.h, stream))
+
+    def advance_noalloc(self, intervals):
+        """qt_mcwf_advance_noalloc: as advance(), with every per-launch device buffer in the library's
+        static scratch (no device allocation in the call)."""
+        iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))
+        rows = np.zeros((len(iv), 2 + self.system.n_avr))
+        check(lib().qt_mcwf_advance_noalloc(self.h, iv.ctypes.data, len(iv), rows.ctypes.data, None))
+        return rows
 
     def advance_async(self, intervals, stream=None):
         iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64))

@@ -11,6 +11,7 @@
 #include <map>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -621,6 +622,7 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->jt.alloc((size_t)n_traj * e->p.jump_log_cap);
     e->jch.alloc((size_t)n_traj * e->p.jump_log_cap);
     e->work.alloc(1);
+    e->lastObs.alloc((size_t)n_traj * std::max<size_t>(1, s.O.size()));  // also what qt_mcwf_advance_noalloc writes
     e->errflag.upload(std::vector<int>(1, 0));
     const int perGroup = e->lay.glanes > 0 ? 64 / e->lay.glanes : 1;  // trajectories per work group
     e->grid = std::min((n_traj + perGroup - 1) / perGroup, resident_groups(e->lay, p->precision == QT_FP32_DERIV, sys->interactionPicture(),
@@ -760,6 +762,90 @@ int qt_mcwf_reset(qt_ens* e, qt_stream stream) {
     hipCheck(hipMemsetAsync(e->cnt.p, 0, n * qt::kCounters * sizeof(long long), st), "reset counters");
     hipCheck(qt::launch_fill_int(e->errflag.p, 1, 0, st), "reset error flag");
     e->time = 0.;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- allocation-free advance
+
+// The per-launch buffers of qt_mcwf_advance_noalloc (intervals, stream sums, rows) live in this
+// static device array; each GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` names a host-side stand-in, not a device address: its device address is
+// taken once per device with hipGetSymbolAddress (scratch_base) and only that address is used.
+constexpr size_t kNoallocScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_base[kMaxDevices] = {};
+
+// device address of the current device's scratch (looked up on its first use on that device)
+unsigned char* scratch_base() {
+  int dev = 0;
+  hipCheck(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw Fail(QT_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_base[dev]) {
+    void* p = nullptr;
+    hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw Fail(QT_EDEVICE, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_base[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_base[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int qt_mcwf_advance_noalloc(qt_ens* e, const double* intervals, int n_int, double* rows, qt_stream stream) {
+  return guard([&] {
+    if (!e || !intervals) throw Fail(QT_EINVAL, "null argument");
+    if (n_int <= 0) throw Fail(QT_EINVAL, "no intervals");
+    for (int k = 0; k < n_int; ++k)
+      if (!(intervals[k] >= 0) || !std::isfinite(intervals[k])) throw Fail(QT_EINVAL, "intervals must be finite and non-negative");
+    const qt::System& s = *e->sys->h;
+    const int no = (int)s.O.size();
+    // scratch layout: intervals | stream sums | rows, each on a 256-byte boundary
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t oIv = take((size_t)n_int * sizeof(double));
+    const size_t oAcc = take(ens_acc_words(e, n_int) * sizeof(unsigned long long));
+    const size_t oRows = take((size_t)n_int * (2 + no) * sizeof(double));
+    if (off > kNoallocScratchBytes)
+      throw Fail(QT_EINVAL, "n_intervals = " + std::to_string(n_int) + " with " + std::to_string(no) +
+                                " averages needs " + std::to_string(off) + " bytes of scratch; the allocation-free advance has " +
+                                std::to_string(kNoallocScratchBytes) + " (16 n_intervals (n_avr + 1) 16 bytes of stream sums)");
+    unsigned char* base = scratch_base();
+    double* ivDev = reinterpret_cast<double*>(base + oIv);
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(base + oAcc);
+    double* rowsDev = reinterpret_cast<double*>(base + oRows);
+    hipStream_t st = (hipStream_t)stream;
+    e->sys->upload();  // made at qt_mcwf_create: no allocation here
+    hipCheck(hipMemcpyAsync(ivDev, intervals, (size_t)n_int * sizeof(double), hipMemcpyHostToDevice, st), "intervals H2D");
+    qt::EnsDev E = e->view();
+    E.intervals = ivDev; E.n_intervals = n_int;
+    E.acc = acc; E.acc_max = std::ldexp(1.0, 43) / e->n; E.last_obs = e->lastObs.p;
+    E.draws = nullptr; E.draw_cap = 0;  // jump draws from the counter RNG inside the kernel
+    const double tStart = e->time;
+    hipCheck(qt::launch_zero_u64(acc, ens_acc_words(e, n_int), st), "zero stream sums");
+    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    ens_kernel(e, E, st);
+    hipCheck(qt::launch_stream_rows(acc, n_int, no, e->n, tStart, ivDev, e->sys->dev.pvar, e->sys->dev.psqrt, rowsDev, st),
+             "stream rows launch");
+    int flag = 0;
+    hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+    if (rows) hipCheck(hipMemcpyAsync(rows, rowsDev, (size_t)n_int * (2 + no) * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (int k = 0; k < n_int; ++k) e->time += intervals[k];
+    e->lastIntervals = n_int;
+    ens_check_flag(flag);
+    if (rows)
+      for (size_t i = 0; i < (size_t)n_int * (2 + no); ++i)
+        if (std::isnan(rows[i])) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
   });
 }
 

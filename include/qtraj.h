@@ -107,6 +107,19 @@ int qt_mcwf_destroy(qt_ens* ens);
  * MCWF_Trajectory.h:406-420).  rows_host may be NULL. */
 int qt_mcwf_advance(qt_ens* ens, const double* intervals, int n_intervals, double* rows_host, qt_stream stream);
 
+/* Allocation-free form of qt_mcwf_advance.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same kernels and results as qt_mcwf_advance (rows_host as there,
+ * may be NULL), but every per-launch buffer -- the uploaded intervals, the stream sums and the
+ * device rows -- lives in the library's static device scratch array of 8 MiB per GPU, and the
+ * jump draws come from the counter RNG inside the kernel instead of a bulk draw buffer.  It calls
+ * no hipMalloc / hipFree (the ensemble's own buffers are made by qt_mcwf_create).  Limit: the
+ * stream sums take 256 n_intervals (n_avr + 1) bytes; beyond 8 MiB in all the call is refused
+ * with QT_EINVAL.  Every call on one GPU shares that GPU's scratch, which keeps the last call's
+ * contents (the stream sums are cleared by each call): calls that overlap on one GPU -- from
+ * other threads or on other streams -- race.  Synchronises `stream`. */
+int qt_mcwf_advance_noalloc(qt_ens* ens, const double* intervals, int n_intervals, double* rows_host, qt_stream stream);
+
 /* Asynchronous form for serving/graph replay: no host synchronisation, no rows. */
 int qt_mcwf_advance_async(qt_ens* ens, const double* intervals, int n_intervals, qt_stream stream);
 
