@@ -98,6 +98,7 @@ def lib():
         l.qt_mcwf_advance.argtypes = [vp, vp, ip, vp, vp]
         l.qt_mcwf_advance_async.argtypes = [vp, vp, ip, vp]
         l.qt_mcwf_advance_noalloc.argtypes = [vp, vp, ip, vp, vp]
+        l.qt_mcwf_set_draws.argtypes = [vp, vp, ip]
         l.qt_mcwf_reset.argtypes = [vp, vp]
         l.qt_mcwf_set_graph.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         l.qt_mcwf_traj_averages.argtypes = [vp, vp]
@@ -339,6 +340,17 @@ class Ensemble:
 
     def reset(self, stream=None):
         check(lib().qt_mcwf_reset(self.h, stream))
+
+    def set_draws(self, draws):
+        """qt_mcwf_set_draws: draws[i, j] is trajectory i's uniform for its j-th MCWF step from now
+        (None: back to the counter RNG)."""
+        if draws is None:
+            check(lib().qt_mcwf_set_draws(self.h, None, 0))
+            return
+        d = np.ascontiguousarray(np.asarray(draws, dtype=np.float64))
+        if d.ndim != 2 or d.shape[0] != self.n:
+            raise ValueError(f"draws must have shape ({self.n}, n_per)")
+        check(lib().qt_mcwf_set_draws(self.h, d.ctypes.data, d.shape[1]))
 
     def advance_noalloc(self, intervals):
         """qt_mcwf_advance_noalloc: as advance(), with every per-launch device buffer in the library's

@@ -340,6 +340,10 @@ struct qt_ens {
   DevBuf<double2> psi, psiInit;
   DevBuf<double> t, t0, dtDid, dtTry, dtTryInit, intervals, jt, draws, lastObs, rowsDev;
   DevBuf<unsigned long long> acc;   // stream sums of the last launch (acc_add)
+  // host-supplied jump draws (qt_mcwf_set_draws): [n][hostDrawsPer], indexed from drawBase[traj]
+  DevBuf<double> hostDraws;
+  DevBuf<unsigned long long> drawBase;
+  int hostDrawsPer = 0;
   DevBuf<unsigned long long> ord;
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
@@ -377,6 +381,9 @@ struct qt_ens {
     E.dpLimit = p.dpLimit; E.overshootTol = p.overshootTol;
     E.work = work.p; E.err_flag = errflag.p;
     E.gsl = p.controller == QT_CONTROLLER_GSL;
+    if (hostDrawsPer > 0) {
+      E.draws = hostDraws.p; E.draw_cap = hostDrawsPer; E.draw_base = drawBase.p; E.draws_strict = 1;
+    }
     return E;
   }
 };
@@ -434,6 +441,7 @@ void ens_check_flag(int flag) {
   if (flag & 2) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
   if (flag & 1) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
   if (flag & 4) throw Fail(QT_ENUMERIC, "a streamed average exceeds the range of the ensemble sums (2^43 / n_traj)");
+  if (flag & 8) throw Fail(QT_EINVAL, "a trajectory needed more jump draws than qt_mcwf_set_draws supplied");
 }
 
 // rates of a host state through the host ELL tables: ||J_m psi||^2
@@ -653,7 +661,7 @@ static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hip
   E.acc = e->acc.p; E.acc_max = std::ldexp(1.0, 43) / e->n; E.last_obs = e->lastObs.p;
   // jump draws for the launch, generated in bulk (one Philox per lane); trajectories that take
   // more steps than the budget evaluate Philox in the kernel
-  if (!s.J.empty() && bulkDraws) {
+  if (!s.J.empty() && bulkDraws && !e->hostDrawsPer) {
     double span = 0.; for (int k = 0; k < n_int; ++k) span += intervals[k];
     const int cap = (int)std::min(2048.0, std::max(64.0, std::ceil(span / std::max(e->dtMinSeen, 1e-6) * 1.25) + 8));
     e->draws.alloc((size_t)e->n * cap);
@@ -689,7 +697,7 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
       e->dropGraph();
       if (!e->capStream) hipCheck(hipStreamCreate(&e->capStream), "hipStreamCreate");
       hipCheck(hipStreamBeginCapture(e->capStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      if (E.draws)
+      if (E.draws && !E.draws_strict)
         hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, E.draw_cap, e->draws.p, e->capStream), "jump draws");
       hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, n_int), e->capStream), "zero stream sums");
       hipCheck(qt::launch_fill_int(e->work.p, 1, 0, e->capStream), "reset work queue");
@@ -829,7 +837,7 @@ int qt_mcwf_advance_noalloc(qt_ens* e, const double* intervals, int n_int, doubl
     qt::EnsDev E = e->view();
     E.intervals = ivDev; E.n_intervals = n_int;
     E.acc = acc; E.acc_max = std::ldexp(1.0, 43) / e->n; E.last_obs = e->lastObs.p;
-    E.draws = nullptr; E.draw_cap = 0;  // jump draws from the counter RNG inside the kernel
+    if (!e->hostDrawsPer) { E.draws = nullptr; E.draw_cap = 0; }  // jump draws from the counter RNG inside the kernel
     const double tStart = e->time;
     hipCheck(qt::launch_zero_u64(acc, ens_acc_words(e, n_int), st), "zero stream sums");
     hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
@@ -846,6 +854,25 @@ int qt_mcwf_advance_noalloc(qt_ens* e, const double* intervals, int n_int, doubl
     if (rows)
       for (size_t i = 0; i < (size_t)n_int * (2 + no); ++i)
         if (std::isnan(rows[i])) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+  });
+}
+
+int qt_mcwf_set_draws(qt_ens* e, const double* draws, int n_per) {
+  return guard([&] {
+    if (!e || n_per < 0 || (n_per > 0 && !draws)) throw Fail(QT_EINVAL, "bad arguments");
+    if (e->lay.glanes > 0) throw Fail(QT_ENOTSUP, "host-supplied jump draws need a layout of one or more wavefronts per trajectory");
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    e->dropGraph();
+    if (n_per == 0) { e->hostDrawsPer = 0; return; }
+    const size_t n = (size_t)e->n * n_per;
+    for (size_t i = 0; i < n; ++i)
+      if (!(draws[i] >= 0. && draws[i] < 1.)) throw Fail(QT_EINVAL, "jump draws must lie in [0, 1)");
+    e->hostDraws.alloc(n);
+    hipCheck(hipMemcpy(e->hostDraws.p, draws, n * sizeof(double), hipMemcpyHostToDevice), "H2D");
+    e->drawBase.alloc(e->n);
+    hipCheck(hipMemcpy(e->drawBase.p, e->ord.p, e->n * sizeof(unsigned long long), hipMemcpyDeviceToDevice), "D2D");
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    e->hostDrawsPer = n_per;
   });
 }
 

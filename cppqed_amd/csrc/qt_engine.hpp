@@ -571,7 +571,8 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   for (int r = 0; r < ROWS; ++r) if (eng.live(r)) X[r * G + lane] = eng.valid(r) ? slot[r * G + lane] : make_double2(0., 0.);
   double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj], t0 = E.t0[traj];
   unsigned long long ord = E.ordinal[traj];
-  const unsigned long long ord0 = ord;
+  // index of the draw of ordinal ord in the trajectory's row of E.draws: ord - ord0
+  const unsigned long long ord0 = E.draw_base ? E.draw_base[traj] : ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
   const double2* exz = S.exz;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
@@ -669,6 +670,7 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
           dtTry = fmin(dtTry, sugg);
           break;
         }
+        if (uPre < 0. && E.draws_strict && lane == 0) atomicOr(E.err_flag, 8);  // host draws ran out
         const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);  // performJump (:320-344)
         ++ord;
         double random = u / dtDid;
@@ -743,7 +745,8 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? E.psi[(size_t)traj * S.dim + r * G + lane] : make_double2(0., 0.);
   double t = E.t[traj], dtDid = E.dtDid[traj], dtTry = E.dtTry[traj], t0 = E.t0[traj];
   unsigned long long ord = E.ordinal[traj];
-  const unsigned long long ord0 = ord;
+  // index of the draw of ordinal ord in the trajectory's row of E.draws: ord - ord0
+  const unsigned long long ord0 = E.draw_base ? E.draw_base[traj] : ord;
   const double* draws = E.draws ? E.draws + (size_t)traj * E.draw_cap : nullptr;
   const double2* exz = S.exz;
   if (lane < kCounters) cnt[lane] = E.counters[(size_t)traj * kCounters + lane];
@@ -894,7 +897,9 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
           dtTry = fmin(dtTry, sugg);
           break;
         }
-        // performJump (MCWF_Trajectory.h:320-344): one draw per step
+        // performJump (MCWF_Trajectory.h:320-344): one draw per step; host-supplied draws that ran
+        // out fail the launch (strict), otherwise the counter RNG fills in
+        if (uPre < 0. && E.draws_strict && lane == 0) atomicOr(E.err_flag, 8);
         const double u = uPre >= 0. ? uPre : jump_uniform(E.seed, key, ord);
         ++ord;
         double random = u / dtDid;

@@ -79,9 +79,13 @@ struct EnsDev {
   double* last_obs;             // [n_traj][no]: raw averages at the launch's last stream point
   int* work;                    // work-queue head (trajectories are claimed in order)
   int gsl;                      // 1: GSL's controller (kernel variant chosen on the host), 0: odeint's
-  int* err_flag;                // bit 0: a non-finite jump norm; bit 1: non-finite jump rates
-  const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals ordinal[traj] + i (may be null)
+  int* err_flag;                // bit 0: a non-finite jump norm; bit 1: non-finite rates or averages;
+                                // bit 2: an average beyond the stream sums' range; bit 3: host draws ran out
+  const double* draws;          // [n_traj][draw_cap]: jump draws for ordinals base + i (may be null), base =
+                                // draw_base[traj] if draw_base, else the trajectory's ordinal at launch
   int draw_cap;
+  const unsigned long long* draw_base;  // [n_traj] or null
+  int draws_strict;             // 1: draws are host-supplied, running out of them fails the launch (bit 3)
 };
 
 // Mixed batch: up to kMaxMembers ensembles of different systems advanced by one launch.

@@ -107,6 +107,16 @@ int qt_mcwf_destroy(qt_ens* ens);
  * MCWF_Trajectory.h:406-420).  rows_host may be NULL. */
 int qt_mcwf_advance(qt_ens* ens, const double* intervals, int n_intervals, double* rows_host, qt_stream stream);
 
+/* Jump draws from the host instead of the counter RNG: trajectory i's uniforms in [0, 1) for its
+ * next n_per MCWF steps, draws_host[i * n_per + j] for the step whose RNG ordinal is j past the
+ * ordinal it has now -- e.g. the reference's pcg64 streams through libstdc++'s
+ * generate_canonical (randomutils::Pars / incrementForNextStream, CPPQEDutils/Random.h:137-161;
+ * MCWF_Trajectory::sampleRandom, MCWF_Trajectory.h:161, 191), so that the device replays the
+ * reference's own runs.  Later advances (also after qt_mcwf_reset, which returns the ordinals to
+ * 0) take their draws from this table; an advance in which a trajectory needs a draw beyond it
+ * fails with QT_EINVAL.  n_per = 0 returns to the counter RNG.  Not for sub-wave layouts. */
+int qt_mcwf_set_draws(qt_ens* ens, const double* draws_host, int n_per);
+
 /* Allocation-free form of qt_mcwf_advance.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  The same kernels and results as qt_mcwf_advance (rows_host as there,
