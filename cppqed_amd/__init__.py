@@ -99,6 +99,7 @@ def lib():
         l.qt_mcwf_advance_async.argtypes = [vp, vp, ip, vp]
         l.qt_mcwf_advance_noalloc.argtypes = [vp, vp, ip, vp, vp]
         l.qt_mcwf_set_draws.argtypes = [vp, vp, ip]
+        l.qt_mcwf_advance_dc.argtypes = [vp, dp, ip, ip, vp, C.POINTER(C.c_int), vp]
         l.qt_mcwf_reset.argtypes = [vp, vp]
         l.qt_mcwf_set_graph.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         l.qt_mcwf_traj_averages.argtypes = [vp, vp]
@@ -340,6 +341,20 @@ class Ensemble:
 
     def reset(self, stream=None):
         check(lib().qt_mcwf_reset(self.h, stream))
+
+    def advance_dc(self, t_end, dc, max_rows=1024):
+        """qt_mcwf_advance_dc (one trajectory): rows after every dc MCWF steps and at t_end, until
+        t_end is reached; shape (rows, 2 + n_avr)."""
+        out = []
+        while True:
+            rows = np.zeros((max_rows, 2 + self.system.n_avr))
+            n = C.c_int()
+            check(lib().qt_mcwf_advance_dc(self.h, float(t_end), int(dc), int(max_rows), rows.ctypes.data,
+                                           C.byref(n), None))
+            out.append(rows[:n.value])
+            if n.value < max_rows or (n.value and rows[n.value - 1, 0] >= t_end):
+                break
+        return np.concatenate(out)
 
     def set_draws(self, draws):
         """qt_mcwf_set_draws: draws[i, j] is trajectory i's uniform for its j-th MCWF step from now

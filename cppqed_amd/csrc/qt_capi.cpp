@@ -857,6 +857,50 @@ int qt_mcwf_advance_noalloc(qt_ens* e, const double* intervals, int n_int, doubl
   });
 }
 
+int qt_mcwf_advance_dc(qt_ens* e, double t_end, int dc, int max_rows, double* rows, int* n_rows, qt_stream stream) {
+  return guard([&] {
+    if (!e || !rows || !n_rows || dc < 1 || max_rows < 1 || !std::isfinite(t_end)) throw Fail(QT_EINVAL, "bad arguments");
+    if (e->n != 1) throw Fail(QT_EINVAL, "dc mode drives a single trajectory (an Ensemble has no step(), Trajectory.h:317-323)");
+    if (e->lay.glanes > 0) throw Fail(QT_ENOTSUP, "dc mode needs a layout of one or more wavefronts per trajectory");
+    if (t_end < e->time) throw Fail(QT_EINVAL, "t_end lies before the trajectory's time");
+    *n_rows = 0;
+    if (e->time >= t_end) return;  // the run loop's `while (t < T)` does not start
+    const qt::System& s = *e->sys->h;
+    const int no = (int)s.O.size();
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> iv(max_rows, 0.);
+    iv[0] = t_end - e->time;  // sizes the launch's draw budget; dc mode ignores the intervals
+    qt::EnsDev E = ens_prepare(e, iv.data(), max_rows, st, true);
+    E.dc = dc; E.t_end = t_end;
+    e->rowsDev.alloc((size_t)max_rows * (2 + no));
+    E.rows_out = e->rowsDev.p;
+    hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, max_rows), st), "zero stream sums");
+    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    ens_kernel(e, E, st);
+    std::vector<double> r((size_t)max_rows * (2 + no));
+    hipCheck(hipMemcpyAsync(r.data(), e->rowsDev.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    int flag = 0;
+    double tNow = 0.;
+    hipCheck(hipMemcpyAsync(&flag, e->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(&tNow, e->t.p, sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    e->time = tNow;
+    e->lastIntervals = max_rows;
+    ens_check_flag(flag);
+    // the rows up to the one at t_end (the run loop streams the end instant once, Trajectory.h:597)
+    int n = 0;
+    while (n < max_rows) {
+      double* row = r.data() + (size_t)n * (2 + no);
+      for (int q = 0; q < no; ++q) if (!std::isfinite(row[2 + q])) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+      process_rows(s, row + 2);
+      ++n;
+      if (row[0] >= t_end) break;
+    }
+    std::memcpy(rows, r.data(), (size_t)n * (2 + no) * sizeof(double));
+    *n_rows = n;
+  });
+}
+
 int qt_mcwf_set_draws(qt_ens* e, const double* draws, int n_per) {
   return guard([&] {
     if (!e || n_per < 0 || (n_per > 0 && !draws)) throw Fail(QT_EINVAL, "bad arguments");

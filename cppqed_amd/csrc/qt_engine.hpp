@@ -639,10 +639,13 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   };
 
   for (int k = 0; k < E.n_intervals; ++k) {
-    const double endTime = t + E.intervals[k];
-    for (;;) {
+    // Dt mode: advance(intervals[k]); dc mode: up to dc MCWF steps step(t_end - t) per stream point
+    // (trajectory::run with dc > 0, Trajectory.h:560-600)
+    const double endTime = E.dc > 0 ? E.t_end : t + E.intervals[k];
+    const int maxSteps = E.dc > 0 ? E.dc : -1;
+    for (int steps = 0;; ++steps) {
       const double Dt = endTime - t;
-      if (Dt == 0.) break;
+      if (Dt == 0. || steps == maxSteps) break;
       const unsigned long long di = ord - ord0;
       const double uPre = (draws && di < (unsigned long long)E.draw_cap) ? gbl(draws)[di] : -1.0;
 #pragma unroll
@@ -705,14 +708,19 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     }
     unsigned long long* accRow = E.acc + ((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) * 2;
     const bool lastPoint = k == E.n_intervals - 1;
+    double* rowOut = E.rows_out ? E.rows_out + ((size_t)traj * E.n_intervals + k) * (2 + S.no) : nullptr;
     for (int q = 0; q < S.no; ++q) {
       const double v = eng.observableX(q);
       if (lane == 0) {
         acc_add(accRow + 2 * q, v, E.acc_max, E.err_flag);
         if (lastPoint) E.last_obs[(size_t)traj * S.no + q] = v;
+        if (rowOut) rowOut[2 + q] = v;
       }
     }
-    if (lane == 0) acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
+    if (lane == 0) {
+      acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
+      if (rowOut) { rowOut[0] = t; rowOut[1] = dtDid; }
+    }
   }
 
 #pragma unroll
@@ -858,10 +866,13 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   };
 
   for (int k = 0; k < E.n_intervals; ++k) {
-    const double endTime = t + E.intervals[k];
-    for (;;) {
+    // Dt mode: advance(intervals[k]); dc mode: up to dc MCWF steps step(t_end - t) per stream point
+    // (trajectory::run with dc > 0, Trajectory.h:560-600)
+    const double endTime = E.dc > 0 ? E.t_end : t + E.intervals[k];
+    const int maxSteps = E.dc > 0 ? E.dc : -1;
+    for (int steps = 0;; ++steps) {
       const double Dt = endTime - t;
-      if (Dt == 0.) break;
+      if (Dt == 0. || steps == maxSteps) break;
       // MCWF_Trajectory::step (MCWF_Trajectory.h:347-374)
       // the step's jump draw, fetched early so the load completes behind the RK work
       const unsigned long long di = ord - ord0;
@@ -931,9 +942,11 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     eng.stagePsi(x);
     unsigned long long* accRow = E.acc + ((size_t)(traj % kAccStripes) * E.n_intervals + k) * (S.no + 1) * 2;
     const bool lastPoint = k == E.n_intervals - 1;
+    double* rowOut = E.rows_out ? E.rows_out + ((size_t)traj * E.n_intervals + k) * (2 + S.no) : nullptr;
     auto store = [&](int q, double v) {
       acc_add(accRow + 2 * q, v, E.acc_max, E.err_flag);
       if (lastPoint) E.last_obs[(size_t)traj * S.no + q] = v;
+      if (rowOut) rowOut[2 + q] = v;
     };
     if (NW == 1 && eng.ored) {
       eng.observablesBatched(x, S.no, store);
@@ -943,7 +956,10 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
         if (lane == 0) store(q, v);
       }
     }
-    if (lane == 0) acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
+    if (lane == 0) {
+      acc_add(accRow + 2 * S.no, dtDid, E.acc_max, E.err_flag);
+      if (rowOut) { rowOut[0] = t; rowOut[1] = dtDid; }
+    }
   }
 
 #pragma unroll

@@ -85,6 +85,11 @@ struct EnsDev {
                                 // draw_base[traj] if draw_base, else the trajectory's ordinal at launch
   int draw_cap;
   const unsigned long long* draw_base;  // [n_traj] or null
+  // dc mode (trajectory::run with dc > 0): each of the n_intervals stream points follows up to dc
+  // MCWF steps step(t_end - t); 0: Dt mode (the intervals)
+  int dc;
+  double t_end;
+  double* rows_out;             // [n_traj][n_intervals][2 + no] raw rows (time, dtDid, averages) or null
   int draws_strict;             // 1: draws are host-supplied, running out of them fails the launch (bit 3)
 };
 

@@ -107,6 +107,16 @@ int qt_mcwf_destroy(qt_ens* ens);
  * MCWF_Trajectory.h:406-420).  rows_host may be NULL. */
 int qt_mcwf_advance(qt_ens* ens, const double* intervals, int n_intervals, double* rows_host, qt_stream stream);
 
+/* dc mode of the run loop for a single trajectory (cppqedutils::run with dc > 0,
+ * CPPQEDutils/Trajectory.h:555-600): MCWF steps MCWF_Trajectory::step(t_end - t)
+ * (MCWF_Trajectory.h:347-374), a streamed row after every dc steps and at t_end, up to max_rows
+ * rows in this call.  rows_host[r*(2+n_avr)] = time, dtDid, processed averages; *n_rows = rows
+ * written (fewer than max_rows only when t_end was reached; 0 when the trajectory is already at
+ * t_end).  An ensemble of one trajectory only: the reference's Ensemble has no step()
+ * (Trajectory.h:317-323), so its dc mode is the Dt mode of qt_mcwf_advance. */
+int qt_mcwf_advance_dc(qt_ens* ens, double t_end, int dc, int max_rows, double* rows_host, int* n_rows,
+                       qt_stream stream);
+
 /* Jump draws from the host instead of the counter RNG: trajectory i's uniforms in [0, 1) for its
  * next n_per MCWF steps, draws_host[i * n_per + j] for the step whose RNG ordinal is j past the
  * ordinal it has now -- e.g. the reference's pcg64 streams through libstdc++'s
