@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--ntraj", type=int, default=16384)
     ap.add_argument("--T", type=float, default=20.0)
     ap.add_argument("--plain-traj", type=int, default=4)
-    ap.add_argument("--cpu-traj", type=int, default=1024)
+    ap.add_argument("--cpu-traj", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-master", action="store_true")
@@ -163,16 +163,7 @@ def main():
         if "sq_insts_valu_per_try" in tj:
             issue = tj["sq_insts_valu_per_try"] * b32["tries"] * 4 / (b32["kernel_s"] * 2.4e9 * 1024)
 
-    cpu = None
-    if not args.no_cpu and args.cpu_traj > 0:
-        import oracle_lib as O
-        so = O.System(JC60)
-        t0 = time.perf_counter()
-        _, ctries = O.ensemble_run(so, psi0, O.params(rng="philox", seed=1001, stream=0),
-                                   O.run_spec("Dt", Dt=0.1, T=args.T), args.cpu_traj)
-        cel = time.perf_counter() - t0
-        cpu = {"value": ctries / cel, "unit": "RK try-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_traj} JC-60 trajectories to T={args.T}, FP64, 1 thread ({ctries} tries, {cel:.1f} s)"}
+    cpu = None if args.no_cpu or args.cpu_traj <= 0 else cpu_baseline(JC60, psi0, args)
 
     master = None
     if not args.no_master:
@@ -246,6 +237,49 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, psi0, args):
+    """The oracle's ensemble (FP64, the reference's step logic, -O3) on the host: one thread over
+    args.cpu_traj trajectories (the reference's serial Ensemble::advance), then one thread per
+    host core over disjoint trajectory ranges (cores available to this process, at most 16, the
+    CPU share of one GPU on the box).  ctypes releases the GIL inside the oracle's calls."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle_lib as O
+    so = O.System(spec)
+    run = O.run_spec("Dt", Dt=0.1, T=args.T)
+
+    def chunk(first, n):
+        return O.ensemble_run(so, psi0, O.params(rng="philox", seed=1001, stream=first), run, n)[1]
+
+    t0 = time.perf_counter()
+    t1tries = chunk(0, args.cpu_traj)
+    t1 = time.perf_counter() - t0
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    per = max(1, args.cpu_traj // 2)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        tn_tries = sum(ex.map(lambda i: chunk(i * per, per), range(cores)))
+    tn = time.perf_counter() - t0
+    return {"value": tn_tries / tn, "unit": "RK try-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} threads x {per} JC-60 trajectories to T={args.T}, FP64 ({tn_tries} tries, {tn:.1f} s)",
+            "cpu_model": cpu_model(),
+            "single_core": {"value": t1tries / t1, "cores": 1,
+                            "sample": f"{args.cpu_traj} JC-60 trajectories to T={args.T}, FP64, 1 thread "
+                                      f"({t1tries} tries, {t1:.1f} s)"},
+            "note": "reported baseline only: the oracle (a C++ restatement of the reference's MCWF step, "
+                    "compiled -O3), not the upstream binary, which cannot be built here"}
 
 
 def master_traffic(args, tries, launches):

@@ -131,6 +131,7 @@ def lib():
         l.qt_master_create.argtypes = [vp, ip, C.POINTER(MasterParams), vp, C.POINTER(vp)]
         l.qt_master_destroy.argtypes = [vp]
         l.qt_master_advance.argtypes = [vp, vp, ip, vp, vp, vp]
+        l.qt_master_advance_dc.argtypes = [vp, dp, ip, ip, vp, C.POINTER(C.c_int), vp]
         l.qt_master_rho.argtypes = [vp, ip, vp]
         l.qt_master_status.argtypes = [vp, vp, vp, vp, vp]
         l.qt_master_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -702,6 +703,19 @@ class MasterSweep:
         dn = np.zeros((self.n, len(iv)))
         check(lib().qt_master_advance(self.h, iv.ctypes.data, len(iv), rows.ctypes.data, dn.ctypes.data, None))
         return rows, dn
+
+    def advance_dc(self, t_end, dc, max_rows=1024):
+        """qt_master_advance_dc (a sweep of one point): rows after every dc steps and at t_end."""
+        out = []
+        while True:
+            rows = np.zeros((max_rows, 2 + self.n_avr))
+            n = C.c_int()
+            check(lib().qt_master_advance_dc(self.h, float(t_end), int(dc), int(max_rows), rows.ctypes.data,
+                                             C.byref(n), None))
+            out.append(rows[:n.value])
+            if n.value < max_rows or (n.value and rows[n.value - 1, 0] >= t_end):
+                break
+        return np.concatenate(out)
 
     def rho(self, point):
         r = np.zeros((self.dim, self.dim), np.complex128)

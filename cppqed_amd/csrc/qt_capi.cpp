@@ -393,7 +393,12 @@ struct qt_master {
   qt_master_params p{};
   qt_sys* sys0 = nullptr;
   double time = 0.;
-  DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn;
+  DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn, tout;
+  DevBuf<double2> exz;             // [point][d] (interaction picture only)
+  bool td = false;
+  int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product (interaction picture)
+  DevBuf<int> rcol, rfi;
+  DevBuf<double2> rval, rfreq;
   DevBuf<int> jcol, done;
   DevBuf<double2> jval, x, k;
   DevBuf<long long> cnt;
@@ -413,6 +418,30 @@ struct qt_master {
 
 namespace {
 
+// The Master kernel takes an interaction-picture system when it is unitary: the reference's Master
+// refuses the others (applicableInMaster, Master.h:73; --picture IP becomes UIP, QM_Picture.cc:45-47).
+void master_picture_check(const qt::System& s) {
+  if (!s.hasEx) {
+    if (s.H.timeDependent()) throw Fail(QT_ENOTSUP, "the Master sweep's interaction picture needs Exact frees");
+    return;
+  }
+  for (auto& z : s.exactZ)
+    if (z.real() != 0.)
+      throw Fail(QT_EINVAL, "SystemNotApplicable: the Master equation needs a unitary interaction picture (UIP)");
+}
+
+// Does entry i (row r) of H_nH/i rotate with exz[r] - exz[col] in the interaction picture?  Then the
+// MFMA product takes it (H(tau) = E H E^-1); otherwise -- a term that couples an Exact free with
+// frequencies the reference leaves out (Tridiagonal.tcc:61-67), e.g. ParticleAlongCavity's
+// dispersive term with its frequency-less nop -- it goes to the sparse remainder.
+bool master_entry_rotates(const qt::System& s, long r, size_t i) {
+  if (!s.hasEx) return true;
+  double scale = 1.;
+  for (auto& z : s.exactZ) scale = std::max(scale, std::abs(z));
+  const qt::cd f = s.H.freqs.empty() ? qt::cd(0.) : s.H.freqs[i];
+  return std::abs(f - (s.exactZ[r] - s.exactZ[s.H.cols[i]])) <= 1e-12 * scale;
+}
+
 // what qt_master_create refuses about its parameters and the sweep's systems
 void check_master(qt_sys* const* systems, int n_points, const qt_master_params* p) {
   if (!systems || !p || n_points <= 0) throw Fail(QT_EINVAL, "bad arguments");
@@ -428,7 +457,7 @@ void check_master(qt_sys* const* systems, int n_points, const qt_master_params* 
     const qt::System& s = *systems[k]->h;
     if (s.dim != s0.dim || s.J.size() != s0.J.size())
       throw Fail(QT_EINVAL, "sweep points must share dimension and jump channels");
-    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the Master sweep runs the Schroedinger picture only");
+    master_picture_check(s);
   }
 }
 
@@ -1357,11 +1386,17 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     std::vector<int> jc((size_t)n_points * std::max(1, nj) * wj * d, 0);
     std::vector<double2> jv((size_t)n_points * std::max(1, nj) * wj * d, make_double2(0., 0.));
     std::vector<double> dtTry(n_points);
+    std::vector<std::vector<std::pair<int, int>>> rest(n_points);  // (row, table index) of the remainder
     for (int k = 0; k < n_points; ++k) {
       const qt::System& s = *systems[k]->h;
       std::vector<qt::cd> H((size_t)DP * DP, 0.);
       for (int r = 0; r < d; ++r)
-        for (int w = 0; w < s.H.width; ++w) H[(size_t)r * DP + s.H.cols[(size_t)r * s.H.width + w]] += s.H.vals[(size_t)r * s.H.width + w];
+        for (int w = 0; w < s.H.width; ++w) {
+          const size_t i = (size_t)r * s.H.width + w;
+          if (s.H.vals[i] == qt::cd(0.)) continue;
+          if (master_entry_rotates(s, r, i)) H[(size_t)r * DP + s.H.cols[i]] += s.H.vals[i];
+          else rest[k].push_back({r, (int)i});
+        }
       for (int rb = 0; rb < NB; ++rb)
         for (int kk = 0; kk < KC; ++kk)
           for (int l = 0; l < 64; ++l) {
@@ -1382,6 +1417,51 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
       dtTry[k] = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
     }
     m->hfrag.upload(hf); m->jcol.upload(jc); m->jval.upload(jv);
+    for (int k = 0; k < n_points; ++k) m->td = m->td || systems[k]->h->hasEx;
+    if (m->td) {  // the Exact frees' exponents per point (zero in the Schroedinger picture)
+      std::vector<double2> z((size_t)n_points * d, make_double2(0., 0.));
+      for (int k = 0; k < n_points; ++k) {
+        const qt::System& sk = *systems[k]->h;
+        if (!sk.hasEx) continue;
+        for (int i = 0; i < d; ++i) z[(size_t)k * d + i] = make_double2(sk.exactZ[i].real(), sk.exactZ[i].imag());
+      }
+      m->exz.upload(z);
+      // the remainder as ELL [point][wr][d] with per-point frequency tables
+      int wr = 0, nrf = 1;
+      std::vector<std::vector<qt::cd>> fl(n_points, std::vector<qt::cd>{qt::cd(0.)});
+      for (int k = 0; k < n_points; ++k) {
+        std::vector<int> per(d, 0);
+        for (auto& e : rest[k]) wr = std::max(wr, ++per[e.first]);
+        const qt::System& sk = *systems[k]->h;
+        for (auto& e : rest[k]) {
+          const qt::cd f = sk.H.freqs.empty() ? qt::cd(0.) : sk.H.freqs[e.second];
+          if (std::find(fl[k].begin(), fl[k].end(), f) == fl[k].end()) fl[k].push_back(f);
+        }
+        nrf = std::max(nrf, (int)fl[k].size());
+      }
+      if (nrf > qt::kMaxFreqs) throw Fail(QT_ENOTSUP, "interaction picture with more than 32 distinct frequencies");
+      m->wr = wr;
+      m->nrf = nrf;
+      if (wr > 0) {
+        std::vector<int> rc((size_t)n_points * wr * d, 0), rf((size_t)n_points * wr * d, 0);
+        std::vector<double2> rv((size_t)n_points * wr * d, make_double2(0., 0.));
+        std::vector<double2> rfq((size_t)n_points * nrf, make_double2(0., 0.));
+        for (int k = 0; k < n_points; ++k) {
+          const qt::System& sk = *systems[k]->h;
+          std::vector<int> per(d, 0);
+          for (auto& e : rest[k]) {
+            const int r = e.first, a = per[r]++;
+            const size_t o = ((size_t)k * wr + a) * d + r;
+            const qt::cd f = sk.H.freqs.empty() ? qt::cd(0.) : sk.H.freqs[e.second];
+            rc[o] = sk.H.cols[e.second];
+            rv[o] = make_double2(sk.H.vals[e.second].real(), sk.H.vals[e.second].imag());
+            rf[o] = (int)(std::find(fl[k].begin(), fl[k].end(), f) - fl[k].begin());
+          }
+          for (size_t q = 0; q < fl[k].size(); ++q) rfq[(size_t)k * nrf + q] = make_double2(fl[k][q].real(), fl[k][q].imag());
+        }
+        m->rcol.upload(rc); m->rval.upload(rv); m->rfi.upload(rf); m->rfreq.upload(rfq);
+      }
+    }
     m->nj = nj; m->wj = wj;
     systems[0]->upload();  // observables of point 0
     // rho0 -> tile order: index (4 t + v) 64 + l holds (16 r + 4 v + l/16, 16 c + l%16), t = r NB + c
@@ -1425,7 +1505,8 @@ int qt_master_destroy(qt_master* m) { delete m; return QT_OK; }
 // in flight before waiting: the launch half uploads the intervals, launches and queues the copies
 // of the stream outputs into pinned host memory, all asynchronous on st; the finish half waits
 // for st and turns the outputs into rows.
-static void master_launch(qt_master* m, const double* intervals, int n_int, hipStream_t st) {
+static void master_launch(qt_master* m, const double* intervals, int n_int, hipStream_t st, int dc = 0,
+                          double t_end = 0.) {
     const qt::System& s0 = *m->sys0->h;
     const int no = (int)s0.O.size();
     m->intervals.alloc(n_int);
@@ -1445,6 +1526,10 @@ static void master_launch(qt_master* m, const double* intervals, int n_int, hipS
     M.as_buf = m->asbuf.p; M.as_meta = m->asmeta.p; M.proc_var = m->pvar.p; M.proc_sqrt = m->psqrt.p;
     M.intervals = m->intervals.p; M.n_intervals = n_int;
     M.obs_out = m->obs.p; M.dtdid_out = m->dtd.p; M.dnorm_out = m->dn.p;
+    M.exz = m->td ? m->exz.p : nullptr;
+    M.wr = m->wr; M.rcol = m->rcol.p; M.rval = m->rval.p; M.rfi = m->rfi.p; M.nrf = m->nrf; M.rfreq = m->rfreq.p;
+    M.dc = dc; M.t_end = t_end;
+    if (dc > 0) { m->tout.alloc((size_t)m->n * n_int); M.time_out = m->tout.p; }
     hipCheck(hipEventRecord(m->e0, st), "hipEventRecord");
     hipCheck(qt::launch_master(m->nb, M, st), "master launch");
     hipCheck(hipEventRecord(m->e1, st), "hipEventRecord");
@@ -1493,6 +1578,42 @@ int qt_master_advance(qt_master* m, const double* intervals, int n_int, double* 
     if (!m || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
     master_launch(m, intervals, n_int, (hipStream_t)stream);
     master_finish(m, intervals, n_int, rows, dnorm, (hipStream_t)stream);
+  });
+}
+
+int qt_master_advance_dc(qt_master* m, double t_end, int dc, int max_rows, double* rows, int* n_rows, double* dnorm) {
+  return guard([&] {
+    if (!m || !rows || !n_rows || dc < 1 || max_rows < 1 || !std::isfinite(t_end)) throw Fail(QT_EINVAL, "bad arguments");
+    if (m->n != 1) throw Fail(QT_EINVAL, "dc mode drives a single density operator (one sweep point)");
+    if (t_end < m->time) throw Fail(QT_EINVAL, "t_end lies before the point's time");
+    *n_rows = 0;
+    if (m->time >= t_end) return;
+    const int no = (int)m->sys0->h->O.size();
+    std::vector<double> iv(max_rows, 0.);
+    master_launch(m, iv.data(), max_rows, nullptr, dc, t_end);
+    hipCheck(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    float ms = 0.f;
+    hipCheck(hipEventElapsedTime(&ms, m->e0, m->e1), "hipEventElapsedTime");
+    m->kernelMs += ms; m->launches += 1;
+    std::vector<double> tt(max_rows);
+    hipCheck(hipMemcpy(tt.data(), m->tout.p, tt.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+    const size_t nob = (size_t)max_rows * std::max(1, no);
+    const double* ob = m->hbuf;
+    const double* dd = m->hbuf + nob;
+    const double* dnv = m->hbuf + nob + max_rows;
+    int n = 0;
+    while (n < max_rows) {
+      double* r = rows + (size_t)n * (2 + no);
+      r[0] = tt[n]; r[1] = dd[n];
+      for (int q = 0; q < no; ++q) r[2 + q] = ob[(size_t)n * no + q];
+      for (int q = 0; q < no; ++q) if (!std::isfinite(r[2 + q])) throw Fail(QT_ENUMERIC, "Infinite detected in averages");
+      process_rows(*m->sys0->h, r + 2);
+      if (dnorm) dnorm[n] = dnv[n];
+      ++n;
+      if (tt[n - 1] >= t_end) break;
+    }
+    m->time = tt[n - 1];
+    *n_rows = n;
   });
 }
 

@@ -34,8 +34,12 @@ constexpr int kMasterWaves = 8;
 
 // K = M + M^dagger + sum_m J_m S J_m^dagger for the stage matrix S in LDS (Sr, Si planes);
 // kept out of line so that the six call sites per RK try share one register allocation.
-template <int NB>
-__device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout) {
+// TD: M = E (H_nH/i) E^-1 S with ph[k] = exp(-exz_k tau) (E^-1, applied to the B operand as it
+// is read) and phi[i] = exp(exz_i tau) (E, applied to the product's rows); the tables are made by
+// the caller.
+template <int NB, bool TD>
+__device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout,
+                                          const double2* ph, const double2* phi, const double2* rph) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB;
   constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -58,12 +62,60 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
       const int r = t / NB, c = t % NB;
       const double* a = hf + ((size_t)(r * KC + kk) * 64 + lane) * 2;
       const double ar = a[0], ai = a[1];
-      const double br = SrL[4 * kk * LD + 16 * c], bi = SiL[4 * kk * LD + 16 * c];
+      double br = SrL[4 * kk * LD + 16 * c], bi = SiL[4 * kk * LD + 16 * c];
+      if constexpr (TD) {  // B[k][j] = exp(-exz_k tau) S[k][j], k = 4 kk + lane / 16
+        const double2 e = ph[4 * kk + (lane >> 4)];
+        const double r0 = br;
+        br = e.x * r0 - e.y * bi;
+        bi = e.x * bi + e.y * r0;
+      }
       cr[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr[tw], 0, 0, 0);
       cr[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, cr[tw], 0, 0, 0);
       ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci[tw], 0, 0, 0);
       ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci[tw], 0, 0, 0);
     }
+  }
+  if constexpr (TD) {  // M = E M': row i times exp(exz_i tau)
+#pragma unroll
+    for (int tw = 0; tw < TW; ++tw) {
+      const int t = w + kMasterWaves * tw;
+      if (t >= NT) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 e = phi[16 * (t / NB) + 4 * v + (lane >> 4)];
+        const double xr = cr[tw][v], xi = ci[tw][v];
+        cr[tw][v] = e.x * xr - e.y * xi;
+        ci[tw][v] = e.x * xi + e.y * xr;
+      }
+    }
+    // the remainder: M_ij += sum_w rval_iw exp(rfreq tau) S_{rcol_iw, j}
+    if (M.wr > 0) {
+      const size_t rb = (size_t)p * M.wr * d;
+#pragma unroll
+      for (int tw = 0; tw < TW; ++tw) {
+        const int t = w + kMasterWaves * tw;
+        if (t >= NT) continue;
+        const int j = 16 * (t % NB) + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * (t / NB) + 4 * v + (lane >> 4);
+          if (i >= d || j >= d) continue;
+          double ax = 0., ay = 0.;
+          for (int a = 0; a < M.wr; ++a) {
+            const size_t o = rb + (size_t)a * d + i;
+            const double2 hv = cmul(M.rval[o], rph[M.rfi[o]]);
+            const int c = M.rcol[o];
+            const double sx = Sr[c * LD + j], sy = Si[c * LD + j];
+            ax += hv.x * sx - hv.y * sy;
+            ay += hv.x * sy + hv.y * sx;
+          }
+          cr[tw][v] += ax;
+          ci[tw][v] += ay;
+        }
+      }
+    }
+  } else {
+    (void)ph; (void)phi; (void)rph;
   }
   // N = M + (sum_m J_m S J_m^dagger) / 2, S still in LDS
   if (M.nj > 0) {
@@ -170,7 +222,7 @@ __device__ int autostop(const MasterDev& M, int p, const double* raw) {
 }
 
 
-template <int NB>
+template <int NB, bool TD>
 __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB, NN = DP * DP;
   constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;  // tiles per wavefront (at most)
@@ -178,6 +230,8 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   __shared__ double Si[DP * LD];
   __shared__ double red[kMasterWaves];
   __shared__ long long cnt[kCounters];
+  // interaction picture: exp(-exz tau), exp(exz tau) of the current stage; exp(-exz (t - t0))
+  __shared__ double2 ph[TD ? DP : 1], phi[TD ? DP : 1], rph[TD ? kMaxFreqs : 1];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int p = blockIdx.x;
@@ -229,9 +283,38 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
     return s;
   };
 
-  // Kout = F(S); S given element-wise by stage(tw, v)
-  auto derivative = [&](auto&& stage, double2* Kout) {
+  const double2* const exzP = TD ? M.exz + (size_t)p * d : nullptr;
+  // the phase tables of picture time tau (interaction picture); the caller's barrier orders them
+  auto setPhases = [&](double tau) {
+    if constexpr (TD) {
+      for (int i = tid; i < DP; i += 64 * kMasterWaves) {
+        double2 a = make_double2(1., 0.), b = make_double2(1., 0.);
+        if (i < d) {
+          const double2 z = exzP[i];
+          double sn, cs;
+          sincos(-z.y * tau, &sn, &cs);
+          const double e = exp(-z.x * tau);
+          a = make_double2(e * cs, e * sn);                 // exp(-z tau)
+          const double f = exp(z.x * tau);
+          b = make_double2(f * cs, -f * sn);                // exp(z tau)
+        }
+        ph[i] = a; phi[i] = b;
+      }
+      if (M.wr > 0 && tid < M.nrf) {  // exp(f tau) of the remainder's frequencies
+        const double2 f = M.rfreq[(size_t)p * M.nrf + tid];
+        double sn, cs;
+        sincos(f.y * tau, &sn, &cs);
+        const double e = exp(f.x * tau);
+        rph[tid] = make_double2(e * cs, e * sn);
+      }
+    } else {
+      (void)tau;
+    }
+  };
+  // Kout = F(S) at picture time tau; S given element-wise by stage(tw, v)
+  auto derivative = [&](auto&& stage, double2* Kout, double tau) {
     __syncthreads();
+    setPhases(tau);
     for (int tw = 0; tw < TW; ++tw) {
       if (!has(tw)) break;
 #pragma unroll
@@ -241,7 +324,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       }
     }
     __syncthreads();
-    master_apply<NB>(M, p, Sr, Si, Kout);
+    master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph);
   };
   auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
 
@@ -252,37 +335,41 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   __shared__ int asStop;
 
   for (int kI = 0; kI < M.n_intervals; ++kI) {
-    const double endTime = t + M.intervals[kI];
+    // Dt mode: advance(intervals[kI]); dc mode: up to dc steps step(t_end - t) per stream point
+    const double endTime = M.dc > 0 ? M.t_end : t + M.intervals[kI];
+    const int maxSteps = M.dc > 0 ? M.dc : -1;
     if (!done) {
-      for (;;) {
+      for (int steps = 0;; ++steps) {
         const double Dt = endTime - t;
-        if (Dt == 0.) break;
-        // Master::step -> ode_engine::Base::step(Dt)
+        if (Dt == 0. || steps == maxSteps) break;
+        // Master::step -> ode_engine::Base::step(Dt); the pictures coincide at the step's start (t0 = t)
         const double timeBefore = t;
+        const double t0 = t;
         const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
         if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
         if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-        derivative([&](int tw, int v) { return ld(X, tw, v); }, K1);  // k1 at (x, t); unchanged on retries
+        derivative([&](int tw, int v) { return ld(X, tw, v); }, K1, t - t0);  // k1 at (x, t); unchanged on retries
         long long tries = 0, failed = 0;
         for (;;) {
           ++tries;
           const double dt = dtTry;
-          derivative([&](int tw, int v) { return axpy(CK::a21 * dt, ld(K1, tw, v), ld(X, tw, v)); }, K2);
+          derivative([&](int tw, int v) { return axpy(CK::a21 * dt, ld(K1, tw, v), ld(X, tw, v)); }, K2,
+                     (t + CK::c2 * dt) - t0);
           derivative([&](int tw, int v) {
             return axpy(CK::a32 * dt, ld(K2, tw, v), axpy(CK::a31 * dt, ld(K1, tw, v), ld(X, tw, v)));
-          }, K3);
+          }, K3, (t + CK::c3 * dt) - t0);
           derivative([&](int tw, int v) {
             return axpy(CK::a43 * dt, ld(K3, tw, v), axpy(CK::a42 * dt, ld(K2, tw, v),
                    axpy(CK::a41 * dt, ld(K1, tw, v), ld(X, tw, v))));
-          }, K4);
+          }, K4, (t + CK::c4 * dt) - t0);
           derivative([&](int tw, int v) {
             return axpy(CK::a54 * dt, ld(K4, tw, v), axpy(CK::a53 * dt, ld(K3, tw, v), axpy(CK::a52 * dt, ld(K2, tw, v),
                    axpy(CK::a51 * dt, ld(K1, tw, v), ld(X, tw, v)))));
-          }, K5);
+          }, K5, (t + CK::c5 * dt) - t0);
           derivative([&](int tw, int v) {
             return axpy(CK::a65 * dt, ld(K5, tw, v), axpy(CK::a64 * dt, ld(K4, tw, v), axpy(CK::a63 * dt, ld(K3, tw, v),
                    axpy(CK::a62 * dt, ld(K2, tw, v), axpy(CK::a61 * dt, ld(K1, tw, v), ld(X, tw, v))))));
-          }, K6);
+          }, K6, (t + CK::c6 * dt) - t0);
           double emax = 0.;
           for (int tw = 0; tw < TW; ++tw) {
             if (!has(tw)) break;
@@ -309,12 +396,22 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
         }
         dtDid = t - timeBefore;
         if (nextDtTry != 0.) dtTry = nextDtTry;
-        // calculateTwoTimesRealPartOfSelf (CMatrix.cc:14-30), then trace renormalisation
+        // Exact part (Master.h:186-193): rho_ij <- u_i rho_ij conj(u_j), u = exp(-exz (t - t0)); then
+        // calculateTwoTimesRealPartOfSelf (CMatrix.cc:14-30) and the trace renormalisation
+        __syncthreads();
+        setPhases(t - t0);  // ph = u
         __syncthreads();
         for (int tw = 0; tw < TW; ++tw) {
           if (!has(tw)) break;
 #pragma unroll
-          for (int v = 0; v < 4; ++v) { const double2 y = ld(XN, tw, v); SrL[lofs(tw, v)] = y.x; SiL[lofs(tw, v)] = y.y; }
+          for (int v = 0; v < 4; ++v) {
+            double2 y = ld(XN, tw, v);
+            if constexpr (TD) {
+              const int i = rowOf(tw, v), j = colOf(tw);
+              if (i < d && j < d) y = cmul(cmul(ph[i], y), make_double2(ph[j].x, -ph[j].y));
+            }
+            SrL[lofs(tw, v)] = y.x; SiL[lofs(tw, v)] = y.y;
+          }
         }
         __syncthreads();
         double tr = 0.;
@@ -369,7 +466,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       const double v = bsum(part);
       if (tid == 0) M.obs_out[((size_t)p * M.n_intervals + kI) * M.no + q] = v;
     }
-    derivative([&](int tw, int v) { return ld(X, tw, v); }, K1);
+    derivative([&](int tw, int v) { return ld(X, tw, v); }, K1, 0.);  // t0 = t at a stream point
     double n2 = 0.;
     for (int tw = 0; tw < TW; ++tw) {
       if (!has(tw)) break;
@@ -378,6 +475,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
     }
     const double dn = sqrt(bsum(n2));
     if (tid == 0) {
+      if (M.time_out) M.time_out[(size_t)p * M.n_intervals + kI] = t;
       M.dtdid_out[(size_t)p * M.n_intervals + kI] = dtDid;
       M.dnorm_out[(size_t)p * M.n_intervals + kI] = dn;
     }
@@ -402,18 +500,23 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
 
 bool master_supported(int nb) { return nb >= 1 && nb <= 6; }
 
-hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st) {
+template <bool TD>
+static hipError_t launch_master_t(int nb, const MasterDev& M, hipStream_t st) {
   const dim3 block(64 * kMasterWaves);
   switch (nb) {
-    case 1: hipLaunchKernelGGL(master_kernel<1>, dim3(M.n_points), block, 0, st, M); break;
-    case 2: hipLaunchKernelGGL(master_kernel<2>, dim3(M.n_points), block, 0, st, M); break;
-    case 3: hipLaunchKernelGGL(master_kernel<3>, dim3(M.n_points), block, 0, st, M); break;
-    case 4: hipLaunchKernelGGL(master_kernel<4>, dim3(M.n_points), block, 0, st, M); break;
-    case 5: hipLaunchKernelGGL(master_kernel<5>, dim3(M.n_points), block, 0, st, M); break;
-    case 6: hipLaunchKernelGGL(master_kernel<6>, dim3(M.n_points), block, 0, st, M); break;
+    case 1: hipLaunchKernelGGL((master_kernel<1, TD>), dim3(M.n_points), block, 0, st, M); break;
+    case 2: hipLaunchKernelGGL((master_kernel<2, TD>), dim3(M.n_points), block, 0, st, M); break;
+    case 3: hipLaunchKernelGGL((master_kernel<3, TD>), dim3(M.n_points), block, 0, st, M); break;
+    case 4: hipLaunchKernelGGL((master_kernel<4, TD>), dim3(M.n_points), block, 0, st, M); break;
+    case 5: hipLaunchKernelGGL((master_kernel<5, TD>), dim3(M.n_points), block, 0, st, M); break;
+    case 6: hipLaunchKernelGGL((master_kernel<6, TD>), dim3(M.n_points), block, 0, st, M); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st) {
+  return M.exz ? launch_master_t<true>(nb, M, st) : launch_master_t<false>(nb, M, st);
 }
 
 }  // namespace qt

@@ -38,6 +38,24 @@ struct MasterDev {
   int* as_meta;          // [point][3]: started, queue size, queue head
   const int* proc_var;   // [no]: index of the mean whose square is subtracted, or -1
   const int* proc_sqrt;  // [no]: square root after the subtraction
+  // interaction picture (the unitary one, Master.h:73, QM_Picture.cc:45-47): H_nH/i at picture time
+  // tau is E (H_nH/i) E^-1 with E = diag(exp(exz tau)) (an entry |r><c| rotates with exz[r] - exz[c],
+  // the Exact frees' frequencies), and after each step rho <- U rho U^dagger with
+  // U = diag(exp(-exz (t - t0))) (Master.h:186-193); null in the Schroedinger picture
+  const double2* exz;    // [point][dim]
+  // entries of H_nH/i that do not rotate so (a term of an Exact free with frequencies the reference
+  // leaves out, Tridiagonal.tcc:61-67): a sparse remainder applied beside the product, entry w of
+  // row i is rval * exp(rfreq[rfi] tau) at column rcol
+  int wr;                // remainder width (0: none)
+  const int* rcol;       // [point][wr][dim]
+  const double2* rval;
+  const int* rfi;
+  int nrf;               // distinct remainder frequencies per point (<= kMaxFreqs)
+  const double2* rfreq;  // [point][nrf]
+  // dc mode (trajectory::run with dc > 0): each stream point follows up to dc Master steps toward t_end
+  int dc;
+  double t_end;
+  double* time_out;      // [point][n_intervals]: the point's time at each stream point
   double* obs_out;       // [point][n_intervals][no]
   double* dtdid_out;     // [point][n_intervals]
   double* dnorm_out;     // [point][n_intervals]: ||drho/dt||_F at the stream point

@@ -286,7 +286,10 @@ typedef struct {
 /* A batch of density operators, point k evolving under systems[k] (same dimension and jump
  * structure), all from rho0_host (d*d complex, row-major): master::make + Master
  * (CPPQEDcore/quantumtrajectory/Master.h:130-136, 50-208) for every sweep point.  The
- * (H_nH/i) rho products run on FP64 MFMA; dimensions up to 96. */
+ * (H_nH/i) rho products run on FP64 MFMA; dimensions up to 96.  Schroedinger picture, or the
+ * unitary interaction picture (UIP; the reference turns IP into UIP for the Master,
+ * QM_Picture.cc:45-47, and refuses non-unitary ones, Master.h:73): then each step ends with
+ * rho <- U rho U^dagger (Master.h:186-193). */
 int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_params* p, const double* rho0_host,
                      qt_master** out);
 int qt_master_destroy(qt_master* m);
@@ -296,6 +299,12 @@ int qt_master_destroy(qt_master* m);
  * ||drho/dt||_F at that stream point.  Either may be NULL. */
 int qt_master_advance(qt_master* m, const double* intervals, int n_intervals, double* rows_host, double* dnorm_host,
                       qt_stream stream);
+/* dc mode of the run loop for one density operator (a sweep of one point; cppqedutils::run with
+ * dc > 0, Trajectory.h:555-600): Master steps step(t_end - t) (Master.h:152-208), a row after every
+ * dc steps and at t_end, at most max_rows in this call; rows as qt_master_advance's (time, dtDid,
+ * processed averages), dnorm_host[r] = ||drho/dt||_F (may be NULL); *n_rows = rows written. */
+int qt_master_advance_dc(qt_master* m, double t_end, int dc, int max_rows, double* rows_host, int* n_rows,
+                         double* dnorm_host);
 /* rho of one point (d*d complex, row-major) */
 int qt_master_rho(qt_master* m, int point, double* rho_host);
 /* per point: steady-state flag, the time it was reached, current time, counters[8] */
