@@ -215,6 +215,7 @@ struct qt_sys {
   }
   DevBuf<int> hcol, jcol, ocol, oimag, owid, hfidx, hoff, pvar, psqrt;
   DevBuf<double2> hval, jval, oval, freq, exz;
+  DevBuf<double> jwts;   // rate weights [nj][dim] (SysDev::jw)
   int nfreq = 0;          // distinct interaction-picture frequencies of H (0: Schroedinger picture)
   DevBuf<float2> hval32;
   qt::SysDev dev{};
@@ -271,6 +272,28 @@ struct qt_sys {
     }
     if (c.empty()) { c.push_back(0); v.push_back(make_double2(0, 0)); }
     jcol.upload(c); jval.upload(v);
+    {  // rate weights (SysDev::jw) when every J_m^dagger J_m is diagonal
+      bool diag = !s.J.empty();
+      std::vector<double> wts;
+      for (auto& j : s.J) {
+        std::vector<double> wm(d, 0.);
+        std::vector<char> used(d, 0);
+        for (int r = 0; r < d && diag; ++r) {
+          int cnt = 0;
+          for (int k = 0; k < j.width; ++k) {
+            const qt::cd z = j.vals[(size_t)r * j.width + k];
+            if (z == qt::cd(0.)) continue;
+            const int col = j.cols[(size_t)r * j.width + k];
+            if (++cnt > 1 || used[col]) { diag = false; break; }
+            used[col] = 1;
+            wm[col] = std::norm(z);
+          }
+        }
+        wts.insert(wts.end(), wm.begin(), wm.end());
+      }
+      if (diag) jwts.upload(wts);
+      dev.jw = diag ? jwts.p : nullptr;
+    }
     int wo = 1;
     for (auto& o : s.O) wo = std::max(wo, o.width);
     c.clear(); v.clear();

@@ -390,6 +390,8 @@ struct Engine {
 
   // LDS scratch [8][64] for the batched stream-point reduction (one-wavefront teams; may be null)
   double* ored = nullptr;
+  // LDS copy of the first two channels' rate weights, [2][ROWS * G] (SysDev::jw; may be null)
+  const double* jwl = nullptr;
 
   // all observables of a stream point at once (one wavefront with ored): each lane writes its
   // partials of eight observables to LDS, lane 8j + c adds the partials of lanes 8c..8c+7 for
@@ -627,11 +629,18 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     eng.sync();  // X complete before other lanes gather from it
   };
 
+  const double* jw = S.jw;
   auto computeRates = [&]() {
     for (int m = 0; m < nj; ++m) {
       double sacc = 0.;
+      if (jw) {  // rate weights: no gather
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+        for (int r = 0; r < ROWS; ++r)
+          if (eng.valid(r)) sacc += gbl(jw)[(unsigned)(m * S.dim + r * G + lane)] * cabs2(X[r * G + lane]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
+      }
       const double v = eng.gsum(sacc);
       if (lane == 0) rts[m] = v;
     }
@@ -828,9 +837,38 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   // state (MCWF_Trajectory.h:353-356): ||J_m psi/|psi| ||^2 = ||J_m psi||^2 / |psi|^2, so the norm
   // and the rates come from one pass over the staged unnormalised state and one joint reduction.
   // psi stays staged unnormalised in ps; jumpScale = 1/|psi| converts J ps to J psi.
+  // With rate weights (SysDev::jw, up to two channels) the rates come from the lane's own rows,
+  // nothing is staged and the rates stay in registers (rw0, rw1); a jump then stages psi.
   double jumpScale = 1.;
+  const bool useW = S.jw != nullptr && nj <= 2;
+  // the weights of the lane's rows: from the team's LDS copy (eng.jwl, [2][ROWS * G]) or global memory
+  auto weight = [&](int m, int r) {
+    const int row = r * G + lane;
+    if (eng.jwl) return eng.jwl[m * ROWS * G + row];
+    return m < nj && row < S.dim ? gbl(S.jw)[(unsigned)(m * S.dim + row)] : 0.;
+  };
+  double rw0 = 0., rw1 = 0.;
+  auto rate = [&](int m) { return useW ? (m == 0 ? rw0 : rw1) : rts[m]; };
   auto renormRates = [&](bool withRates) {
     double part[3] = {0., 0., 0.};
+    if (withRates && useW) {
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const double a = cabs2(x[r]);
+        part[0] += a;
+        part[1] += weight(0, r) * a;
+        part[2] += weight(1, r) * a;
+      }
+      wave_sum3(part);
+      const double inv = fast_rsqrt(part[0]);
+      const double inv2 = inv * inv;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) { x[r].x *= inv; x[r].y *= inv; }
+      rw0 = part[1] * inv2;
+      rw1 = part[2] * inv2;
+      jumpScale = 1.;  // a jump stages the renormalised psi
+      return;
+    }
     const bool fused = withRates && nj <= 2;
     if (withRates) eng.stagePsi(x);
 #pragma unroll
@@ -888,7 +926,7 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
       if (nj > 0) {
         for (;;) {  // manageTimeStep (MCWF_Trajectory.h:294-317)
           double totalRate = 0.;
-          for (int m = 0; m < nj; ++m) totalRate += rts[m];
+          for (int m = 0; m < nj; ++m) totalRate += rate(m);
           // Liouvillean::rates goes through LiouvilleanAveragedCommon::average, which refuses
           // non-finite values (LiouvilleanAveragedCommon.h:82-93): flag the launch
           if (!isfinite(totalRate) && lane == 0) atomicOr(E.err_flag, 2);
@@ -915,12 +953,13 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
         ++ord;
         double random = u / dtDid;
         int no = 0;
-        for (; random > 0. && no != nj; random -= rts[no++]) ;
+        for (; random > 0. && no != nj; random -= rate(no++)) ;
         if (random < 0.) {
           --no;
-          const double nf = sqrt(rts[no]);
+          const double nf = sqrt(rate(no));
           if (!isfinite(nf) && lane == 0) atomicOr(E.err_flag, 1);
-          const double inv = jumpScale / nf;  // ps holds psi before the renorm
+          if (useW) eng.stagePsi(x);
+          const double inv = jumpScale / nf;  // ps holds psi before the renorm (after it with weights)
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
             const double2 y = eng.valid(r) ? eng.jumpRow(no, r) : make_double2(0., 0.);
@@ -1018,8 +1057,20 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     __shared__ double2 ps[ROWS * G];
     __shared__ double2 pc[kPcLds ? ROWS * G : 1];
     __shared__ double ored[8 * 64];
+    __shared__ double jwl[2 * ROWS * G];  // rate weights of the system (the same for every trajectory)
     Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane, ph);
     eng.ored = ored;
+    if (S.jw && S.nj <= 2) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          const int row = r * G + lane;
+          jwl[m * ROWS * G + row] = m < S.nj && row < S.dim ? gbl(S.jw)[(unsigned)(m * S.dim + row)] : 0.;
+        }
+      eng.jwl = jwl;
+      wave_sync();
+    }
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds, GSL>(S, E, eng, traj, pc, rts, cnt);
   }
 }

@@ -32,6 +32,10 @@ struct SysDev {
   int nj, wj;
   const int* jcol;         // [nj][wj][dim]
   const double2* jval;
+  // rate weights [nj][dim] when every J_m^dagger J_m is diagonal (each row of J_m has at most one
+  // entry and no two rows share a column: a, sigma, ...): rate_m = sum_c jw[m][c] |psi_c|^2, the
+  // reference's own form of the rates (Mode_.h:180-184); null otherwise
+  const double* jw;
   int no, wo;
   const int* ocol;         // [no][wo][dim]
   const double2* oval;
