@@ -419,7 +419,8 @@ struct qt_master {
   DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn, tout;
   DevBuf<double2> exz;             // [point][d] (interaction picture only)
   bool td = false;
-  int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product (interaction picture)
+  int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product
+  bool sharedH = false;            // one copy of the fragments for every point
   DevBuf<int> rcol, rfi;
   DevBuf<double2> rval, rfreq;
   DevBuf<int> jcol, done;
@@ -1405,28 +1406,25 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     m->n = n_points; m->d = d; m->nb = NB; m->p = *p;
     m->sys0 = systems[0];
     // H_nH/i A fragments: [point][NB][KC][64] {re, im}, H[16 r + l%16][4 kk + l/16]
-    std::vector<double> hf((size_t)n_points * NB * KC * 64 * 2, 0.);
+    // per point: the part of H_nH/i that rotates with the Exact exponents (all of it in the
+    // Schroedinger picture) as a dense matrix -- the MFMA product's A operand -- and the remainder,
+    // entries applied beside the product with their own frequencies (master_entry_rotates)
+    struct RestEntry { int r, c; qt::cd v, f; };
+    std::vector<std::vector<qt::cd>> Hrot(n_points);
+    std::vector<std::vector<RestEntry>> rest(n_points);
     std::vector<int> jc((size_t)n_points * std::max(1, nj) * wj * d, 0);
     std::vector<double2> jv((size_t)n_points * std::max(1, nj) * wj * d, make_double2(0., 0.));
     std::vector<double> dtTry(n_points);
-    std::vector<std::vector<std::pair<int, int>>> rest(n_points);  // (row, table index) of the remainder
     for (int k = 0; k < n_points; ++k) {
       const qt::System& s = *systems[k]->h;
-      std::vector<qt::cd> H((size_t)DP * DP, 0.);
+      Hrot[k].assign((size_t)DP * DP, 0.);
       for (int r = 0; r < d; ++r)
         for (int w = 0; w < s.H.width; ++w) {
           const size_t i = (size_t)r * s.H.width + w;
           if (s.H.vals[i] == qt::cd(0.)) continue;
-          if (master_entry_rotates(s, r, i)) H[(size_t)r * DP + s.H.cols[i]] += s.H.vals[i];
-          else rest[k].push_back({r, (int)i});
+          if (master_entry_rotates(s, r, i)) Hrot[k][(size_t)r * DP + s.H.cols[i]] += s.H.vals[i];
+          else rest[k].push_back({r, s.H.cols[i], s.H.vals[i], s.H.freqs.empty() ? qt::cd(0.) : s.H.freqs[i]});
         }
-      for (int rb = 0; rb < NB; ++rb)
-        for (int kk = 0; kk < KC; ++kk)
-          for (int l = 0; l < 64; ++l) {
-            const qt::cd z = H[(size_t)(16 * rb + l % 16) * DP + 4 * kk + l / 16];
-            const size_t o = ((((size_t)k * NB + rb) * KC + kk) * 64 + l) * 2;
-            hf[o] = z.real(); hf[o + 1] = z.imag();
-          }
       for (int mI = 0; mI < nj; ++mI) {
         const qt::Ell& e = s.J[mI];
         for (int w = 0; w < e.width; ++w)
@@ -1439,8 +1437,28 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
       }
       dtTry[k] = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
     }
-    m->hfrag.upload(hf); m->jcol.upload(jc); m->jval.upload(jv);
     for (int k = 0; k < n_points; ++k) m->td = m->td || systems[k]->h->hasEx;
+    auto widthOf = [&](const std::vector<RestEntry>& v) {
+      std::vector<int> per(d, 0);
+      int w = 0;
+      for (auto& e : v) w = std::max(w, ++per[e.r]);
+      return w;
+    };
+    // (one shared copy of the fragments with the per-point differences as remainder was measured
+    // slower on the 512-point sweep, 591 k vs 650 k tries/s: the fragments are not what bounds it)
+    const bool shared = false;
+    m->sharedH = shared;
+    const int nfrag = shared ? 1 : n_points;
+    std::vector<double> hf((size_t)nfrag * NB * KC * 64 * 2, 0.);
+    for (int k = 0; k < nfrag; ++k)
+      for (int rb = 0; rb < NB; ++rb)
+        for (int kk = 0; kk < KC; ++kk)
+          for (int l = 0; l < 64; ++l) {
+            const qt::cd z = Hrot[k][(size_t)(16 * rb + l % 16) * DP + 4 * kk + l / 16];
+            const size_t o = ((((size_t)k * NB + rb) * KC + kk) * 64 + l) * 2;
+            hf[o] = z.real(); hf[o + 1] = z.imag();
+          }
+    m->hfrag.upload(hf); m->jcol.upload(jc); m->jval.upload(jv);
     if (m->td) {  // the Exact frees' exponents per point (zero in the Schroedinger picture)
       std::vector<double2> z((size_t)n_points * d, make_double2(0., 0.));
       for (int k = 0; k < n_points; ++k) {
@@ -1449,20 +1467,19 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
         for (int i = 0; i < d; ++i) z[(size_t)k * d + i] = make_double2(sk.exactZ[i].real(), sk.exactZ[i].imag());
       }
       m->exz.upload(z);
-      // the remainder as ELL [point][wr][d] with per-point frequency tables
+    }
+    // the remainder as ELL [point][wr][d] with per-point frequency tables
+    {
       int wr = 0, nrf = 1;
       std::vector<std::vector<qt::cd>> fl(n_points, std::vector<qt::cd>{qt::cd(0.)});
       for (int k = 0; k < n_points; ++k) {
-        std::vector<int> per(d, 0);
-        for (auto& e : rest[k]) wr = std::max(wr, ++per[e.first]);
-        const qt::System& sk = *systems[k]->h;
-        for (auto& e : rest[k]) {
-          const qt::cd f = sk.H.freqs.empty() ? qt::cd(0.) : sk.H.freqs[e.second];
-          if (std::find(fl[k].begin(), fl[k].end(), f) == fl[k].end()) fl[k].push_back(f);
-        }
+        wr = std::max(wr, widthOf(rest[k]));
+        for (auto& e : rest[k])
+          if (std::find(fl[k].begin(), fl[k].end(), e.f) == fl[k].end()) fl[k].push_back(e.f);
         nrf = std::max(nrf, (int)fl[k].size());
       }
       if (nrf > qt::kMaxFreqs) throw Fail(QT_ENOTSUP, "interaction picture with more than 32 distinct frequencies");
+      if (!m->td && nrf > 1) throw Fail(QT_EINVAL, "internal: a Schroedinger-picture remainder with frequencies");
       m->wr = wr;
       m->nrf = nrf;
       if (wr > 0) {
@@ -1470,15 +1487,13 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
         std::vector<double2> rv((size_t)n_points * wr * d, make_double2(0., 0.));
         std::vector<double2> rfq((size_t)n_points * nrf, make_double2(0., 0.));
         for (int k = 0; k < n_points; ++k) {
-          const qt::System& sk = *systems[k]->h;
           std::vector<int> per(d, 0);
           for (auto& e : rest[k]) {
-            const int r = e.first, a = per[r]++;
-            const size_t o = ((size_t)k * wr + a) * d + r;
-            const qt::cd f = sk.H.freqs.empty() ? qt::cd(0.) : sk.H.freqs[e.second];
-            rc[o] = sk.H.cols[e.second];
-            rv[o] = make_double2(sk.H.vals[e.second].real(), sk.H.vals[e.second].imag());
-            rf[o] = (int)(std::find(fl[k].begin(), fl[k].end(), f) - fl[k].begin());
+            const int a = per[e.r]++;
+            const size_t o = ((size_t)k * wr + a) * d + e.r;
+            rc[o] = e.c;
+            rv[o] = make_double2(e.v.real(), e.v.imag());
+            rf[o] = (int)(std::find(fl[k].begin(), fl[k].end(), e.f) - fl[k].begin());
           }
           for (size_t q = 0; q < fl[k].size(); ++q) rfq[(size_t)k * nrf + q] = make_double2(fl[k][q].real(), fl[k][q].imag());
         }
@@ -1550,6 +1565,7 @@ static void master_launch(qt_master* m, const double* intervals, int n_int, hipS
     M.intervals = m->intervals.p; M.n_intervals = n_int;
     M.obs_out = m->obs.p; M.dtdid_out = m->dtd.p; M.dnorm_out = m->dn.p;
     M.exz = m->td ? m->exz.p : nullptr;
+    M.shared_h = m->sharedH ? 1 : 0;
     M.wr = m->wr; M.rcol = m->rcol.p; M.rval = m->rval.p; M.rfi = m->rfi.p; M.nrf = m->nrf; M.rfreq = m->rfreq.p;
     M.dc = dc; M.t_end = t_end;
     if (dc > 0) { m->tout.alloc((size_t)m->n * n_int); M.time_out = m->tout.p; }

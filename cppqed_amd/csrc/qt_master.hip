@@ -32,19 +32,40 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMasterWaves = 8;
 
-// K = M + M^dagger + sum_m J_m S J_m^dagger for the stage matrix S in LDS (Sr, Si planes);
-// kept out of line so that the six call sites per RK try share one register allocation.
+// K = M + M^dagger + sum_m J_m S J_m^dagger for the stage matrix S in LDS (Sr, Si planes); kept out
+// of line so that its three call sites (a step's first derivative, the stage loop, the stream
+// point) share one register allocation (inlined, the 512-point d = 80 sweep ran 632 k instead of
+// 654 k tries/s).
 // TD: M = E (H_nH/i) E^-1 S with ph[k] = exp(-exz_k tau) (E^-1, applied to the B operand as it
 // is read) and phi[i] = exp(exz_i tau) (E, applied to the product's rows); the tables are made by
 // the caller.
+// Cash-Karp tableau by stage index (stage s = 0..5 computes K_{s+1}): a_{s+1, i+1} and c_{s+1}
+__device__ __forceinline__ double ck_a(int s, int i) {
+  constexpr double A[6][5] = {{0., 0., 0., 0., 0.},
+                              {CK::a21, 0., 0., 0., 0.},
+                              {CK::a31, CK::a32, 0., 0., 0.},
+                              {CK::a41, CK::a42, CK::a43, 0., 0.},
+                              {CK::a51, CK::a52, CK::a53, CK::a54, 0.},
+                              {CK::a61, CK::a62, CK::a63, CK::a64, CK::a65}};
+  return A[s][i];
+}
+__device__ __forceinline__ double ck_c(int s) {
+  constexpr double C[6] = {0., CK::c2, CK::c3, CK::c4, CK::c5, CK::c6};
+  return C[s];
+}
+
+// one entry of the remainder as kept in LDS (the point's copy, master_kernel)
+struct RemEntry { double2 v; int c, f; };
+
 template <int NB, bool TD>
 __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout,
-                                          const double2* ph, const double2* phi, const double2* rph) {
+                                          const double2* ph, const double2* phi, const double2* rph,
+                                          const RemEntry* remL) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB;
   constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int d = M.dim;
-  const double* const hf = M.hfrag + (size_t)p * NB * KC * 64 * 2;
+  const double* const hf = M.hfrag + (M.shared_h ? (size_t)0 : (size_t)p * NB * KC * 64 * 2);
   double* const SrL = Sr + (lane >> 4) * LD + (lane & 15);
   double* const SiL = Si + (lane >> 4) * LD + (lane & 15);
   const double* const SrT = Sr + (lane & 15) * LD + (lane >> 4);
@@ -52,16 +73,25 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
   d4 cr[TW], ci[TW];
 #pragma unroll
   for (int tw = 0; tw < TW; ++tw) { cr[tw] = d4{0., 0., 0., 0.}; ci[tw] = d4{0., 0., 0., 0.}; }
-  // M = (H_nH/i) S on FP64 MFMA
-#pragma unroll 1
-  for (int kk = 0; kk < KC; ++kk) {
+  // M = (H_nH/i) S on FP64 MFMA.  The A fragments come from global memory (L2 / MALL): each
+  // k-step's are loaded one step ahead (two named buffers, KC is even), so their latency hides
+  // behind the MFMAs of the step before instead of stalling every step.
+  auto loadA = [&](int kk, double2 (&a)[TW]) {
+#pragma unroll
+    for (int tw = 0; tw < TW; ++tw) {
+      const int t = w + kMasterWaves * tw;
+      if (t >= NT || kk >= KC) { a[tw] = make_double2(0., 0.); continue; }
+      const double* q = hf + ((size_t)((t / NB) * KC + kk) * 64 + lane) * 2;
+      a[tw] = make_double2(q[0], q[1]);
+    }
+  };
+  auto mfmaStep = [&](int kk, const double2 (&a)[TW]) {
 #pragma unroll
     for (int tw = 0; tw < TW; ++tw) {
       const int t = w + kMasterWaves * tw;
       if (t >= NT) continue;
-      const int r = t / NB, c = t % NB;
-      const double* a = hf + ((size_t)(r * KC + kk) * 64 + lane) * 2;
-      const double ar = a[0], ai = a[1];
+      const int c = t % NB;
+      const double ar = a[tw].x, ai = a[tw].y;
       double br = SrL[4 * kk * LD + 16 * c], bi = SiL[4 * kk * LD + 16 * c];
       if constexpr (TD) {  // B[k][j] = exp(-exz_k tau) S[k][j], k = 4 kk + lane / 16
         const double2 e = ph[4 * kk + (lane >> 4)];
@@ -74,6 +104,16 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
       ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci[tw], 0, 0, 0);
       ci[tw] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci[tw], 0, 0, 0);
     }
+  };
+  static_assert(KC % 2 == 0, "k-steps come in pairs");
+  double2 a0[TW], a1[TW];
+  loadA(0, a0);
+#pragma unroll 1
+  for (int kk = 0; kk < KC; kk += 2) {
+    loadA(kk + 1, a1);
+    mfmaStep(kk, a0);
+    loadA(kk + 2, a0);
+    mfmaStep(kk + 1, a1);
   }
   if constexpr (TD) {  // M = E M': row i times exp(exz_i tau)
 #pragma unroll
@@ -88,35 +128,45 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
         ci[tw][v] = e.x * xi + e.y * xr;
       }
     }
-    // the remainder: M_ij += sum_w rval_iw exp(rfreq tau) S_{rcol_iw, j}
-    if (M.wr > 0) {
-      const size_t rb = (size_t)p * M.wr * d;
+  } else {
+    (void)ph; (void)phi;
+  }
+  // the remainder: M_ij += sum_w rval_iw exp(rfreq tau) S_{rcol_iw, j} (frequencies in the
+  // interaction picture only)
+  if (M.wr > 0) {
+    const size_t rb = (size_t)p * M.wr * d;
 #pragma unroll
-      for (int tw = 0; tw < TW; ++tw) {
-        const int t = w + kMasterWaves * tw;
-        if (t >= NT) continue;
-        const int j = 16 * (t % NB) + (lane & 15);
+    for (int tw = 0; tw < TW; ++tw) {
+      const int t = w + kMasterWaves * tw;
+      if (t >= NT) continue;
+      const int j = 16 * (t % NB) + (lane & 15);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 16 * (t / NB) + 4 * v + (lane >> 4);
-          if (i >= d || j >= d) continue;
-          double ax = 0., ay = 0.;
-          for (int a = 0; a < M.wr; ++a) {
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * (t / NB) + 4 * v + (lane >> 4);
+        if (i >= d || j >= d) continue;
+        double ax = 0., ay = 0.;
+        for (int a = 0; a < M.wr; ++a) {
+          double2 hv;
+          int c, f;
+          if (remL) {
+            const RemEntry& e = remL[a * d + i];
+            hv = e.v; c = e.c; f = e.f;
+          } else {
             const size_t o = rb + (size_t)a * d + i;
-            const double2 hv = cmul(M.rval[o], rph[M.rfi[o]]);
-            const int c = M.rcol[o];
-            const double sx = Sr[c * LD + j], sy = Si[c * LD + j];
-            ax += hv.x * sx - hv.y * sy;
-            ay += hv.x * sy + hv.y * sx;
+            hv = M.rval[o]; c = M.rcol[o]; f = M.rfi[o];
           }
-          cr[tw][v] += ax;
-          ci[tw][v] += ay;
+          if constexpr (TD) hv = cmul(hv, rph[f]);
+          else (void)f;
+          const double sx = Sr[c * LD + j], sy = Si[c * LD + j];
+          ax += hv.x * sx - hv.y * sy;
+          ay += hv.x * sy + hv.y * sx;
         }
+        cr[tw][v] += ax;
+        ci[tw][v] += ay;
       }
     }
-  } else {
-    (void)ph; (void)phi; (void)rph;
   }
+  if constexpr (!TD) (void)rph;
   // N = M + (sum_m J_m S J_m^dagger) / 2, S still in LDS
   if (M.nj > 0) {
 #pragma unroll
@@ -232,12 +282,22 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   __shared__ long long cnt[kCounters];
   // interaction picture: exp(-exz tau), exp(exz tau) of the current stage; exp(-exz (t - t0))
   __shared__ double2 ph[TD ? DP : 1], phi[TD ? DP : 1], rph[TD ? kMaxFreqs : 1];
+  // the point's remainder when it fits (read by every derivative)
+  constexpr int kRemBytes = NB <= 5 ? 8192 : 4096;
+  __shared__ RemEntry remS[kRemBytes / sizeof(RemEntry)];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int p = blockIdx.x;
   if (p >= M.n_points) return;
   const int d = M.dim;
-  const double* const hf = M.hfrag + (size_t)p * NB * KC * 64 * 2;
+  const RemEntry* remL = nullptr;
+  if (M.wr > 0 && (size_t)M.wr * d <= kRemBytes / sizeof(RemEntry)) {
+    for (int e = tid; e < M.wr * d; e += 64 * kMasterWaves) {
+      const size_t o = (size_t)p * M.wr * d + e;
+      remS[e] = RemEntry{M.rval[o], M.rcol[o], M.rfi[o]};
+    }
+    remL = remS;  // ordered before its first use by the derivative's barrier
+  }
 
   double2* const X = M.x + (size_t)p * NN;
   double2* const K = M.k + (size_t)p * 7 * NN;
@@ -324,7 +384,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       }
     }
     __syncthreads();
-    master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph);
+    master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph, remL);
   };
   auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
 
@@ -353,24 +413,17 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
         for (;;) {
           ++tries;
           const double dt = dtTry;
-          derivative([&](int tw, int v) { return axpy(CK::a21 * dt, ld(K1, tw, v), ld(X, tw, v)); }, K2,
-                     (t + CK::c2 * dt) - t0);
-          derivative([&](int tw, int v) {
-            return axpy(CK::a32 * dt, ld(K2, tw, v), axpy(CK::a31 * dt, ld(K1, tw, v), ld(X, tw, v)));
-          }, K3, (t + CK::c3 * dt) - t0);
-          derivative([&](int tw, int v) {
-            return axpy(CK::a43 * dt, ld(K3, tw, v), axpy(CK::a42 * dt, ld(K2, tw, v),
-                   axpy(CK::a41 * dt, ld(K1, tw, v), ld(X, tw, v))));
-          }, K4, (t + CK::c4 * dt) - t0);
-          derivative([&](int tw, int v) {
-            return axpy(CK::a54 * dt, ld(K4, tw, v), axpy(CK::a53 * dt, ld(K3, tw, v), axpy(CK::a52 * dt, ld(K2, tw, v),
-                   axpy(CK::a51 * dt, ld(K1, tw, v), ld(X, tw, v)))));
-          }, K5, (t + CK::c5 * dt) - t0);
-          derivative([&](int tw, int v) {
-            return axpy(CK::a65 * dt, ld(K5, tw, v), axpy(CK::a64 * dt, ld(K4, tw, v), axpy(CK::a63 * dt, ld(K3, tw, v),
-                   axpy(CK::a62 * dt, ld(K2, tw, v), axpy(CK::a61 * dt, ld(K1, tw, v), ld(X, tw, v))))));
-          }, K6, (t + CK::c6 * dt) - t0);
+          // stages 2..6: S = X + dt sum_{i<s} a_si K_i, accumulated in the tableau's order
+#pragma unroll 1
+          for (int st = 1; st < 6; ++st) {
+            derivative([&](int tw, int v) {
+              double2 acc = ld(X, tw, v);
+              for (int i = 0; i < st; ++i) acc = axpy(ck_a(st, i) * dt, ld(K + (size_t)i * NN, tw, v), acc);
+              return acc;
+            }, K + (size_t)st * NN, (t + ck_c(st) * dt) - t0);
+          }
           double emax = 0.;
+#pragma unroll 1
           for (int tw = 0; tw < TW; ++tw) {
             if (!has(tw)) break;
 #pragma unroll

@@ -10,7 +10,8 @@ constexpr int kMaxAutostopAvr = 64;  // averages a point's autostop can watch
 struct MasterDev {
   int n_points;
   int dim;               // true dimension d
-  const double* hfrag;   // [point][NB][DP/4][64] x {re, im}: A fragments of H_nH/i
+  const double* hfrag;   // [point][NB][DP/4][64] x {re, im}: A fragments of H_nH/i ([1][...] when shared_h)
+  int shared_h;          // 1: every point's product uses the one copy (point 0's); its differences are remainder
   int nj, wj;
   const int* jcol;       // [point][nj][wj][dim] ELL of the jump operators
   const double2* jval;
