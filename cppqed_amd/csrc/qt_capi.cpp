@@ -32,9 +32,10 @@ hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, b
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
                              int n_traj, int cap, double* draws, hipStream_t st);
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw);
-int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32);
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl);
 int mixed_variants(int* table, int cap);
-hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st);
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
+                        const MixDev* M, hipStream_t st);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t st);
 hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
@@ -640,6 +641,8 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     sys->checkPictureSupported();
     if (sys->interactionPicture() && p->layout_group_lanes > 0)
       throw Fail(QT_ENOTSUP, "sub-wave layouts run the Schroedinger picture only");
+    if (p->controller == QT_CONTROLLER_GSL && p->layout_group_lanes > 0)
+      throw Fail(QT_ENOTSUP, "sub-wave layouts run the odeint controller only");
     std::unique_ptr<qt_ens> e(new qt_ens);
     e->sys = sys; e->n = n_traj; e->p = *p;
     if (e->p.jump_log_cap < 1) e->p.jump_log_cap = 1;
@@ -650,8 +653,6 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     } else {
       e->lay = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
       const bool gsl = p->controller == QT_CONTROLLER_GSL;
-      if (gsl && e->lay.nw != 1)
-        throw Fail(QT_ENOTSUP, "the GSL controller runs in the one-wavefront engine (dimensions up to 256)");
       if (sys->interactionPicture() || gsl) {
         // interaction-picture and GSL variants: no occupancy hint, the ELL width may round up
         e->lay.occ = 1;
@@ -665,8 +666,6 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
           throw Fail(QT_ENOTSUP, "requested occupancy variant is not compiled in");
       }
     }
-    if (p->controller == QT_CONTROLLER_GSL && useSub)
-      throw Fail(QT_ENOTSUP, "the GSL controller runs in the one-wavefront engine (dimensions up to 256)");
     e->psi.alloc((size_t)n_traj * d);
     std::vector<double2> rep((size_t)n_traj * d);
     for (int k = 0; k < n_traj; ++k)
@@ -1171,6 +1170,8 @@ struct qt_batch_s {
   int variant = 0;
   std::vector<int> team;
   bool f32 = false;
+  bool td = false;              // some member runs in the interaction picture
+  bool gsl = false;             // some member uses GSL's controller
   DevBuf<qt::MixDev> mix;
   DevBuf<int> queue;            // [3] queue heads + [1] barrier error flag
   qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
@@ -1185,6 +1186,11 @@ struct qt_batch_s {
     if (e1) (void)hipEventDestroy(e1);
   }
   void choose(int v, const int* req, int policy = 0) {
+    td = gsl = false;
+    for (auto* e : mem) {
+      td = td || e->sys->interactionPicture();
+      gsl = gsl || e->p.controller == QT_CONTROLLER_GSL;
+    }
     auto V = mix_variants();
     if (v >= (int)V.size()) throw Fail(QT_EINVAL, "unknown batch layout variant");
     if (v >= 0) {
@@ -1204,7 +1210,7 @@ struct qt_batch_s {
       hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
       cus = prop.multiProcessorCount;
     }
-    int per = qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, f32);
+    int per = qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, f32, td, gsl);
     grid = cus * std::max(1, per);
   }
 };
@@ -1220,10 +1226,6 @@ int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params
     std::unique_ptr<qt_batch> b(new qt_batch);
     for (int m = 0; m < n_members; ++m) {
       if (!members[m]) throw Fail(QT_EINVAL, "null member");
-      if (members[m]->sys->interactionPicture())
-        throw Fail(QT_ENOTSUP, "mixed batches run the Schroedinger picture only");
-      if (members[m]->p.controller != QT_CONTROLLER_ODEINT)
-        throw Fail(QT_ENOTSUP, "mixed batches run the odeint controller only");
       b->mem.push_back(members[m]);
     }
     b->f32 = members[0]->p.precision == QT_FP32_DERIV;
@@ -1286,7 +1288,7 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   // items then spread over all compute units instead of filling a few
   const int grid = std::max(1, b->grid);
   hipCheck(hipEventRecord(b->e0, st), "hipEventRecord");
-  hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, b->f32, grid, b->mix.p, st), "mixed launch");
+  hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, b->f32, b->td, b->gsl, grid, b->mix.p, st), "mixed launch");
   hipCheck(hipEventRecord(b->e1, st), "hipEventRecord");
   for (size_t m = 0; m < b->mem.size(); ++m) {
     b->mem[m]->lastIntervals = n_int[m];
@@ -1948,7 +1950,6 @@ int qt_server_create(const qt_server_params* p, qt_server** out) {
     if (p->chunk_intervals < 1) throw Fail(QT_EINVAL, "chunk_intervals must be positive");
     if (p->hbm_budget == 0) throw Fail(QT_EINVAL, "zero memory budget");
     check_params(&p->mcwf);
-    if (p->mcwf.controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "the request loop runs the odeint controller only");
     std::unique_ptr<qt_server> s(new qt_server);
     s->P = *p;
     s->batch.f32 = p->mcwf.precision == QT_FP32_DERIV;
@@ -1968,7 +1969,7 @@ int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0, 
   return guard([&] {
     if (!s || !sys || !psi0 || n_traj <= 0 || !(T > 0) || !(Dt > 0)) throw Fail(QT_EINVAL, "bad arguments");
     const qt::System& h = *sys->h;
-    if (h.H.timeDependent() || h.hasEx) throw Fail(QT_ENOTSUP, "the request loop runs the Schroedinger picture only");
+    sys->checkPictureSupported();
     std::unique_ptr<Request> r(new Request);
     r->id = s->nextId++;
     r->sys = sys; r->n = n_traj;

@@ -477,6 +477,9 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
 
   // One RKCK54 try from X with k1 = (H/i) X; returns the odeint error norm and, when the try is
   // accepted (norm <= 1), replaces the lane's rows of X by the 5th-order solution.
+  // GSL: GSL's controller instead -- X always takes the 5th-order solution, and the return value
+  // is GSL's error ratio (not squared), which needs the derivative at the new point
+  template <bool GSL = false>
   __device__ double tryStepX(const T k1[ROWS], double dt, double epsAbs, double epsRel, long long& derivs,
                              double t = 0., double t0 = 0.) const {
     T k2[ROWS], k3[ROWS], k4[ROWS], k5[ROWS], k6[ROWS];
@@ -510,6 +513,32 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
     B::stage(v); this->setTime((t + CK::c6 * dt) - t0); B::gather(k6);
     derivs += 5;
     const double q1 = CK::b1 * dt, q3 = CK::b3 * dt, q4 = CK::b4 * dt, q6 = CK::b6 * dt;
+    if constexpr (GSL) {
+      // ODE_GSL.h:44-84: rkck's yerr, the new state, its derivative (dydt_out), then
+      // max over the real components of |yerr| / (epsAbs + epsRel (|y| + |dt dydt|))
+      const double f1 = CK::e1 * dt, f3 = CK::e3 * dt, f4 = CK::e4 * dt, f5 = CK::e5 * dt, f6 = CK::e6 * dt;
+      double2 xe[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (!B::live(r)) continue;
+        const double2 d1 = kd(k1[r]), d3 = kd(k3[r]), d4 = kd(k4[r]), d5 = kd(k5[r]), d6 = kd(k6[r]);
+        xe[r] = axpy(f6, d6, axpy(f5, d5, axpy(f4, d4, axpy(f3, d3, make_double2(f1 * d1.x, f1 * d1.y)))));
+        X[r * G + lane] = axpy(q6, d6, axpy(q4, d4, axpy(q3, d3, axpy(q1, d1, xr(r)))));
+      }
+      T kg[ROWS];
+      derivativeX(kg, (t + dt) - t0);
+      ++derivs;
+      double m = 2.2250738585072014e-308;  // DBL_MIN, gsl_odeiv2_control_hadjust's start
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (!B::valid(r)) continue;
+        const double2 g = to_d(kg[r]), y = xr(r);
+        const double Dx = epsAbs + epsRel * (fabs(y.x) + fabs(dt * g.x));
+        const double Dy = epsAbs + epsRel * (fabs(y.y) + fabs(dt * g.y));
+        m = fmax(m, fmax(fabs(xe[r].x) / Dx, fabs(xe[r].y) / Dy));
+      }
+      return gmax(m);
+    }
     double e = 0.;
     {
       const double f1 = CK::e1 * dt, f3 = CK::e3 * dt, f4 = CK::e4 * dt, f5 = CK::e5 * dt, f6 = CK::e6 * dt;
@@ -557,7 +586,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
 };
 
 // advance_traj for the LDS-state engine: the same MCWF / ODE control flow, with psi in X.
-template <class Eng>
+template <class Eng, bool GSL = false>
 __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& E, const Eng& eng, int traj,
                                                  double* rts, long long* cnt) {
   constexpr int ROWS = Eng::kRows, G = Eng::G, NW = Eng::kNW;
@@ -590,7 +619,20 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
     if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
     eng.derivativeX(k1, t - t0);
     long long derivs = 1, tries = 0, failed = 0;
-    for (;;) {
+    if constexpr (GSL) {  // ControlledErrorStepperGSL::try_step (ODE_GSL.h:44-84), as in advance_traj
+      ++tries;
+      const double rmax = eng.template tryStepX<true>(k1, dtTry, epsAbs, epsRel, derivs, t, t0);
+      const double hOld = dtTry;
+      t += dtTry;
+      if (rmax > 1.1) {
+        const double r = 0.9 * inv_root<5>(rmax);
+        dtTry = (r < 0.2 ? 0.2 : r) * hOld;
+      } else if (rmax < 0.5) {
+        const double r = rmax < 3.4012224e-5 ? 5.0 : 0.9 * inv_root<6>(rmax);
+        dtTry = (r > 5.0 ? 5.0 : r < 1.0 ? 1.0 : r) * hOld;
+      }
+    }
+    for (; !GSL;) {
       ++tries;
       const double err = eng.tryStepX(k1, dtTry, epsAbs, epsRel, derivs, t, t0);
       // err is the squared norm m: err > 1 <=> m > 1, err < 0.5 <=> m < 0.25,
@@ -1018,7 +1060,6 @@ constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2)
 
 template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD, bool GSL = false>
 __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
-  static_assert(!GSL || NW == 1, "the GSL controller runs in the one-wavefront engine");
   using Eng = Engine<ROWS, NW, WH, F32, false, TD, kOffsets>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
@@ -1050,7 +1091,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     __shared__ T xs2[kTwo ? ROWS * G : 1];
     LdsEngine<ROWS, NW, WH, F32, false, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
                                                                                  kTwo ? xs2 : nullptr);
-    for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds(S, E, eng, traj, rts, cnt);
+    for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds<decltype(eng), GSL>(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
     // state copy share storage
@@ -1076,12 +1117,21 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
 }
 
 // One-wavefront trajectory runner of the mixed kernel, kept out of line: it gets its own
-// register allocation, so the spills of the large-state paths do not reach it.
-template <class Eng>
+// register allocation, so the spills of the large-state paths do not reach it.  ph: the
+// wavefront's phasor table (interaction-picture members), else null.
+template <class Eng, bool GSL>
 __device__ __noinline__ void run_wave_traj(const SysDev* S, const EnsDev* E, double2* ps, double2* pc, double* rts,
-                                           long long* cnt, int lane, int traj) {
-  Eng eng(*S, reinterpret_cast<typename Eng::T*>(ps), ps, nullptr, lane);
-  advance_traj<Eng, true>(*S, *E, eng, traj, pc, rts, cnt);
+                                           long long* cnt, int lane, int traj, typename Eng::T* ph) {
+  Eng eng(*S, reinterpret_cast<typename Eng::T*>(ps), ps, nullptr, lane, ph);
+  advance_traj<Eng, true, GSL>(*S, *E, eng, traj, pc, rts, cnt);
+}
+
+// the members' controller picks the advance: GSL's (ODE_GSL.h) when the variant carries it
+template <bool GSLC, class Eng>
+__device__ __forceinline__ void advance_member_lds(const SysDev& S, const EnsDev& E, const Eng& eng, int traj,
+                                                   double* rts, long long* cnt) {
+  if (GSLC && E.gsl) advance_traj_lds<Eng, true>(S, E, eng, traj, rts, cnt);
+  else advance_traj_lds<Eng, false>(S, E, eng, traj, rts, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1095,22 +1145,27 @@ __device__ __noinline__ void run_wave_traj(const SysDev* S, const EnsDev* E, dou
 // queue on its own, then each wavefront the kind-1 queue.  A quarter's LDS region holds its
 // four wavefronts' kind-1 regions, so a quarter that moves on never touches LDS another
 // quarter still uses.
+// TDC: members may run in the interaction picture (engines with phasor tables; a member without
+// frequencies skips them at run time); GSLC: members may use GSL's controller.  The host picks
+// the plain variant unless some member needs one of them.
 // ---------------------------------------------------------------------------
 
-template <int NWB, int RB, int WHB, int RW, int WHW, bool F32>
+template <int NWB, int RB, int WHB, int RW, int WHW, bool F32, bool TDC = false, bool GSLC = false>
 __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
   constexpr int NWQ = NWB / 4;
   // (a second stage buffer for the work-group team measured 5% slower here: 404 vs 384 ms on
   // the 8192-trajectory four-system batch, same GPU and layout)
-  using EngB = LdsEngine<RB, NWB, WHB, F32, false, false, kOffsets>;
-  using EngQ = LdsEngine<RB, NWQ, WHB, F32, true>;
-  using EngW = Engine<RW, 1, WHW, F32>;
+  using EngB = LdsEngine<RB, NWB, WHB, F32, false, TDC, kOffsets>;
+  using EngQ = LdsEngine<RB, NWQ, WHB, F32, true, TDC>;
+  using EngW = Engine<RW, 1, WHW, F32, false, TDC>;
   using TB = typename EngB::T;
   using TW = typename EngW::T;
   constexpr int GB = 64 * NWB, GQ = 64 * NWQ;
+  // a team's phasor table (16-byte aligned) at the end of its LDS region
+  constexpr size_t kPh = TDC ? kMaxFreqs * sizeof(TB) + 16 : 0;
   constexpr size_t kBlockBytes = (size_t)RB * GB * (sizeof(double2) + sizeof(TB));
-  constexpr size_t kWaveEach = (size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8;
-  constexpr size_t kQuarterOwn = (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ + kMaxJumps + kCounters) * 8;
+  constexpr size_t kWaveEach = ((size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8 + kPh + 15) / 16 * 16;
+  constexpr size_t kQuarterOwn = (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ + kMaxJumps + kCounters) * 8 + kPh;
   constexpr size_t kQuarterRaw = kQuarterOwn > kWaveEach * NWQ ? kQuarterOwn : kWaveEach * NWQ;
   constexpr size_t kQuarter = (kQuarterRaw + 15) / 16 * 16;
   constexpr size_t kBytes = kBlockBytes > 4 * kQuarter ? kBlockBytes : 4 * kQuarter;
@@ -1124,6 +1179,10 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   // passed its quarter's last barrier reuses the pool while the others may still read them
   __shared__ int qslot[4];
   __shared__ unsigned qbar[4];
+  __shared__ TB phB[TDC ? kMaxFreqs : 1];
+  auto align16 = [](void* p) {
+    return reinterpret_cast<TB*>((reinterpret_cast<size_t>(p) + 15) / 16 * 16);
+  };
 
   {  // copy the member descriptors to LDS (16-byte words)
     const int4* src = reinterpret_cast<const int4*>(Mg);
@@ -1150,8 +1209,8 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       const int item = shItem;
       if (item >= M.items[0]) break;
       const int m = member(0, item);
-      EngB eng(M.S[m], xs, X, red, (int)threadIdx.x);
-      advance_traj_lds(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
+      EngB eng(M.S[m], xs, X, red, (int)threadIdx.x, TDC ? phB : nullptr);
+      advance_member_lds<GSLC>(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
     }
     __syncthreads();
   }
@@ -1167,6 +1226,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
     double* qred = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(xs) + RB * GQ * sizeof(TB));
     double* rts = qred + NWQ;
     long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
+    TB* qph = TDC ? align16(cnt + kCounters) : nullptr;
     int* slot = qslot + q;
     unsigned* bar = qbar + q;
     if (qlane == 0) *bar = 0u;
@@ -1181,9 +1241,9 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       const int item = *slot;
       if (item >= M.items[2]) break;
       const int m = member(2, item);
-      EngQ e2(M.S[m], xs, X, qred, qlane);
+      EngQ e2(M.S[m], xs, X, qred, qlane, qph);
       e2.bar = bar; e2.barErr = M.err; e2.barTarget = eng.barTarget; e2.barBroken = eng.barBroken;
-      advance_traj_lds(M.S[m], M.E[m], e2, item - M.first[m], rts, cnt);
+      advance_member_lds<GSLC>(M.S[m], M.E[m], e2, item - M.first[m], rts, cnt);
       eng.barTarget = e2.barTarget; eng.barBroken = e2.barBroken;
     }
     eng.sync();  // the quarter's LDS is free only when all of its wavefronts are done with it
@@ -1197,13 +1257,15 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
     double2* pc = ps + RW * 64;
     double* rts = reinterpret_cast<double*>(pc + RW * 64);
     long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
+    TW* wph = TDC ? reinterpret_cast<TW*>(align16(cnt + kCounters)) : nullptr;
     for (;;) {
       int item = 0;
       if (lane == 0) item = atomicAdd(M.queue + 1, 1);
       item = __builtin_amdgcn_readfirstlane(item);
       if (item >= M.items[1]) break;
       const int m = member(1, item);
-      run_wave_traj<EngW>(&M.S[m], &M.E[m], ps, pc, rts, cnt, lane, item - M.first[m]);
+      if (GSLC && M.E[m].gsl) run_wave_traj<EngW, true>(&M.S[m], &M.E[m], ps, pc, rts, cnt, lane, item - M.first[m], wph);
+      else run_wave_traj<EngW, false>(&M.S[m], &M.E[m], ps, pc, rts, cnt, lane, item - M.first[m], wph);
       wave_sync();
     }
   }
@@ -1237,10 +1299,11 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1) \
   X(1, 4, 16, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
 
-// GSL-controller variants (ODE_GSL.h), Schroedinger and interaction picture: the one-wavefront
-// layouts, the ELL width rounded up to a compiled one
+// GSL-controller variants (ODE_GSL.h), Schroedinger and interaction picture: one per dimension
+// class, the ELL width rounded up to a compiled one
 #define QT_BATCH_GSL(X) \
-  X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1)
+  X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1) \
+  X(1, 4, 16, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
 
 // One launch table over a list of batched variants: NAME##_blocks (resident work groups per
 // compute unit, registers and LDS permitting) and NAME##_launch.  Each list is instantiated in

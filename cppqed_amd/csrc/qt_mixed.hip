@@ -1,12 +1,11 @@
 // qt_mixed.hip -- launch table of the mixed-batch kernel (several ensembles in one launch).
-#include "qt_engine.hpp"
+// The interaction-picture and GSL-capable instances are compiled in their own translation
+// units (qt_mixed_td.hip, qt_mixed_gsl.hip, qt_mixed_td_gsl.hip) by the same table macro.
+#include "qt_mixed.hpp"
 
 namespace qt {
 
-// Mixed-batch variants: (waves per work group, rows per lane of the work-group team, its ELL
-// register width, rows per lane of the wavefront team, its ELL register width).
-#define QT_MIXED(X) \
-  X(16, 4, 16, 1, 8) X(16, 4, 16, 2, 8) X(16, 2, 16, 1, 8) X(16, 1, 16, 1, 8)
+QT_MIXED_TABLE(plain, false, false)
 
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw) {
 #define QT_CASE(A, B, C, D, F) if (nwb == A && rb == B && whb == C && rw == D && whw == F) return true;
@@ -25,29 +24,19 @@ int mixed_variants(int* table, int cap) {
   return n;
 }
 
-int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32) {
-  int n = 0;
-#define QT_CASE(A, B, C, D, F)                                                                                      \
-  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                     \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, true>, 64 * A, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, false>, 64 * A, 0);   \
-    return n;                                                                                                       \
-  }
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return 0;
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl) {
+  if (td && gsl) return mixed_blocks_td_gsl(nwb, rb, whb, rw, whw, f32);
+  if (td) return mixed_blocks_td(nwb, rb, whb, rw, whw, f32);
+  if (gsl) return mixed_blocks_gsl(nwb, rb, whb, rw, whw, f32);
+  return mixed_blocks_plain(nwb, rb, whb, rw, whw, f32);
 }
 
-hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, hipStream_t st) {
-#define QT_CASE(A, B, C, D, F)                                                                                          \
-  if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                         \
-    if (f32) hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, true>), dim3(grid), dim3(64 * A), 0, st, M);          \
-    else hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, false>), dim3(grid), dim3(64 * A), 0, st, M);             \
-    return hipGetLastError();                                                                                           \
-  }
-  QT_MIXED(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
+                        const MixDev* M, hipStream_t st) {
+  if (td && gsl) return launch_mixed_td_gsl(nwb, rb, whb, rw, whw, f32, grid, M, st);
+  if (td) return launch_mixed_td(nwb, rb, whb, rw, whw, f32, grid, M, st);
+  if (gsl) return launch_mixed_gsl(nwb, rb, whb, rw, whw, f32, grid, M, st);
+  return launch_mixed_plain(nwb, rb, whb, rw, whw, f32, grid, M, st);
 }
 
 }  // namespace qt

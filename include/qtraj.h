@@ -84,8 +84,9 @@ typedef struct {
   double dtInit;             /* <= 0: 1/(10*highest_frequency), Trajectory.h:126 */
   int precision;             /* QT_FP64 | QT_FP32_DERIV */
   int controller;            /* QT_CONTROLLER_ODEINT (ODE.h:189-223) or QT_CONTROLLER_GSL (ODE_GSL.h:44-84:
-                                every step accepted, gsl_odeiv2_control_hadjust; ensembles of the
-                                one-wavefront engine, dimensions up to 256) */
+                                every step accepted, gsl_odeiv2_control_hadjust; ensembles, mixed
+                                batches and the request loop; not the sub-wave layouts or the plain
+                                form) */
   unsigned long long seed;   /* counter-RNG key word 0 */
   unsigned long long traj0;  /* RNG key word 1 of trajectory 0 (trajectory k uses traj0+k) */
   int jump_log_cap;          /* jump records kept per trajectory */
@@ -193,7 +194,8 @@ int qt_mcwf_set_states(qt_ens* ens, int first, int count, const qt_traj_state* s
  * serially (trajectory::Ensemble::advance, StochasticTrajectory.h:220) and separate ensembles
  * one after the other.  Each member runs one trajectory per work group (team 0), per quarter
  * of a work group (team 2) or per wavefront (team 1); the layout variant sets the work-group
- * width and the rows per lane of the teams.  All members share one precision. */
+ * width and the rows per lane of the teams.  All members share one precision; each keeps its
+ * own controller (odeint's or GSL's) and picture (Schroedinger or interaction picture). */
 typedef struct qt_batch_s qt_batch;
 
 typedef struct {
@@ -248,9 +250,10 @@ typedef struct {
 
 int qt_server_create(const qt_server_params* p, qt_server** out);
 int qt_server_destroy(qt_server* s);
-/* Queues an ensemble of n_traj trajectories of sys from psi0_host, run to T with stream
- * interval Dt (trajectory::run Dt mode) and RNG key (seed, trajectory index).  sys must outlive
- * the request.  Returns the request id. */
+/* Queues an ensemble of n_traj trajectories of sys (either picture) from psi0_host, run to T
+ * with stream interval Dt (trajectory::run Dt mode) and RNG key (seed, trajectory index), under
+ * the server's ODE / MCWF parameters (controller included).  sys must outlive the request.
+ * Returns the request id. */
 int qt_server_submit(qt_server* s, qt_sys* sys, int n_traj, const double* psi0_host, double T, double Dt,
                      unsigned long long seed, long long* id);
 /* One iteration of the loop (admit, one launch, retire); reports active and waiting counts.

@@ -82,3 +82,41 @@ def test_mixed_autotune_keeps_results():
     rows = b.advance(iv)
     for a, r in zip(rows, ref):
         np.testing.assert_allclose(a, r, rtol=1e-9, atol=1e-12)
+
+
+def test_mixed_gsl_and_interaction_picture_members_match_oracle():
+    """Members that differ in controller and picture share one launch: GSL's controller
+    (ODE_GSL.h:44-84) on work-group, quarter and wavefront teams, interaction-picture systems
+    (Exact::actWithU after each step, MCWF_Trajectory.h:269-272) beside Schroedinger ones."""
+    from sysdefs import JC60, MODES4096, P1M1, PCAV1024, RING, ground, p1m1_initial, ring_initial
+    cases = [("pcav1024", PCAV1024, basis(1024, 32), 3, "gsl"),
+             ("modes4096", MODES4096, basis(4096, 0), 2, "gsl"),
+             ("jc60", JC60, ground(60), 8, "odeint"),
+             ("p1m1_ip", P1M1, p1m1_initial(), 8, "odeint"),
+             ("ring_ip", RING, ring_initial(), 6, "gsl")]
+    iv = np.full(5, 0.1)
+    mem = [Q.Ensemble(Q.System(spec), n, psi, Q.mcwf_pars(seed=SEED + i, jump_log_cap=256, controller=ctl))
+           for i, (_, spec, psi, n, ctl) in enumerate(cases)]
+    b = Q.Batch(mem)
+    v, team = b.layout()
+    print(f"[gpu] mixed batch gsl + interaction picture: variant {v} teams {team}", flush=True)
+    assert {0, 2} <= set(team) and 1 in team
+    rows = b.advance(iv)
+    for i, ((name, spec, psi0, n, ctl), e) in enumerate(zip(cases, mem)):
+        so = O.System(spec)
+        per, _ = e.counters()
+        for k in range(n):
+            p = O.params(rng="philox", seed=SEED + i, stream=k, controller=ctl)
+            res = O.mcwf_run(so, psi0, p, O.run_spec("NDt", Dt=0.1, NDt=len(iv)))
+            psi_g, _ = e.state(k)
+            jt, jch, nj = e.jumplog(k)
+            assert nj == len(res["jump_t"]), (name, k)
+            np.testing.assert_array_equal(jch, res["jump_ch"])
+            np.testing.assert_allclose(jt, res["jump_t"], rtol=1e-10, atol=1e-12)
+            err = np.linalg.norm(psi_g - res["psi"]) / np.linalg.norm(res["psi"])
+            assert err <= 1e-9, (name, k, err)
+            for c in (0, 1, 2, 4, 5, 7) + ((3,) if ctl == "gsl" else ()):
+                assert per[k][c] == res["counters"][c], (name, k, c)
+        ro, _ = O.ensemble_run(so, psi0, O.params(rng="philox", seed=SEED + i, stream=0, controller=ctl),
+                               O.run_spec("NDt", Dt=0.1, NDt=len(iv)), n)
+        np.testing.assert_allclose(rows[i], ro[1:], rtol=1e-9, atol=1e-9, err_msg=name)

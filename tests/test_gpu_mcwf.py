@@ -4,7 +4,7 @@ import pytest
 
 import cppqed_amd as Q
 import oracle_lib as O
-from sysdefs import JC60, P1M1, PLM20, PTLA, coherent, ground, p1m1_initial
+from sysdefs import JC60, MODES4096, P1M1, PCAV1024, PLM20, PTLA, basis, coherent, ground, p1m1_initial
 
 pytestmark = pytest.mark.gpu
 
@@ -183,14 +183,39 @@ def test_gsl_controller_trajectories_match_oracle(name, spec, psi0, T, ntraj):
         assert per[k][3] == res_o["counters"][3]
 
 
+@pytest.mark.parametrize("name,spec,psi0,T,ntraj,layout", [
+    # multi-wavefront teams with the state in LDS (LdsEngine): a small system forced onto four
+    # wavefronts, in both pictures, and the two large members of the mixed batch
+    ("jc60_4w", JC60, ground(60), 1.5, 12, (1, 4)),
+    ("p1m1_ip_4w", P1M1, p1m1_initial(), 0.3, 8, (1, 4)),
+    ("pcav1024", PCAV1024, basis(1024, 32), 0.4, 4, (0, 0)),
+    ("modes4096", MODES4096, basis(4096, 0), 0.3, 3, (0, 0)),
+])
+def test_gsl_controller_lds_teams_match_oracle(name, spec, psi0, T, ntraj, layout):
+    """ODE_EngineGSL (ODE_GSL.h:44-84) in the multi-wavefront engine: the GSL error ratio needs the
+    derivative of the new state, formed in LDS by the whole team."""
+    print(f"[gpu] gsl controller lds team {name} ntraj={ntraj}", flush=True)
+    sq, so = Q.System(spec), O.System(spec)
+    seed = 815
+    iv = np.full(int(round(T / 0.1)), 0.1)
+    ens = Q.Ensemble(sq, ntraj, psi0, Q.mcwf_pars(seed=seed, jump_log_cap=256, controller="gsl", layout=layout))
+    ens.advance(iv)
+    per, tot = ens.counters()
+    assert tot[2] == 0 and tot[3] == 7 * tot[4]
+    for k in range(ntraj):
+        res_o = _oracle_traj(so, psi0, k, seed, iv, controller="gsl")
+        psi_g, _ = ens.state(k)
+        jt, jch, nj = ens.jumplog(k)
+        _assert_traj_parity(res_o, psi_g, jt, jch, nj, per[k])
+        assert per[k][3] == res_o["counters"][3]
+
+
 def test_gsl_controller_is_refused_where_not_built():
     sq = Q.System(JC60)
-    with pytest.raises(Q.QtError):
+    with pytest.raises(Q.QtError, match="odeint controller only"):
         Q.mcwf_plain_run(sq, ground(60), np.full(2, 0.1), Q.mcwf_pars(controller="gsl"))
-    big = Q.System("\n".join([f"free mode cutoff=8 delta=-1 kappa=1 eta={1 if i == 0 else 0}" for i in range(3)]
-                             + ["ia bs 0 1 g=0.5", "ia bs 1 2 g=0.5", "system composite"]))
-    with pytest.raises(Q.QtError):
-        Q.Ensemble(big, 4, ground(big.dim), Q.mcwf_pars(controller="gsl"))
+    with pytest.raises(Q.QtError, match="odeint controller only"):
+        Q.Ensemble(sq, 4, ground(60), Q.mcwf_pars(controller="gsl", group_lanes=16))
 
 
 @pytest.mark.parametrize("prec", [Q.QT_FP64, Q.QT_FP32_DERIV])

@@ -190,3 +190,20 @@ def test_failed_requests_are_retired_alone():
     np.testing.assert_array_equal(_drain(srv, [rid2])[rid2][0], alone)
     for r in ids + [rid, rid2]:
         srv.release(r)
+
+
+@pytest.mark.parametrize("ctl", ["odeint", "gsl"])
+def test_served_interaction_picture_and_gsl_match_oracle(ctl):
+    """Requests of interaction-picture systems (Exact::actWithU after each step, MCWF_Trajectory.h:
+    269-272) and a server running GSL's controller (ODE_GSL.h:44-84) match the oracle's ensembles."""
+    from sysdefs import P1M1, RING, p1m1_initial, ring_initial
+    print(f"[gpu] serve interaction picture, controller {ctl}", flush=True)
+    srv = Q.Server(chunk_intervals=3, pars=Q.mcwf_pars(precision=Q.QT_FP64, controller=ctl))
+    cases = [(P1M1, p1m1_initial(), 16, 0.5, 31), (PLM20.replace("kappa", "picture=IP kappa"), coherent(0.5, 20), 24, 0.6, 32),
+             (RING, ring_initial(), 8, 0.4, 33), (JC60, ground(60), 12, 0.5, 34)]
+    ids = [srv.submit(Q.System(spec), n, psi, T=T, seed=seed) for spec, psi, n, T, seed in cases]
+    res = _drain(srv, ids)
+    for (spec, psi, n, T, seed), r in zip(cases, ids):
+        ro, _ = O.ensemble_run(O.System(spec), psi, O.params(rng="philox", seed=seed, stream=0, controller=ctl),
+                               O.run_spec("Dt", Dt=0.1, T=T), n)
+        np.testing.assert_allclose(res[r][0], ro[1:], rtol=1e-9, atol=1e-9)
