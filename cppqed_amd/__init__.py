@@ -47,7 +47,11 @@ class McwfParams(C.Structure):
 
 class MasterParams(C.Structure):
     _fields_ = [("epsRel", C.c_double), ("epsAbs", C.c_double), ("dtInit", C.c_double), ("ss_tol", C.c_double),
-                ("autostop_eps", C.c_double), ("autostop_eps_abs", C.c_double), ("autostop_rep", C.c_int)]
+                ("autostop_eps", C.c_double), ("autostop_eps_abs", C.c_double), ("autostop_rep", C.c_int),
+                ("kernel", C.c_int)]
+
+
+MASTER_KERNELS = {"auto": 0, "mfma": 1, "sparse": 2}   # QT_MASTER_AUTO / _MFMA / _SPARSE
 
 
 class BatchParams(C.Structure):
@@ -599,7 +603,7 @@ class Server:
         return rid.value
 
     def submit_sweep(self, systems, rho0, T, Dt=0.1, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0,
-                     autostop=None):
+                     autostop=None, kernel="auto"):
         """A Master-equation parameter sweep (one point per system) as a request; its result rows
         are point-major, shape (points * intervals, 2 + n_avr).  autostop as for MasterSweep."""
         d = systems[0].dim
@@ -608,7 +612,7 @@ class Server:
         rho = np.ascontiguousarray(np.asarray(rho0, dtype=np.complex128))
         arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
         a_eps, a_abs, a_rep = autostop if autostop else (0.0, 0.0, 0)
-        mp = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep)
+        mp = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep, MASTER_KERNELS[kernel])
         rid = C.c_longlong()
         check(lib().qt_server_submit_sweep(self.h, arr, len(systems), rho.ctypes.data, float(T), float(Dt),
                                            C.byref(mp), C.byref(rid)))
@@ -674,9 +678,11 @@ def mcwf_plain_run(system, psi0, intervals, pars=None, traj=0, jcap=4096):
 class MasterSweep:
     """A batch of density operators on device, point k under systems[k] (master::make + Master)."""
 
-    def __init__(self, systems, rho0, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0, autostop=None):
+    def __init__(self, systems, rho0, epsRel=1e-6, epsAbs=1e-12, dtInit=0.0, ss_tol=0.0, autostop=None,
+                 kernel="auto"):
         """autostop = (epsilon, epsilonAbs, repetition): the reference's AutostopHandlerGeneric
-        criterion per point (CPPQEDutils/Trajectory.h:179-222); None: off."""
+        criterion per point (CPPQEDutils/Trajectory.h:179-222); None: off.  kernel: "auto" (the
+        MFMA kernel up to d = 96, the sparse one beyond), "mfma" or "sparse"."""
         self.systems = list(systems)
         self.n = len(self.systems)
         self.dim = self.systems[0].dim
@@ -686,7 +692,7 @@ class MasterSweep:
             raise ValueError("rho0 has the wrong shape")
         arr = (C.c_void_p * self.n)(*[s_.h.value if hasattr(s_.h, "value") else s_.h for s_ in self.systems])
         a_eps, a_abs, a_rep = autostop if autostop else (0.0, 0.0, 0)
-        self.pars = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep)
+        self.pars = MasterParams(epsRel, epsAbs, dtInit, ss_tol, a_eps, a_abs, a_rep, MASTER_KERNELS[kernel])
         h = C.c_void_p()
         check(lib().qt_master_create(arr, self.n, C.byref(self.pars), rho0.ctypes.data, C.byref(h)))
         self.h = h

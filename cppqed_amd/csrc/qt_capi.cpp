@@ -49,6 +49,8 @@ hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const 
 struct MasterDev;
 bool master_supported(int nb);
 hipError_t launch_master(int nb, const MasterDev& M, hipStream_t st);
+bool master_sparse_supported(int d);
+hipError_t launch_master_sparse(const MasterDev& M, hipStream_t st);
 }  // namespace qt
 
 #include "qt_master_dev.hpp"
@@ -420,8 +422,9 @@ struct qt_master {
   DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn, tout;
   DevBuf<double2> exz;             // [point][d] (interaction picture only)
   bool td = false;
-  int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product
+  int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product (all of it: sparse)
   bool sharedH = false;            // one copy of the fragments for every point
+  bool sparse = false;             // the sparse kernel (every matrix in HBM, H_nH/i as ELL rows)
   DevBuf<int> rcol, rfi;
   DevBuf<double2> rval, rfreq;
   DevBuf<int> jcol, done;
@@ -477,7 +480,10 @@ void check_master(qt_sys* const* systems, int n_points, const qt_master_params* 
   const qt::System& s0 = *systems[0]->h;
   if (p->autostop_rep > 0 && (int)s0.O.size() > qt::kMaxAutostopAvr)
     throw Fail(QT_ENOTSUP, "autostop watches at most 64 averages");
-  if (!qt::master_supported((int)(s0.dim + 15) / 16)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
+  if (p->kernel < QT_MASTER_AUTO || p->kernel > QT_MASTER_SPARSE) throw Fail(QT_EINVAL, "unknown Master kernel");
+  if (p->kernel == QT_MASTER_MFMA && !qt::master_supported((int)(s0.dim + 15) / 16))
+    throw Fail(QT_ENOTSUP, "the MFMA Master kernel supports dimensions up to 96");
+  if (!qt::master_sparse_supported((int)s0.dim)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 4096");
   for (int k = 0; k < n_points; ++k) {
     const qt::System& s = *systems[k]->h;
     if (s.dim != s0.dim || s.J.size() != s0.J.size())
@@ -1399,7 +1405,7 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     const qt::System& s0 = *systems[0]->h;
     const int d = (int)s0.dim;
     const int NB = (d + 15) / 16, DP = 16 * NB, KC = DP / 4, NN = DP * DP;
-    if (!qt::master_supported(NB)) throw Fail(QT_ENOTSUP, "Master sweep supports dimensions up to 96");
+    const bool sparse = p->kernel == QT_MASTER_SPARSE || (p->kernel == QT_MASTER_AUTO && !qt::master_supported(NB));
     const int nj = (int)s0.J.size();
     int wj = 1;
     for (int k = 0; k < n_points; ++k)
@@ -1407,6 +1413,7 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     std::unique_ptr<qt_master> m(new qt_master);
     m->n = n_points; m->d = d; m->nb = NB; m->p = *p;
     m->sys0 = systems[0];
+    m->sparse = sparse;
     // H_nH/i A fragments: [point][NB][KC][64] {re, im}, H[16 r + l%16][4 kk + l/16]
     // per point: the part of H_nH/i that rotates with the Exact exponents (all of it in the
     // Schroedinger picture) as a dense matrix -- the MFMA product's A operand -- and the remainder,
@@ -1419,12 +1426,13 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     std::vector<double> dtTry(n_points);
     for (int k = 0; k < n_points; ++k) {
       const qt::System& s = *systems[k]->h;
-      Hrot[k].assign((size_t)DP * DP, 0.);
+      if (!sparse) Hrot[k].assign((size_t)DP * DP, 0.);
       for (int r = 0; r < d; ++r)
         for (int w = 0; w < s.H.width; ++w) {
           const size_t i = (size_t)r * s.H.width + w;
           if (s.H.vals[i] == qt::cd(0.)) continue;
-          if (master_entry_rotates(s, r, i)) Hrot[k][(size_t)r * DP + s.H.cols[i]] += s.H.vals[i];
+          // the sparse kernel takes every entry with its own frequency
+          if (!sparse && master_entry_rotates(s, r, i)) Hrot[k][(size_t)r * DP + s.H.cols[i]] += s.H.vals[i];
           else rest[k].push_back({r, s.H.cols[i], s.H.vals[i], s.H.freqs.empty() ? qt::cd(0.) : s.H.freqs[i]});
         }
       for (int mI = 0; mI < nj; ++mI) {
@@ -1450,8 +1458,8 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     // slower on the 512-point sweep, 591 k vs 650 k tries/s: the fragments are not what bounds it)
     const bool shared = false;
     m->sharedH = shared;
-    const int nfrag = shared ? 1 : n_points;
-    std::vector<double> hf((size_t)nfrag * NB * KC * 64 * 2, 0.);
+    const int nfrag = sparse ? 0 : shared ? 1 : n_points;
+    std::vector<double> hf(std::max<size_t>(2, (size_t)nfrag * NB * KC * 64 * 2), 0.);
     for (int k = 0; k < nfrag; ++k)
       for (int rb = 0; rb < NB; ++rb)
         for (int kk = 0; kk < KC; ++kk)
@@ -1514,7 +1522,7 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
         }
     for (int k = 0; k < n_points; ++k) std::copy(x0.begin(), x0.end(), xs.begin() + (size_t)k * NN);
     m->x.upload(xs);
-    m->k.alloc((size_t)n_points * 7 * NN);
+    m->k.alloc((size_t)n_points * (sparse ? 9 : 7) * NN);
     m->t.upload(std::vector<double>(n_points, 0.));
     m->dtDid.upload(std::vector<double>(n_points, 0.));
     m->dtTry.upload(dtTry);
@@ -1572,7 +1580,7 @@ static void master_launch(qt_master* m, const double* intervals, int n_int, hipS
     M.dc = dc; M.t_end = t_end;
     if (dc > 0) { m->tout.alloc((size_t)m->n * n_int); M.time_out = m->tout.p; }
     hipCheck(hipEventRecord(m->e0, st), "hipEventRecord");
-    hipCheck(qt::launch_master(m->nb, M, st), "master launch");
+    hipCheck(m->sparse ? qt::launch_master_sparse(M, st) : qt::launch_master(m->nb, M, st), "master launch");
     hipCheck(hipEventRecord(m->e1, st), "hipEventRecord");
     const size_t nob = (size_t)m->n * n_int * std::max(1, no), nd = (size_t)m->n * n_int;
     if (m->hcap < nob + 2 * nd) {

@@ -23,6 +23,7 @@
 #include "qt_device.hpp"
 #include "qt_kernels.hpp"
 #include "qt_master_dev.hpp"
+#include "qt_master_common.hpp"
 
 #include <algorithm>
 
@@ -32,6 +33,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMasterWaves = 8;
 
+// one entry of the remainder as kept in LDS (the point's copy, master_kernel)
+struct RemEntry { double2 v; int c, f; };
+
 // K = M + M^dagger + sum_m J_m S J_m^dagger for the stage matrix S in LDS (Sr, Si planes); kept out
 // of line so that its three call sites (a step's first derivative, the stage loop, the stream
 // point) share one register allocation (inlined, the 512-point d = 80 sweep ran 632 k instead of
@@ -39,24 +43,6 @@ constexpr int kMasterWaves = 8;
 // TD: M = E (H_nH/i) E^-1 S with ph[k] = exp(-exz_k tau) (E^-1, applied to the B operand as it
 // is read) and phi[i] = exp(exz_i tau) (E, applied to the product's rows); the tables are made by
 // the caller.
-// Cash-Karp tableau by stage index (stage s = 0..5 computes K_{s+1}): a_{s+1, i+1} and c_{s+1}
-__device__ __forceinline__ double ck_a(int s, int i) {
-  constexpr double A[6][5] = {{0., 0., 0., 0., 0.},
-                              {CK::a21, 0., 0., 0., 0.},
-                              {CK::a31, CK::a32, 0., 0., 0.},
-                              {CK::a41, CK::a42, CK::a43, 0., 0.},
-                              {CK::a51, CK::a52, CK::a53, CK::a54, 0.},
-                              {CK::a61, CK::a62, CK::a63, CK::a64, CK::a65}};
-  return A[s][i];
-}
-__device__ __forceinline__ double ck_c(int s) {
-  constexpr double C[6] = {0., CK::c2, CK::c3, CK::c4, CK::c5, CK::c6};
-  return C[s];
-}
-
-// one entry of the remainder as kept in LDS (the point's copy, master_kernel)
-struct RemEntry { double2 v; int c, f; };
-
 template <int NB, bool TD>
 __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout,
                                           const double2* ph, const double2* phi, const double2* rph,
@@ -223,52 +209,6 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
       Kout[(4 * t + v) * 64 + lane] = make_double2(cr[tw][v] + SrT[o], ci[tw][v] - SiT[o]);
     }
   }
-}
-
-
-// AutostopHandlerGeneric::operator() (CPPQEDutils/Trajectory.h:187-214) for one point, on the
-// processed stream row (Mode.cc:198-201 processing of the raw averages `raw`).  The first call
-// only starts the running average; the queue then fills to as_rep rows; from there on a row
-// within as_eps (relative, max over the averages) of the running average stops the point,
-// otherwise it replaces the oldest row.  Returns 1 to stop.  One thread per point.
-__device__ int autostop(const MasterDev& M, int p, const double* raw) {
-  const int no = M.no, rep = M.as_rep;
-  double buf[kMaxAutostopAvr];
-  for (int q = 0; q < no; ++q) {
-    double v = raw[q];
-    const int vq = M.proc_var[q];
-    if (vq >= 0) {
-      v -= raw[vq] * raw[vq];
-      if (M.proc_sqrt[q]) v = sqrt(v);
-    }
-    buf[q] = fabs(v) < M.as_epsAbs ? M.as_epsAbs : v;
-  }
-  int* meta = M.as_meta + 3 * (size_t)p;
-  double* avg = M.as_buf + (size_t)p * (rep + 1) * no;
-  double* ring = avg + no;
-  if (!meta[0]) {
-    meta[0] = 1;
-    for (int q = 0; q < no; ++q) avg[q] = 0.;
-    return 0;
-  }
-  int qs = meta[1], qh = meta[2];
-  if (qs == rep) {
-    double m = -INFINITY;
-    for (int q = 0; q < no; ++q) {
-      const double r = fabs(avg[q] - buf[q]) / (fabs(avg[q]) + fabs(buf[q]));
-      m = (m < r) ? r : m;
-    }
-    if (m < M.as_eps) return 1;
-    double* front = ring + (size_t)qh * no;
-    for (int q = 0; q < no; ++q) { avg[q] = avg[q] + (buf[q] - front[q]) / (double)rep; front[q] = buf[q]; }
-    qh = (qh + 1) % rep;
-  } else {
-    double* slot = ring + (size_t)((qh + qs) % rep) * no;
-    for (int q = 0; q < no; ++q) { avg[q] = ((double)qs * avg[q] + buf[q]) / (double)(qs + 1); slot[q] = buf[q]; }
-    ++qs;
-  }
-  meta[1] = qs; meta[2] = qh;
-  return 0;
 }
 
 

@@ -284,12 +284,18 @@ typedef struct {
    * autostop_rep = 0: off.  Either criterion stops a point (qt_master_status: done, t_ss). */
   double autostop_eps, autostop_eps_abs;
   int autostop_rep;
+  int kernel;              /* QT_MASTER_AUTO (MFMA up to d = 96, sparse beyond), QT_MASTER_MFMA or
+                              QT_MASTER_SPARSE (any d up to 4096) */
 } qt_master_params;
+
+enum { QT_MASTER_AUTO = 0, QT_MASTER_MFMA = 1, QT_MASTER_SPARSE = 2 };
 
 /* A batch of density operators, point k evolving under systems[k] (same dimension and jump
  * structure), all from rho0_host (d*d complex, row-major): master::make + Master
- * (CPPQEDcore/quantumtrajectory/Master.h:130-136, 50-208) for every sweep point.  The
- * (H_nH/i) rho products run on FP64 MFMA; dimensions up to 96.  Schroedinger picture, or the
+ * (CPPQEDcore/quantumtrajectory/Master.h:130-136, 50-208) for every sweep point.  Up to d = 96
+ * the (H_nH/i) rho products run on FP64 MFMA with the stage matrix in LDS; beyond (up to 4096,
+ * or on request) a sparse kernel applies the superoperator from the ELL rows of H_nH/i and the
+ * jump operators with every matrix in HBM.  Schroedinger picture, or the
  * unitary interaction picture (UIP; the reference turns IP into UIP for the Master,
  * QM_Picture.cc:45-47, and refuses non-unitary ones, Master.h:73): then each step ends with
  * rho <- U rho U^dagger (Master.h:186-193). */

@@ -1,4 +1,5 @@
-"""GPU parity of the FP64-MFMA Master sweep against the CPU oracle and the reference's golden output."""
+"""GPU parity of the Master sweep -- the FP64-MFMA kernel (d <= 96) and the sparse kernel (any d up
+to 4096) -- against the CPU oracle and the reference's golden output."""
 import numpy as np
 import pytest
 
@@ -10,9 +11,12 @@ from test_oracle import load_golden
 pytestmark = pytest.mark.gpu
 
 
-def _golden_master(spec, rho0, mode, Dt, T, NDt, dc):
+KERNELS = ["mfma", "sparse"]
+
+
+def _golden_master(spec, rho0, mode, Dt, T, NDt, dc, kernel="mfma"):
     """rows of the device Master (one point) in the run loop's dc, Dt or NDt mode, from t = 0"""
-    m = Q.MasterSweep([Q.System(spec)], rho0, epsRel=1e-6, epsAbs=1e-5)
+    m = Q.MasterSweep([Q.System(spec)], rho0, epsRel=1e-6, epsAbs=1e-5, kernel=kernel)
     first, _ = m.advance([0.0])
     if mode == "dc":
         rest = m.advance_dc(T, dc, max_rows=5)
@@ -30,47 +34,51 @@ def _compare(rows, g, dev_col=None):
     assert not bad.any(), (np.argwhere(bad)[:5], rows[bad][:5], g[bad][:5])
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("mode", ["dc", "dt", "ndt"])
-def test_master_ptla_golden(mode):
+def test_master_ptla_golden(mode, kernel):
     """Testing/expected/run_PTLA_CPPQED_{dc,dt,ndt}_master replayed by the device Master (d=2, one tile)."""
-    print(f"[gpu] master ptla golden {mode}", flush=True)
-    rows = _golden_master(PTLA, np.diag([1.0, 0.0]).astype(complex), mode, 0.1, 1.0, 10, 10)
+    print(f"[gpu] master ptla golden {mode} {kernel}", flush=True)
+    rows = _golden_master(PTLA, np.diag([1.0, 0.0]).astype(complex), mode, 0.1, 1.0, 10, 10, kernel)
     _compare(rows, load_golden(f"run_PTLA_CPPQED_{mode}_master"))
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("mode", ["dc", "dt", "ndt"])
-def test_master_1particle1mode_golden(mode):
+def test_master_1particle1mode_golden(mode, kernel):
     """run_1particle1mode_{dc,dt,ndt}_master: the unitary interaction picture on the device Master --
     H(tau) = E H E^-1 around the MFMA product and rho <- U rho U^dagger after every step
     (Master.h:186-193) -- for the pumped particle, the particle-cavity term and the lossy mode."""
     from sysdefs import P1M1_MASTER, p1m1_initial
-    print(f"[gpu] master 1particle1mode golden {mode}", flush=True)
+    print(f"[gpu] master 1particle1mode golden {mode} {kernel}", flush=True)
     psi = p1m1_initial()
-    rows = _golden_master(P1M1_MASTER, np.outer(psi, psi.conj()), mode, 0.01, 0.03, 3, 1)
+    rows = _golden_master(P1M1_MASTER, np.outer(psi, psi.conj()), mode, 0.01, 0.03, 3, 1, kernel)
     g = load_golden(f"run_1particle1mode_{mode}_master")
     _compare(rows, g, dev_col=g.shape[1] - 1)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("conf", [2, 3, 4])
-def test_master_1particle1mode_conf_golden(conf):
+def test_master_1particle1mode_conf_golden(conf, kernel):
     """run_1particle1mode_ndt_conf{2,3,4}_master (--dc 1 --T 0.03 with the Master): ParticleAlongCavity."""
     from sysdefs import P1M1_CONF, p1m1_initial
-    print(f"[gpu] master 1particle1mode conf{conf} golden", flush=True)
+    print(f"[gpu] master 1particle1mode conf{conf} golden {kernel}", flush=True)
     psi = p1m1_initial()
     spec = P1M1_CONF[conf].replace("eta=10 picture=IP", "eta=10 picture=UIP")
-    rows = _golden_master(spec, np.outer(psi, psi.conj()), "dc", 0.01, 0.03, 3, 1)
+    rows = _golden_master(spec, np.outer(psi, psi.conj()), "dc", 0.01, 0.03, 3, 1, kernel)
     g = load_golden(f"run_1particle1mode_ndt_conf{conf}_master")
     _compare(rows, g, dev_col=g.shape[1] - 1)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("mode", ["dc", "dt", "ndt"])
-def test_master_ring_golden(mode):
+def test_master_ring_golden(mode, kernel):
     """run_Ring_{dc,dt,ndt}_master: a composite of three Exact frees (d = 72) in the unitary
     interaction picture on the device Master."""
     from sysdefs import RING_MASTER, ring_initial
-    print(f"[gpu] master Ring golden {mode}", flush=True)
+    print(f"[gpu] master Ring golden {mode} {kernel}", flush=True)
     psi = ring_initial()
-    rows = _golden_master(RING_MASTER, np.outer(psi, psi.conj()), mode, 0.0001, 0.0002, 2, 1)
+    rows = _golden_master(RING_MASTER, np.outer(psi, psi.conj()), mode, 0.0001, 0.0002, 2, 1, kernel)
     _compare(rows, load_golden(f"run_Ring_{mode}_master"), dev_col=5)
 
 
@@ -81,18 +89,24 @@ def test_master_picture_refusals():
     rho[0, 0] = 1
     with pytest.raises(Q.QtError, match="SystemNotApplicable"):
         Q.MasterSweep([ip], rho)
+    with pytest.raises(Q.QtError, match="SystemNotApplicable"):
+        Q.MasterSweep([ip], rho, kernel="sparse")
 
 
-@pytest.mark.parametrize("cutoff,etas,T", [(4, [0.6, 2.0], 1.0), (8, [1.0, 3.0, 5.0], 0.5), (40, [0.5, 7.0], 0.2)])
-def test_master_sweep_matches_oracle(cutoff, etas, T):
-    print(f"[gpu] master sweep cutoff={cutoff} etas={etas}", flush=True)
+@pytest.mark.parametrize("cutoff,etas,T,kernel", [
+    (4, [0.6, 2.0], 1.0, "mfma"), (8, [1.0, 3.0, 5.0], 0.5, "mfma"), (40, [0.5, 7.0], 0.2, "mfma"),
+    (8, [1.0, 3.0, 5.0], 0.5, "sparse"), (40, [0.5, 7.0], 0.2, "sparse"),
+    # beyond the MFMA kernel's 96: the sparse kernel is the automatic choice
+    (60, [0.5, 7.0], 0.2, "auto"), (100, [2.0, 9.0, 15.0], 0.1, "auto")])
+def test_master_sweep_matches_oracle(cutoff, etas, T, kernel):
+    print(f"[gpu] master sweep cutoff={cutoff} etas={etas} kernel={kernel}", flush=True)
     specs = [jc_master_spec(cutoff, eta=e, g=10.0).replace("gamma=0", "gamma=0.3") for e in etas]
     sq = [Q.System(sp) for sp in specs]
     d = sq[0].dim
     rho0 = np.zeros((d, d), complex)
     rho0[0, 0] = 1
     iv = np.full(int(round(T / 0.05)), 0.05)
-    m = Q.MasterSweep(sq, rho0)
+    m = Q.MasterSweep(sq, rho0, kernel=kernel)
     rows, dn = m.advance(iv)
     _, _, _, cnt = m.status()
     for k, sp in enumerate(specs):
@@ -136,3 +150,26 @@ def test_master_sweep_autostop_matches_oracle():
         # along their own step sequences
         assert np.abs(m.rho(k) - rho_o).max() < 1e-7
     assert len(set(np.round(tss, 6))) > 1  # the points stop at different times
+
+
+def test_master_kernels_agree():
+    """The two kernels on one 16-point sweep at d = 80: the same averages, norms and states to the
+    ODE tolerance (they sum in different orders; the step-size controller may tip)."""
+    print("[gpu] master kernels agree", flush=True)
+    etas = np.linspace(0.5, 12.0, 16)
+    sq = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in etas]
+    rho0 = np.zeros((80, 80), complex)
+    rho0[0, 0] = 1
+    iv = np.full(20, 0.1)
+    out = {}
+    for kernel in KERNELS:
+        m = Q.MasterSweep(sq, rho0, epsRel=1e-10, epsAbs=1e-10, kernel=kernel)
+        rows, dn = m.advance(iv)
+        out[kernel] = (rows, dn, m.status()[0], [m.rho(k) for k in range(len(sq))])
+    (ra, da, sa, pa), (rb, db, sb, pb) = out["mfma"], out["sparse"]
+    np.testing.assert_allclose(ra[..., 0], rb[..., 0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(ra[..., 2:], rb[..., 2:], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(da, db, rtol=1e-5, atol=1e-9)
+    np.testing.assert_array_equal(sa, sb)
+    for a, b in zip(pa, pb):
+        assert np.abs(a - b).max() < 1e-8

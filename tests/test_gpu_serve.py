@@ -118,8 +118,9 @@ def test_sweep_requests_same_alone_and_in_mix():
 
 
 def test_sweep_request_refusals():
-    """Sweep points must share dimension and jumps, run the Schroedinger picture and fit the
-    Master kernel (d <= 96); refused at submission, so the loop itself never fails on them."""
+    """Sweep points must share dimension and jumps, have a unitary interaction picture and fit the
+    requested Master kernel (the MFMA one: d <= 96); refused at submission, so the loop itself
+    never fails on them."""
     from sysdefs import jc_master_spec
     srv = Q.Server(chunk_intervals=4)
     a = Q.System(jc_master_spec(40, eta=1.0, g=10.0))
@@ -131,8 +132,8 @@ def test_sweep_request_refusals():
     big = Q.System(jc_master_spec(60, eta=1.0, g=10.0))
     rho_big = np.zeros((120, 120), complex)
     rho_big[0, 0] = 1
-    with pytest.raises(Q.QtError):
-        srv.submit_sweep([big], rho_big, T=1.0)              # beyond the Master kernel's 96
+    with pytest.raises(Q.QtError, match="up to 96"):
+        srv.submit_sweep([big], rho_big, T=1.0, kernel="mfma")   # beyond the MFMA kernel's 96
     ip = Q.System("free mode cutoff=20 delta=-1 kappa=1 eta=2 picture=IP")
     rho20 = np.zeros((20, 20), complex)
     rho20[0, 0] = 1
