@@ -19,7 +19,7 @@ CXXFLAGS = ["-O2", "-std=c++17", "-fPIC"]
 
 # heaviest first: the kernel templates are instantiated in several translation units, compiled in parallel
 SOURCES = [("qt_batch.hip", "hip"), ("qt_batch_td.hip", "hip"), ("qt_batch_gsl.hip", "hip"), ("qt_mixed.hip", "hip"),
-           ("qt_mixed_td.hip", "hip"), ("qt_mixed_gsl.hip", "hip"), ("qt_mixed_td_gsl.hip", "hip"), ("qt_aux.hip", "hip"),
+           ("qt_mixed_td.hip", "hip"), ("qt_mixed_gsl.hip", "hip"), ("qt_mixed_td_gsl.hip", "hip"), ("qt_aux.hip", "hip"), ("qt_plain.hip", "hip"),
            ("qt_subgroup.hip", "hip"), ("qt_master.hip", "hip"), ("qt_master_sparse.hip", "hip"), ("qt_capi.cpp", "hip"), ("qt_system.cpp", "cxx")]
 
 

@@ -1,5 +1,5 @@
-// qt_aux.hip -- jump draws, ensemble means, the plain one-trajectory form, the addContribution
-// drop-in (qt_apply_heff) and small fill kernels.
+// qt_aux.hip -- jump draws, the stream rows of the fixed-point sums, the addContribution drop-in
+// (qt_apply_heff) and small fill kernels.
 #include "qt_engine.hpp"
 
 namespace qt {
@@ -52,84 +52,6 @@ __global__ void stream_rows_kernel(const unsigned long long* acc, int n_int, int
       r[2 + q] -= r[2 + pvar[q]] * r[2 + pvar[q]];
       if (psqrt[q]) r[2 + q] = sqrt(r[2 + q]);
     }
-}
-
-// ---------------------------------------------------------------------------
-// plain form: one trajectory, one launch per RKCK try / per post-step phase
-// ---------------------------------------------------------------------------
-
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_try_kernel(SysDev S, PlainTry P) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS], xn[ROWS], k1[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? P.x[r * G + threadIdx.x] : make_double2(0., 0.);
-  long long derivs = 0;
-  eng.derivative(x, k1);  // controlled_runge_kutta::try_step_v1 recomputes dxdt on every try
-  const double err = sqrt(eng.tryStep(x, k1, P.dt, P.epsAbs, P.epsRel, xn, derivs));  // tryStep: squared norm
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) P.xn[r * G + threadIdx.x] = xn[r];
-  if (threadIdx.x == 0) *P.err = err;
-}
-
-// renormalise psi in place and compute the jump rates: out[0] = norm, out[1..nj] = rates
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_post_kernel(SysDev S, double2* psi, double* out) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-  double s = 0.;
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) { x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.); s += cabs2(x[r]); }
-  const double nrm = sqrt(eng.gsum(s));
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) { x[r].x /= nrm; x[r].y /= nrm; if (eng.valid(r)) psi[r * G + threadIdx.x] = x[r]; }
-  eng.stagePsi(x);
-  double rt[kMaxJumps];
-  eng.rates(rt);
-  if (threadIdx.x == 0) { out[0] = nrm; for (int m = 0; m < S.nj; ++m) out[1 + m] = rt[m]; }
-}
-
-// psi <- J_m psi / nf
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_jump_kernel(SysDev S, double2* psi, int m, double nf) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
-  eng.stagePsi(x);
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) { double2 y = eng.jumpRow(m, r); psi[r * G + threadIdx.x] = make_double2(y.x / nf, y.y / nf); }
-}
-
-// observables of one state: out[q]
-template <int ROWS, int NW, int WH>
-__global__ void __launch_bounds__(64 * NW) plain_obs_kernel(SysDev S, const double2* psi, double* out) {
-  using Eng = Engine<ROWS, NW, WH, false>;
-  constexpr int G = Eng::G;
-  __shared__ double2 xs[ROWS * G];
-  __shared__ double2 ps[ROWS * G];
-  __shared__ double red[NW > 1 ? NW : 1];
-  Eng eng(S, xs, ps, red, (int)threadIdx.x);
-  double2 x[ROWS];
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r) x[r] = eng.valid(r) ? psi[r * G + threadIdx.x] : make_double2(0., 0.);
-  eng.stagePsi(x);
-  for (int q = 0; q < S.no; ++q) { double v = eng.observable(q, x); if (threadIdx.x == 0) out[q] = v; }
 }
 
 // dpsidt += (H_nH/i)(t - t0) psi for a batch of state vectors, one group per vector (qt_apply_heff)
@@ -187,38 +109,6 @@ hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, 
   hipLaunchKernelGGL(stream_rows_kernel, dim3((n_int + 63) / 64), dim3(64), 0, st, acc, n_int, no, n_traj, t_start,
                      intervals, pvar, psqrt, rows);
   return hipGetLastError();
-}
-
-hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_try_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, P); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_post_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, out); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_jump_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, m, nf); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st) {
-#define QT_CASE(R, N, W) \
-  if (rows == R && nw == N && wh == W) { hipLaunchKernelGGL((plain_obs_kernel<R, N, W>), dim3(1), dim3(64 * N), 0, st, S, psi, out); return hipGetLastError(); }
-  QT_LAYOUTS(QT_CASE)
-#undef QT_CASE
-  return hipErrorInvalidValue;
 }
 
 hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const SysDev& S, const double2* psi,

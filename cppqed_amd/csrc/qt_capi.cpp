@@ -40,8 +40,10 @@ hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t st);
 hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
                               const double* intervals, const int* pvar, const int* psqrt, double* rows, hipStream_t st);
-hipError_t launch_plain_try(int rows, int nw, int wh, const SysDev& S, const PlainTry& P, hipStream_t st);
-hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, hipStream_t st);
+hipError_t launch_plain_try(int rows, int nw, int wh, bool td, bool gsl, const SysDev& S, const PlainTry& P,
+                            hipStream_t st);
+hipError_t launch_plain_post(int rows, int nw, int wh, const SysDev& S, double2* psi, double* out, double dtau,
+                             hipStream_t st);
 hipError_t launch_plain_jump(int rows, int nw, int wh, const SysDev& S, double2* psi, int m, double nf, hipStream_t st);
 hipError_t launch_plain_obs(int rows, int nw, int wh, const SysDev& S, const double2* psi, double* out, hipStream_t st);
 hipError_t launch_apply_heff(int rows, int nw, int wh, bool f32, bool td, const SysDev& S, const double2* psi,
@@ -1761,10 +1763,10 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     if (!sys || !psi0 || !intervals || n_int <= 0) throw Fail(QT_EINVAL, "bad arguments");
     check_params(p);
     const qt::System& s = *sys->h;
-    if (s.H.timeDependent() || s.hasEx) throw Fail(QT_ENOTSUP, "the plain form runs the Schroedinger picture only");
-    if (p->controller != QT_CONTROLLER_ODEINT) throw Fail(QT_ENOTSUP, "the plain form runs the odeint controller only");
+    sys->checkPictureSupported();
     if ((int)s.J.size() > qt::kMaxJumps) throw Fail(QT_ENOTSUP, "too many jump channels");
     sys->upload();
+    const bool td = sys->interactionPicture(), gsl = p->controller == QT_CONTROLLER_GSL;
     LayoutChoice L = pick_layout(s.dim, s.H.width, p->layout_rows, p->layout_waves);
     const long d = s.dim;
     const int nj = (int)s.J.size(), no = (int)s.O.size();
@@ -1775,7 +1777,7 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
     hipCheck(hipMemcpy(x.p, psi0, d * sizeof(double2), hipMemcpyHostToDevice), "H2D");
     std::vector<qt::cd> psi(d);
     for (long i = 0; i < d; ++i) psi[i] = qt::cd(psi0[2 * i], psi0[2 * i + 1]);
-    double t = 0., dtDid = 0., dtTry = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
+    double t = 0., t0 = 0., dtDid = 0., dtTry = p->dtInit > 0 ? p->dtInit : 1. / (10. * s.highestFrequency);
     const double epsAbs = p->epsRel, epsRel = p->epsAbs;  // ODE.h:205
     long long c[qt::kCounters] = {0};
     if (nj) {
@@ -1792,11 +1794,21 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
       if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
       if (std::fabs(dtTry) > std::fabs(Dt)) dtTry = Dt;
       for (;;) {
-        qt::PlainTry P{(int)d, x.p, xn.p, err.p, t, dtTry, epsAbs, epsRel};
-        hipCheck(qt::launch_plain_try(L.rows, L.nw, L.wh, sys->dev, P, st), "plain try");
+        qt::PlainTry P{(int)d, x.p, xn.p, err.p, t, dtTry, epsAbs, epsRel, t0};
+        hipCheck(qt::launch_plain_try(L.rows, L.nw, L.wh, td, gsl, sys->dev, P, st), "plain try");
         double e = 0.;
         hipCheck(hipMemcpy(&e, err.p, sizeof(double), hipMemcpyDeviceToHost), "D2H");
-        c[qt::kTries]++; c[qt::kDerivs] += 6;
+        c[qt::kTries]++; c[qt::kDerivs] += gsl ? 7 : 6;
+        if (gsl) {
+          // ControlledErrorStepperGSL::try_step (ODE_GSL.h:44-84): accepted, then
+          // gsl_odeiv2_control_hadjust with the order 5 of GSL's rkck
+          const double hOld = dtTry;
+          t += dtTry;
+          if (e > 1.1) dtTry = std::max(0.9 * std::pow(e, -1. / 5.), 0.2) * hOld;
+          else if (e < 0.5) dtTry = std::min(std::max(0.9 * std::pow(e, -1. / 6.), 1.), 5.) * hOld;
+          std::swap(x.p, xn.p);
+          break;
+        }
         if (e > 1.0) { dtTry *= std::max(9. / 10. * std::pow(e, -1. / (4 - 1)), 1. / 5.); c[qt::kFailed]++; continue; }
         t += dtTry;
         if (e < 0.5) { e = std::max(std::pow(5.0, -5.0), e); dtTry *= 9. / 10. * std::pow(e, -1. / 5); }
@@ -1806,7 +1818,9 @@ int qt_mcwf_plain_run(qt_sys* sys, const qt_mcwf_params* p, unsigned long long t
       dtDid = t - timeBefore;
       if (nextDtTry != 0.) dtTry = nextDtTry;
       c[qt::kOdeSteps]++;
-      hipCheck(qt::launch_plain_post(L.rows, L.nw, L.wh, sys->dev, x.p, post.p, st), "plain post");
+      // Exact::actWithU and t0 = t (MCWF_Trajectory.h:269-272), then renorm and the rates
+      hipCheck(qt::launch_plain_post(L.rows, L.nw, L.wh, sys->dev, x.p, post.p, td ? t - t0 : 0., st), "plain post");
+      if (td) t0 = t;
       hipCheck(hipMemcpy(hp.data(), post.p, (1 + nj) * sizeof(double), hipMemcpyDeviceToHost), "D2H");
     };
     double now = 0.;

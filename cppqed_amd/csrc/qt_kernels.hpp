@@ -120,6 +120,7 @@ struct PlainTry {
   double* err;       // [1]
   double t, dt;
   double epsAbs, epsRel;
+  double t0;         // picture origin (interaction picture)
 };
 
 }  // namespace qt

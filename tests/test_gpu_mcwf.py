@@ -83,16 +83,26 @@ def test_batched_fp64_trajectories_match_oracle(name, spec, psi0, T, ntraj, glan
     assert tot[4] == tries_o
 
 
-def test_plain_form_matches_oracle():
-    print("[gpu] plain form parity jc60", flush=True)
-    sq, so = Q.System(JC60), O.System(JC60)
+@pytest.mark.parametrize("name,spec,psi0,n_iv,ctl", [
+    ("jc60", JC60, ground(60), 10, "odeint"),
+    ("jc60_gsl", JC60, ground(60), 10, "gsl"),
+    # interaction picture: the post-step kernel applies Exact::actWithU before the renormalisation
+    ("p1m1_ip", P1M1, p1m1_initial(), 3, "odeint"),
+    ("p1m1_ip_gsl", P1M1, p1m1_initial(), 3, "gsl"),
+    ("plm20_ip", PLM20.replace("kappa", "picture=IP kappa"), coherent(0.5, 20), 10, "odeint"),
+])
+def test_plain_form_matches_oracle(name, spec, psi0, n_iv, ctl):
+    print(f"[gpu] plain form parity {name}", flush=True)
+    sq, so = Q.System(spec), O.System(spec)
     seed = 77
-    iv = np.full(10, 0.1)
+    iv = np.full(n_iv, 0.1)
     for k in range(3):
-        res_g = Q.mcwf_plain_run(sq, ground(60), iv, Q.mcwf_pars(seed=seed), traj=k)
-        res_o = _oracle_traj(so, ground(60), k, seed, iv)
+        res_g = Q.mcwf_plain_run(sq, psi0, iv, Q.mcwf_pars(seed=seed, controller=ctl), traj=k)
+        res_o = _oracle_traj(so, psi0, k, seed, iv, controller=ctl)
         _assert_traj_parity(res_o, res_g["psi"], res_g["jump_t"], res_g["jump_ch"], int(res_g["counters"][7]),
                             res_g["counters"])
+        if ctl == "gsl":
+            assert res_g["counters"][3] == res_o["counters"][3]
 
 
 def test_fp32_derivative_observable_error():
@@ -212,8 +222,6 @@ def test_gsl_controller_lds_teams_match_oracle(name, spec, psi0, T, ntraj, layou
 
 def test_gsl_controller_is_refused_where_not_built():
     sq = Q.System(JC60)
-    with pytest.raises(Q.QtError, match="odeint controller only"):
-        Q.mcwf_plain_run(sq, ground(60), np.full(2, 0.1), Q.mcwf_pars(controller="gsl"))
     with pytest.raises(Q.QtError, match="odeint controller only"):
         Q.Ensemble(sq, 4, ground(60), Q.mcwf_pars(controller="gsl", group_lanes=16))
 
