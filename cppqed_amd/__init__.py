@@ -96,6 +96,7 @@ def lib():
         l.qt_sys_destroy.argtypes = [vp]
         l.qt_sys_info.argtypes = [vp, C.POINTER(SysInfo)]
         l.qt_sys_table.argtypes = [vp, ip, C.POINTER(C.c_int), vp, vp, vp]
+        l.qt_sys_rules.argtypes = [vp, vp, vp, vp, vp]
         l.qt_apply_heff.argtypes = [vp, dp, dp, vp, vp, ip, ip, vp]
         l.qt_mcwf_create.argtypes = [vp, ip, C.POINTER(McwfParams), vp, C.POINTER(vp)]
         l.qt_mcwf_destroy.argtypes = [vp]
@@ -253,6 +254,17 @@ class System:
         self.spec = spec
         h = C.c_void_p()
         check(lib().qt_sys_build(spec.encode(), C.byref(h)))
+        self._init(h)
+
+    @classmethod
+    def adopt(cls, handle):
+        """A System owning an already built qt_sys handle (e.g. from qt_sys_build_terms)."""
+        s = cls.__new__(cls)
+        s.spec = None
+        s._init(handle)
+        return s
+
+    def _init(self, h):
         self.h = h
         info = SysInfo()
         check(lib().qt_sys_info(self.h, C.byref(info)))
@@ -279,6 +291,13 @@ class System:
         freqs = np.zeros(n * w, np.complex128)
         check(lib().qt_sys_table(self.h, which, None, cols.ctypes.data, vals.ctypes.data, freqs.ctypes.data))
         return cols.reshape(n, w), vals.reshape(n, w), freqs.reshape(n, w)
+
+    def rules(self):
+        """(Exact exponents per basis state, avr_imag, avr_var_of, avr_sqrt) (qt_sys_rules)"""
+        z = np.zeros(self.dim, np.complex128)
+        im, var, sq = (np.zeros(self.n_avr, np.int32) for _ in range(3))
+        check(lib().qt_sys_rules(self.h, z.ctypes.data, im.ctypes.data, var.ctypes.data, sq.ctypes.data))
+        return z, im, var, sq
 
     def dense(self, which=0):
         """Dense matrix of table `which` assembled from its ELL rows (testing aid)."""

@@ -554,6 +554,157 @@ int qt_sys_build(const char* spec, qt_sys** out) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// the entries of one Tridiagonal term (qt_tridiag): diagonal i's digits select per axis the main
+// (|n><n|), lower (|n+K><n|) or upper (|n><n+K|) diagonal; its values run row-major over the
+// axes with extent dim (main) or dim - K
+qt::Term tridiag_term(const qt::System& s, const qt_tridiag& t) {
+  const int R = t.rank;
+  if (R < 1 || R > (int)s.dims.size() || !t.axes || !t.k || !t.diags) throw Fail(QT_EINVAL, "bad Tridiagonal term");
+  qt::Term term;
+  term.explicitFreq = true;
+  std::vector<int> lstr(R, 1);
+  for (int a = 0; a < R; ++a) {
+    const int ax = t.axes[a];
+    if (ax < 0 || ax >= (int)s.dims.size()) throw Fail(QT_EINVAL, "Tridiagonal axis out of range");
+    for (int b = 0; b < a; ++b)
+      if (t.axes[b] == ax) throw Fail(QT_EINVAL, "Tridiagonal axes must differ");
+    if (t.k[a] < 0) throw Fail(QT_EINVAL, "negative Tridiagonal offset");
+    term.axes.push_back(ax);
+    term.op.dims.push_back(s.dims[ax]);
+  }
+  for (int a = R - 2; a >= 0; --a) lstr[a] = lstr[a + 1] * term.op.dims[a + 1];
+  int L = 1;
+  for (int a = 0; a < R; ++a) L *= 3;
+  std::vector<int> code(R), ext(R);
+  for (int i = 0; i < L; ++i) {
+    if (!t.diags[i]) continue;
+    long n = 1;
+    for (int a = R - 1, x = i; a >= 0; --a, x /= 3) {
+      code[a] = x % 3;
+      // Tridiagonal consistency (Tridiagonal.cc:47-70): off-diagonals need an offset
+      if (code[a] && t.k[a] == 0) throw Fail(QT_EINVAL, "Tridiagonal consistency error");
+      ext[a] = code[a] ? std::max(0, term.op.dims[a] - t.k[a]) : term.op.dims[a];
+      n *= ext[a];
+    }
+    const double* dv = t.diags[i];
+    const double* fv = t.freqs && i > 0 ? t.freqs[i] : nullptr;  // the main diagonal never rotates
+    for (long e = 0; e < n; ++e) {
+      int row = 0, col = 0;
+      long rem = e;
+      for (int a = R - 1; a >= 0; --a) {
+        const int m = (int)(rem % ext[a]);
+        rem /= ext[a];
+        row += (m + (code[a] == 1 ? t.k[a] : 0)) * lstr[a];
+        col += (m + (code[a] == 2 ? t.k[a] : 0)) * lstr[a];
+      }
+      term.op.entries.push_back({row, col, qt::cd(dv[2 * e], dv[2 * e + 1]),
+                                 fv ? qt::cd(fv[2 * e], fv[2 * e + 1]) : qt::cd(0.)});
+    }
+  }
+  return term;
+}
+
+qt::Term coo_term(const qt::System& s, const qt_coo& c) {
+  const int R = c.rank;
+  if (R < 1 || R > (int)s.dims.size() || !c.axes || c.nnz < 0 || (c.nnz && (!c.rows || !c.cols || !c.vals)))
+    throw Fail(QT_EINVAL, "bad sparse term");
+  qt::Term term;
+  term.explicitFreq = true;
+  long size = 1;
+  for (int a = 0; a < R; ++a) {
+    const int ax = c.axes[a];
+    if (ax < 0 || ax >= (int)s.dims.size()) throw Fail(QT_EINVAL, "sparse term axis out of range");
+    for (int b = 0; b < a; ++b)
+      if (c.axes[b] == ax) throw Fail(QT_EINVAL, "sparse term axes must differ");
+    term.axes.push_back(ax);
+    term.op.dims.push_back(s.dims[ax]);
+    size *= s.dims[ax];
+  }
+  for (long long e = 0; e < c.nnz; ++e) {
+    if (c.rows[e] < 0 || c.rows[e] >= size || c.cols[e] < 0 || c.cols[e] >= size)
+      throw Fail(QT_EINVAL, "sparse term index out of range");
+    term.op.entries.push_back({(int)c.rows[e], (int)c.cols[e], qt::cd(c.vals[2 * e], c.vals[2 * e + 1]), qt::cd(0.)});
+  }
+  return term;
+}
+
+std::vector<qt::Term> operator_terms(const qt::System& s, const qt_operator& o) {
+  if (o.n_tri < 0 || o.n_coo < 0 || (o.n_tri && !o.tri) || (o.n_coo && !o.coo)) throw Fail(QT_EINVAL, "bad operator");
+  std::vector<qt::Term> out;
+  for (int i = 0; i < o.n_tri; ++i) out.push_back(tridiag_term(s, o.tri[i]));
+  for (int i = 0; i < o.n_coo; ++i) out.push_back(coo_term(s, o.coo[i]));
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qt_sys_build_terms(const qt_system_desc* d, qt_sys** out) {
+  return guard([&] {
+    if (!d || !out) throw Fail(QT_EINVAL, "null argument");
+    if (d->n_frees < 1 || !d->frees) throw Fail(QT_EINVAL, "system without free elements");
+    if (d->n_jumps < 0 || d->n_avr < 0 || (d->n_jumps && !d->jumps) ||
+        (d->n_avr && (!d->avr || !d->avr_imag || !d->avr_var_of || !d->avr_sqrt)))
+      throw Fail(QT_EINVAL, "bad jump or average list");
+    if (!(d->highest_frequency >= 0.)) throw Fail(QT_EINVAL, "bad highest frequency");
+    std::unique_ptr<qt::System> sp(new qt::System);
+    qt::System& s = *sp;
+    for (int a = 0; a < d->n_frees; ++a) {
+      const qt_free_desc& f = d->frees[a];
+      if (f.dim < 1) throw Fail(QT_EINVAL, "free of no dimension");
+      qt::FreeInfo fi;
+      fi.kind = "terms";
+      fi.dim = f.dim;
+      if (f.exact) {
+        fi.hasEx = true;
+        for (int n = 0; n < f.dim; ++n) fi.exd.push_back(qt::cd(f.exact[2 * n], f.exact[2 * n + 1]));
+      }
+      s.frees.push_back(fi);
+      s.dims.push_back(f.dim);
+    }
+    s.strides.assign(s.dims.size(), 1);
+    s.dim = 1;
+    for (int i = (int)s.dims.size() - 1; i >= 0; --i) {
+      s.strides[i] = s.dim;
+      s.dim *= s.dims[i];
+      if (s.dim > (1L << 30)) throw Fail(QT_EINVAL, "system dimension too large");
+    }
+    s.hTerms = operator_terms(s, d->h);
+    for (int m = 0; m < d->n_jumps; ++m) s.jumps.push_back(operator_terms(s, d->jumps[m]));
+    for (int q = 0; q < d->n_avr; ++q) {
+      qt::Observable o;
+      o.terms = operator_terms(s, d->avr[q]);
+      o.imagPart = d->avr_imag[q] != 0;
+      s.obs.push_back(o);
+      const int v = d->avr_var_of[q];
+      if (v < -1 || v >= d->n_avr) throw Fail(QT_EINVAL, "bad variance rule");
+      s.processVarOf.push_back(v);
+      s.processSqrt.push_back(d->avr_sqrt[q] ? 1 : 0);
+    }
+    s.highestFrequency = d->highest_frequency;
+    for (auto& f : s.frees) s.hasEx = s.hasEx || f.hasEx;
+    if (s.hasEx) {
+      s.exactZ.assign(s.dim, 0.);
+      for (long r = 0; r < s.dim; ++r)
+        for (size_t a = 0; a < s.frees.size(); ++a)
+          if (s.frees[a].hasEx) s.exactZ[r] += s.frees[a].exd[(r / s.strides[a]) % s.dims[a]];
+    }
+    try {
+      s.lower();
+    } catch (const std::invalid_argument& ex) {
+      throw Fail(QT_EINVAL, ex.what());
+    }
+    std::unique_ptr<qt_sys> h(new qt_sys);
+    h->h = std::move(sp);
+    *out = h.release();
+  });
+}
+
 int qt_sys_destroy(qt_sys* sys) { delete sys; return QT_OK; }
 
 int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
@@ -581,6 +732,23 @@ int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info) {
     std::vector<int> offs, oc;
     std::vector<qt::cd> ov;
     info->h_offsets = offset_form(s.H, (int)s.dim, offs, oc, ov) ? (int)offs.size() : 0;
+  });
+}
+
+int qt_sys_rules(const qt_sys* sys, double* exact_z, int* avr_imag, int* avr_var_of, int* avr_sqrt) {
+  return guard([&] {
+    if (!sys) throw Fail(QT_EINVAL, "null argument");
+    const qt::System& s = *sys->h;
+    if (exact_z)
+      for (long r = 0; r < s.dim; ++r) {
+        const qt::cd z = s.hasEx ? s.exactZ[r] : qt::cd(0.);
+        exact_z[2 * r] = z.real(); exact_z[2 * r + 1] = z.imag();
+      }
+    for (size_t q = 0; q < s.obs.size(); ++q) {
+      if (avr_imag) avr_imag[q] = s.obs[q].imagPart ? 1 : 0;
+      if (avr_var_of) avr_var_of[q] = s.processVarOf[q];
+      if (avr_sqrt) avr_sqrt[q] = s.processSqrt[q] ? 1 : 0;
+    }
   });
 }
 

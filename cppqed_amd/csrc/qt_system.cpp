@@ -573,7 +573,7 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
     // (Tridiagonal.cc:24-36 leaves the main diagonal's empty).  So an entry of a term rotates with
     // the interaction picture only if all of the term's frees are Exact and the entry is off the
     // main diagonal on every one of its axes (a single-axis term needs only the former).
-    bool allEx = true;
+    bool allEx = !t.explicitFreq;
     for (int i = 0; i < r; ++i) allEx = allEx && s.frees[t.axes[i]].hasEx;
     // group entries by local row
     std::vector<std::vector<const LocalOp::Entry*>> byRow(op.size());
@@ -594,6 +594,7 @@ Ell embed(const System& s, const std::vector<Term>& terms) {
           if (allEx) freq += s.frees[ax].exd[rc] - s.frees[ax].exd[cc];
         }
         if (r > 1 && !offEverywhere) freq = 0.;
+        if (t.explicitFreq) freq = e->freq;
         auto& rv = rows[row];
         auto it = std::find_if(rv.begin(), rv.end(), [&](const E& x) { return x.col == (int)col && x.freq == freq; });
         if (it == rv.end()) rv.push_back({(int)col, e->val, freq});

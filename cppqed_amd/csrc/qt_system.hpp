@@ -43,7 +43,10 @@ LocalOp adjoint(const LocalOp& a);                         // conjugate transpos
 LocalOp scale(const LocalOp& a, cd f);
 LocalOp plus(const LocalOp& a, const LocalOp& b);          // concatenation (merged on embedding)
 
-struct Term { std::vector<int> axes; LocalOp op; };       // op acting on the listed axes (V order)
+// op acting on the listed axes (V order).  The interaction-picture frequency of an entry comes
+// from the Exact frees' exponents (the catalogue's elements), or, with explicitFreq, from the
+// entry itself (terms handed over as Tridiagonal data, whose frequencies they carry).
+struct Term { std::vector<int> axes; LocalOp op; bool explicitFreq = false; };
 
 struct Observable {
   std::vector<Term> terms;  // summed

@@ -30,19 +30,97 @@ const char* qt_last_error(void);
 
 /* ---------------------------------------------------------------- systems */
 
-/* Builds a system from an element spec (one statement per line or ';'):
+/* Builds a system from an element spec (one statement per line or ';'; frees are the axes
+ * 0, 1, ... in order, interactions name them):
  *   free mode cutoff=N delta=D kappa=K eta=E [etaI=..] [picture=Sch|IP|UIP]
- *   free qbit deltaA=D gamma=G eta=E [etaI=..] [picture=..]
- *   free ptla gamma=G deltaA=D eta=E
- *   ia jc i j g=G | ia bs i j g=U | ia zz i j J=J
+ *   free qbit deltaA=D gamma=G eta=E [etaI=..] [picture=Sch|IP|UIP]
+ *   free ptla gamma=G deltaA=D eta=E [etaI=..]
+ *   free particle fin=F omrec=W [vclass=V kpart=K mf=sin|cos|plus|minus] [picture=Sch|IP]
+ *   ia jc q m g=G [gI=..]              ia bs m1 m2 g=U [gI=..]          ia zz q1 q2 J=J
+ *   ia pcav m p u0=U kcav=K mf=cos|sin|plus|minus [vclass=V]
+ *   ia pcav_orth m p u0=U
+ *   ia ptm m0 m1 p u0=U0 u1=U1 phi=PHI k0=K k1=K mf0=.. mf1=..
  *   system binary|composite
  * Replaces the element makers and composition of the reference:
  *   mode::make  CPPQEDelements/frees/Mode.tcc:40-60,   qbit::make  Qbit.cc:217-246,
- *   PumpedTwoLevelAtomSch  PumpedTwoLevelAtom.cc:35-49, jaynescummings::make  JaynesCummings_.h:85-90,
+ *   PumpedTwoLevelAtomSch  PumpedTwoLevelAtom.cc:35-49, (Pumped)Particle  Particle.cc:79-110, 210-252,
+ *   jaynescummings::make  JaynesCummings_.h:85-90, CoupledModes  CoupledModes.cc:36-41,
+ *   ParticleAlongCavity / ParticleOrthogonalToCavity  ParticleCavity.cc:26-110,
+ *   ParticleTwoModes  ParticleTwoModes.cc:35-46 (equal wavenumbers); zz has no reference
+ *   counterpart (Qbit_.h:68-70 declares sigmazop without defining it);
  *   binary::make  CPPQEDcore/composites/BinarySystem.cc:323-333, composite::make  Composite.h:363-367.
- * The device copy of the tables is made on first use. */
+ * Elements outside this catalogue come in through qt_sys_build_terms.  The device copy of the
+ * tables is made on first use. */
 int qt_sys_build(const char* spec, qt_sys** out);
 int qt_sys_destroy(qt_sys* sys);
+
+/* A system from its operators, for elements outside the spec catalogue: the same lowering as
+ * qt_sys_build, fed with the data of the reference's quantumoperator::Tridiagonal terms.
+ *
+ * qt_tridiag is one Tridiagonal<R> (CPPQEDcore/quantumoperator/Tridiagonal.h:71-287) acting on
+ * the system axes axes[0..rank) -- the V of its Act<V...> (structure/Interaction.h), or the free's
+ * own axis.  k[a] is axis a's off-diagonal offset (Tridiagonal::getDifferences).  diags[i],
+ * i < 3^rank, is the diagonal Tridiagonal::get()(i): digit a of i in base 3 (axis 0 the most
+ * significant, as the direct product numbers them, Tridiagonal.tcc:40-50) selects on axis a the
+ * main diagonal (0, |n><n|), the lower one (1, |n+K><n|) or the upper one (2, |n><n+K|); the
+ * array is row-major over those axes with extent dims[a] (digit 0) or dims[a] - k[a], as
+ * interleaved (re, im) doubles, or NULL for an empty diagonal.  freqs (may be NULL) likewise
+ * holds Tridiagonal::getFreqs(): an entry's value at picture time tau is diag exp(freq tau)
+ * (Tridiagonal::propagate, Tridiagonal.tcc:154-163); the main diagonal never rotates.
+ * qt_coo is a general sparse operator on axes (an observable that is no Tridiagonal, e.g. the
+ * particle's position operator): entry e is vals[e] |rows[e]><cols[e]| with row-major local
+ * indices over the axes' dimensions. */
+typedef struct {
+  int rank;
+  const int* axes;              /* [rank] */
+  const int* k;                 /* [rank] */
+  const double* const* diags;   /* [3^rank] */
+  const double* const* freqs;   /* [3^rank] or NULL */
+} qt_tridiag;
+
+typedef struct {
+  int rank;
+  const int* axes;              /* [rank] */
+  long long nnz;
+  const long long* rows;        /* [nnz] */
+  const long long* cols;        /* [nnz] */
+  const double* vals;           /* [2 nnz] */
+} qt_coo;
+
+/* an operator: the sum of its terms (Tridiagonals first, in order, then the sparse ones) */
+typedef struct {
+  int n_tri;
+  const qt_tridiag* tri;
+  int n_coo;
+  const qt_coo* coo;
+} qt_operator;
+
+typedef struct {
+  int dim;
+  /* Exact part (structure::Exact, Exact.h): U = exp(-exact[n] (t - t0)) on level n, interleaved
+   * (re, im) [2 dim]; NULL: the free has none (Schroedinger picture) */
+  const double* exact;
+} qt_free_desc;
+
+typedef struct {
+  int n_frees;
+  const qt_free_desc* frees;    /* the system's axes in order (binary / composite frees) */
+  qt_operator h;                /* H_nH/i: every element's TridiagonalHamiltonian terms
+                                   (TridiagonalHamiltonian.h:21-87), applied in this order */
+  int n_jumps;
+  const qt_operator* jumps;     /* J_m in the reference's channel order (frees first) */
+  int n_avr;
+  const qt_operator* avr;       /* raw linear averages <psi|O_q|psi> */
+  const int* avr_imag;          /* [n_avr]: 1 streams Im<O_q>, 0 Re<O_q> */
+  const int* avr_var_of;        /* [n_avr]: q' whose square is subtracted (VAR), or -1 */
+  const int* avr_sqrt;          /* [n_avr]: square root after the subtraction (DEV) */
+  double highest_frequency;     /* QuantumSystem::highestFrequency (DynamicsBase.cc:49-66) */
+} qt_system_desc;
+
+/* Builds a system from its operators (copied; the caller keeps its arrays).  Replaces the
+ * element classes' construction of their Tridiagonals and their binary / composite assembly
+ * (structure/Free.h, composites/BinarySystem.cc:323-333, Composite.h:363-367). */
+int qt_sys_build_terms(const qt_system_desc* desc, qt_sys** out);
 
 typedef struct {
   long dim;                 /* total Hilbert-space dimension */
@@ -67,12 +145,18 @@ int qt_sys_info(const qt_sys* sys, qt_sys_info_t* info);
  * which: 0 = H_nH/i, 1+m = J_m, 1+n_jumps+q = observable q, -1 = H_nH/i in its
  * constant-offset form (as the device holds it; QT_EINVAL when h_offsets is 0). */
 int qt_sys_table(const qt_sys* sys, int which, int* width, int* cols, double* vals, double* freqs);
+/* The lowered system's Exact exponents per basis state (the sum over the frees of their exact[n],
+ * interleaved, [2*dim]; zeros without an Exact part) and the streaming rules of its averages (as
+ * qt_system_desc's avr_imag, avr_var_of, avr_sqrt, [n_avr] each).  Any pointer may be NULL. */
+int qt_sys_rules(const qt_sys* sys, double* exact_z, int* avr_imag, int* avr_var_of, int* avr_sqrt);
 
 /* dpsidt += (H_nH/i) psi for n state vectors, device pointers, on `stream`.
  * Replaces structure::Hamiltonian<RANK>::addContribution(t, psi, dpsidt, t0)
  * (CPPQEDcore/structure/Hamiltonian.h:38-43) as dispatched by
  * binary::Hamiltonian::addContribution_v (BinarySystem.cc:194-207) /
- * composite (Composite.tcc:178-197).  Schroedinger-picture systems only. */
+ * composite (Composite.tcc:178-197).  In the interaction picture the entries are taken at
+ * picture time t - t0 (TDH_Base::addContribution_v with Tridiagonal::propagate,
+ * TridiagonalHamiltonian.h:42-47). */
 int qt_apply_heff(qt_sys* sys, double t, double t0, const double* psi_dev, double* dpsidt_dev, int n,
                   int precision, qt_stream stream);
 
@@ -84,9 +168,8 @@ typedef struct {
   double dtInit;             /* <= 0: 1/(10*highest_frequency), Trajectory.h:126 */
   int precision;             /* QT_FP64 | QT_FP32_DERIV */
   int controller;            /* QT_CONTROLLER_ODEINT (ODE.h:189-223) or QT_CONTROLLER_GSL (ODE_GSL.h:44-84:
-                                every step accepted, gsl_odeiv2_control_hadjust; ensembles, mixed
-                                batches and the request loop; not the sub-wave layouts or the plain
-                                form) */
+                                every step accepted, gsl_odeiv2_control_hadjust; every form but
+                                the sub-wave layouts) */
   unsigned long long seed;   /* counter-RNG key word 0 */
   unsigned long long traj0;  /* RNG key word 1 of trajectory 0 (trajectory k uses traj0+k) */
   int jump_log_cap;          /* jump records kept per trajectory */
