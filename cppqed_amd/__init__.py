@@ -42,7 +42,7 @@ class McwfParams(C.Structure):
                 ("overshootTol", C.c_double), ("dtInit", C.c_double), ("precision", C.c_int),
                 ("controller", C.c_int), ("seed", C.c_ulonglong), ("traj0", C.c_ulonglong),
                 ("jump_log_cap", C.c_int), ("layout_rows", C.c_int), ("layout_waves", C.c_int),
-                ("layout_occupancy", C.c_int), ("layout_group_lanes", C.c_int)]
+                ("layout_occupancy", C.c_int), ("layout_group_lanes", C.c_int), ("sched_chunk", C.c_int)]
 
 
 class MasterParams(C.Structure):
@@ -329,12 +329,14 @@ def run_intervals(T=None, Dt=0.1, NDt=0):
 # ---------------------------------------------------------------- MCWF ensemble
 
 def mcwf_pars(epsRel=1e-6, epsAbs=1e-12, dpLimit=0.01, overshootTol=10.0, dtInit=0.0, precision=QT_FP64,
-              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0), occupancy=0, group_lanes=0, controller="odeint"):
+              seed=1001, traj0=0, jump_log_cap=64, layout=(0, 0), occupancy=0, group_lanes=0, controller="odeint",
+              sched_chunk=0):
     """mcwf::Pars + ode_engine::Pars with the reference defaults (MCWF_Trajectory.h:28-47, ODE.h:31-63).
-    controller: "odeint" (ODE_EngineBoost, the default) or "gsl" (ODE_EngineGSL, ODE_GSL.h)."""
+    controller: "odeint" (ODE_EngineBoost, the default) or "gsl" (ODE_EngineGSL, ODE_GSL.h).
+    sched_chunk: stream intervals per scheduling chunk (0 automatic, < 0 whole trajectories)."""
     ctl = {"odeint": QT_CONTROLLER_ODEINT, "gsl": QT_CONTROLLER_GSL}[controller]
     return McwfParams(epsRel, epsAbs, dpLimit, overshootTol, dtInit, precision, ctl, seed, traj0,
-                      jump_log_cap, layout[0], layout[1], occupancy, group_lanes)
+                      jump_log_cap, layout[0], layout[1], occupancy, group_lanes, sched_chunk)
 
 
 class Ensemble:

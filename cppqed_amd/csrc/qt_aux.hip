@@ -5,21 +5,25 @@
 namespace qt {
 
 // jump draws for the next `cap` MCWF steps of every trajectory, one Philox evaluation per lane:
-// draws[traj][i] = jump_uniform(seed, traj0 + traj, ordinal[traj] + i)
+// draws[traj][i] = jump_uniform(seed, traj0 + traj, ordinal[traj] + i); base[traj] = ordinal[traj]
+// (when base is given: the ordinal of draws[traj][0], for kernels that resume a trajectory
+// mid-launch)
 __global__ void jump_draws_kernel(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
-                                  int n_traj, int cap, double* draws) {
+                                  int n_traj, int cap, double* draws, unsigned long long* base) {
   const size_t total = (size_t)n_traj * cap;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const int traj = (int)(i / cap), k = (int)(i % cap);
-    draws[i] = jump_uniform(seed, traj0 + (unsigned long long)traj, ordinal[traj] + (unsigned long long)k);
+    const unsigned long long o = ordinal[traj];
+    draws[i] = jump_uniform(seed, traj0 + (unsigned long long)traj, o + (unsigned long long)k);
+    if (base && k == 0) base[traj] = o;
   }
 }
 
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
-                             int n_traj, int cap, double* draws, hipStream_t st) {
+                             int n_traj, int cap, double* draws, hipStream_t st, unsigned long long* base) {
   const size_t total = (size_t)n_traj * cap;
   const int grid = (int)std::min<size_t>((total + 255) / 256, 16384);
-  hipLaunchKernelGGL(jump_draws_kernel, dim3(grid), dim3(256), 0, st, ordinal, seed, traj0, n_traj, cap, draws);
+  hipLaunchKernelGGL(jump_draws_kernel, dim3(grid), dim3(256), 0, st, ordinal, seed, traj0, n_traj, cap, draws, base);
   return hipGetLastError();
 }
 
@@ -89,7 +93,8 @@ __global__ void fill_int_kernel(int* p, int n, int v) {
 }
 
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st) {
-  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, st, p, n, v);
+  const int grid = std::max(1, std::min((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(fill_int_kernel, dim3(grid), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ hipError_t launch_sub(int g, int rows, int wh, bool f32, int grid, const SysDev&
 hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, bool gsl, int grid, const SysDev& S,
                         const EnsDev& E, hipStream_t st);
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
-                             int n_traj, int cap, double* draws, hipStream_t st);
+                             int n_traj, int cap, double* draws, hipStream_t st, unsigned long long* base);
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw);
 int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl);
 int mixed_variants(int* table, int cap);
@@ -373,8 +373,10 @@ struct qt_ens {
   DevBuf<unsigned long long> drawBase;
   int hostDrawsPer = 0;
   DevBuf<unsigned long long> ord;
+  DevBuf<unsigned long long> ordBase;  // each trajectory's RNG ordinal at launch start (index 0 of its bulk draws)
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
+  DevBuf<int> prog;                    // finished chunks per trajectory (interleaved scheduling)
   int lastIntervals = 0;
   int grid = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -504,6 +506,7 @@ void ens_check_flag(int flag) {
   if (flag & 1) throw Fail(QT_ENUMERIC, "Infinite detected in MCWF_Trajectory::performJump");
   if (flag & 4) throw Fail(QT_ENUMERIC, "a streamed average exceeds the range of the ensemble sums (2^43 / n_traj)");
   if (flag & 8) throw Fail(QT_EINVAL, "a trajectory needed more jump draws than qt_mcwf_set_draws supplied");
+  if (flag & 16) throw Fail(QT_EDEVICE, "a chunk of the interleaved schedule waited too long for its trajectory's previous chunk");
 }
 
 // rates of a host state through the host ELL tables: ||J_m psi||^2
@@ -858,6 +861,8 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
     e->jt.alloc((size_t)n_traj * e->p.jump_log_cap);
     e->jch.alloc((size_t)n_traj * e->p.jump_log_cap);
     e->work.alloc(1);
+    e->prog.alloc(n_traj);
+    e->ordBase.alloc(n_traj);
     e->lastObs.alloc((size_t)n_traj * std::max<size_t>(1, s.O.size()));  // also what qt_mcwf_advance_noalloc writes
     e->errflag.upload(std::vector<int>(1, 0));
     const int perGroup = e->lay.glanes > 0 ? 64 / e->lay.glanes : 1;  // trajectories per work group
@@ -870,6 +875,32 @@ int qt_mcwf_create(qt_sys* sys, int n_traj, const qt_mcwf_params* p, const doubl
 }
 
 int qt_mcwf_destroy(qt_ens* ens) { delete ens; return QT_OK; }
+
+// Interleaved scheduling of the one-wavefront engines (EnsDev::chunk): with more trajectories than
+// resident wavefronts, a launch is cut into chunks of about n_intervals / 8 stream intervals
+// (qt_mcwf_params::sched_chunk overrides: > 0 intervals per chunk, < 0 off) and every
+// trajectory's chunk c runs before any chunk c + 1.  Results do not depend on it: draws are
+// indexed by RNG ordinal and the stream sums are order-independent.
+static void ens_schedule(const qt_ens* e, qt::EnsDev& E) {
+  E.chunk = 0; E.prog = nullptr;
+  if (e->lay.glanes > 0 || e->lay.nw != 1 || E.dc > 0 || E.n_intervals < 2) return;
+  int ch = e->p.sched_chunk;
+  if (ch < 0) return;
+  if (ch == 0) {
+    if (e->n <= e->grid) return;  // every trajectory is resident from the start
+    ch = (E.n_intervals + 7) / 8;  // JC-60, 16384 trajectories, 200 intervals: 14.4 ms with 25, 17 ms whole
+  }
+  const long long items = (long long)((E.n_intervals + ch - 1) / ch) * e->n;
+  if (items <= e->n || items > (1ll << 30)) return;  // one chunk, or a queue beyond the 32-bit head
+  E.chunk = ch; E.prog = e->prog.p;
+}
+
+// Resets the work queue (and the chunk progress counts) before a launch, with fill kernels so
+// that the sequence can be captured.
+static void ens_reset_queue(qt_ens* e, const qt::EnsDev& E, hipStream_t st) {
+  hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+  if (E.chunk > 0) hipCheck(qt::launch_fill_int(E.prog, e->n, 0, st), "reset chunk progress");
+}
 
 // Per-launch preparation of one ensemble: uploads the intervals, sizes the stream outputs,
 // generates the launch's jump draws (unless launchDraws is false: then only sized); returns the
@@ -894,9 +925,12 @@ static qt::EnsDev ens_prepare(qt_ens* e, const double* intervals, int n_int, hip
     const int cap = (int)std::min(2048.0, std::max(64.0, std::ceil(span / std::max(e->dtMinSeen, 1e-6) * 1.25) + 8));
     e->draws.alloc((size_t)e->n * cap);
     if (launchDraws)
-      hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st), "jump draws");
-    E.draws = e->draws.p; E.draw_cap = cap;
+      hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, cap, e->draws.p, st, e->ordBase.p), "jump draws");
+    // index 0 of a trajectory's draws is its ordinal at launch start (the draw kernel records it):
+    // a chunk that resumes the trajectory mid-launch finds its draws from there
+    E.draws = e->draws.p; E.draw_cap = cap; E.draw_base = e->ordBase.p;
   }
+  ens_schedule(e, E);
   return E;
 }
 
@@ -926,9 +960,10 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
       if (!e->capStream) hipCheck(hipStreamCreate(&e->capStream), "hipStreamCreate");
       hipCheck(hipStreamBeginCapture(e->capStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       if (E.draws && !E.draws_strict)
-        hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, E.draw_cap, e->draws.p, e->capStream), "jump draws");
+        hipCheck(qt::launch_jump_draws(e->ord.p, e->p.seed, e->p.traj0, e->n, E.draw_cap, e->draws.p, e->capStream,
+                                       e->ordBase.p), "jump draws");
       hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, n_int), e->capStream), "zero stream sums");
-      hipCheck(qt::launch_fill_int(e->work.p, 1, 0, e->capStream), "reset work queue");
+      ens_reset_queue(e, E, e->capStream);
       ens_kernel(e, E, e->capStream);
       hipCheck(hipStreamEndCapture(e->capStream, &e->graph), "hipStreamEndCapture");
       hipCheck(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -942,7 +977,7 @@ static void ens_launch(qt_ens* e, const double* intervals, int n_int, hipStream_
   } else {
     qt::EnsDev E = ens_prepare(e, intervals, n_int, st, e->lay.glanes == 0);
     hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, n_int), st), "zero stream sums");
-    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    ens_reset_queue(e, E, st);
     if (timed) hipCheck(hipEventRecord(e->e0, st), "hipEventRecord");
     ens_kernel(e, E, st);
     if (timed) hipCheck(hipEventRecord(e->e1, st), "hipEventRecord");
@@ -1066,9 +1101,10 @@ int qt_mcwf_advance_noalloc(qt_ens* e, const double* intervals, int n_int, doubl
     E.intervals = ivDev; E.n_intervals = n_int;
     E.acc = acc; E.acc_max = std::ldexp(1.0, 43) / e->n; E.last_obs = e->lastObs.p;
     if (!e->hostDrawsPer) { E.draws = nullptr; E.draw_cap = 0; }  // jump draws from the counter RNG inside the kernel
+    ens_schedule(e, E);
     const double tStart = e->time;
     hipCheck(qt::launch_zero_u64(acc, ens_acc_words(e, n_int), st), "zero stream sums");
-    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    ens_reset_queue(e, E, st);
     ens_kernel(e, E, st);
     hipCheck(qt::launch_stream_rows(acc, n_int, no, e->n, tStart, ivDev, e->sys->dev.pvar, e->sys->dev.psqrt, rowsDev, st),
              "stream rows launch");
@@ -1100,10 +1136,11 @@ int qt_mcwf_advance_dc(qt_ens* e, double t_end, int dc, int max_rows, double* ro
     iv[0] = t_end - e->time;  // sizes the launch's draw budget; dc mode ignores the intervals
     qt::EnsDev E = ens_prepare(e, iv.data(), max_rows, st, true);
     E.dc = dc; E.t_end = t_end;
+    ens_schedule(e, E);  // (dc mode runs unchunked)
     e->rowsDev.alloc((size_t)max_rows * (2 + no));
     E.rows_out = e->rowsDev.p;
     hipCheck(qt::launch_zero_u64(E.acc, ens_acc_words(e, max_rows), st), "zero stream sums");
-    hipCheck(qt::launch_fill_int(e->work.p, 1, 0, st), "reset work queue");
+    ens_reset_queue(e, E, st);
     ens_kernel(e, E, st);
     std::vector<double> r((size_t)max_rows * (2 + no));
     hipCheck(hipMemcpyAsync(r.data(), e->rowsDev.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
@@ -1451,6 +1488,7 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
     qt_ens* e = b->mem[m];
     if (!intervals[m]) throw Fail(QT_EINVAL, "null interval list");
     M.E[m] = ens_prepare(e, intervals[m], n_int[m], st, true);
+    M.E[m].chunk = 0; M.E[m].prog = nullptr;  // the mixed kernel runs whole trajectories
     hipCheck(qt::launch_zero_u64(M.E[m].acc, ens_acc_words(e, n_int[m]), st), "zero stream sums");
     M.S[m] = e->sys->dev;
     M.team[m] = b->team[m];

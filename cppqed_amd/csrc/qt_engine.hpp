@@ -789,9 +789,13 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 // Called by every lane of a team (eng.lane); pc (LDS, when PC_LDS) keeps psi at the start of the
 // MCWF step for a step-back, rts / cnt (LDS) hold the step's jump rates and the trajectory's
 // counters.
+// kFirst, kLast: the stream intervals [kFirst, kLast) of the launch this call covers (kLast < 0: to
+// the last one);
+// the trajectory's state is read from E at the start and written back at the end, so a later call
+// continues where this one stopped.
 template <class Eng, bool PC_LDS, bool GSL = false>
 __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, const Eng& eng, int traj, double2* pc,
-                                             double* rts, long long* cnt) {
+                                             double* rts, long long* cnt, int kFirst = 0, int kLast = -1) {
   constexpr int ROWS = Eng::kRows, G = Eng::G, NW = Eng::kNW;
   using T = typename Eng::T;
   const int lane = eng.lane;
@@ -945,7 +949,8 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
     if (withRates) eng.sync();
   };
 
-  for (int k = 0; k < E.n_intervals; ++k) {
+  const int kEnd = kLast < 0 ? E.n_intervals : kLast;
+  for (int k = kFirst; k < kEnd; ++k) {
     // Dt mode: advance(intervals[k]); dc mode: up to dc MCWF steps step(t_end - t) per stream point
     // (trajectory::run with dc > 0, Trajectory.h:560-600)
     const double endTime = E.dc > 0 ? E.t_end : t + E.intervals[k];
@@ -1050,6 +1055,38 @@ __device__ __forceinline__ void advance_traj(const SysDev& S, const EnsDev& E, c
   if (lane < kCounters) E.counters[(size_t)traj * kCounters + lane] = cnt[lane];
 }
 
+// ---------------------------------------------------------------------------
+// interleaved chunks of trajectories (EnsDev::chunk)
+// ---------------------------------------------------------------------------
+
+constexpr int kErrChunkWait = 16;  // err_flag bit: a chunk's wait for its predecessor timed out
+
+// Waits until the trajectory's progress count *p reaches `target` (its previous chunk is done),
+// then acquires that chunk's stores of the trajectory state.  The chunk was claimed earlier by a
+// resident wavefront, which waits only on chunks claimed before its own, so the wait ends; as a
+// guard it gives up after ~10 s (100 MHz real-time counter) or once another wait has given up,
+// flagging the launch.
+__device__ __forceinline__ bool chunk_wait(const int* p, int target, int* err) {
+  const unsigned long long start = __builtin_amdgcn_s_memrealtime();
+  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrChunkWait) return false;
+    if (__builtin_amdgcn_s_memrealtime() - start > 1000000000ull) {
+      if ((threadIdx.x & 63) == 0) atomicOr(err, kErrChunkWait);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+// Publishes the trajectory state written by this wavefront, then the new progress count.
+__device__ __forceinline__ void chunk_publish(int* p, int count) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  wave_sync();
+  if ((threadIdx.x & 63) == 0) __hip_atomic_store(p, count, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // psi at the start of the MCWF step goes to LDS when the stage vector and it fit
 template <int ROWS, int NW>
 constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2) <= 150 * 1024; }
@@ -1112,7 +1149,23 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
       eng.jwl = jwl;
       wave_sync();
     }
-    for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds, GSL>(S, E, eng, traj, pc, rts, cnt);
+    if (E.chunk > 0) {
+      // interleaved chunks: every trajectory's chunk c is claimed before any chunk c + 1, so all of
+      // them start early instead of the last ones queueing behind whole trajectories (the costs of
+      // trajectories are heavy-tailed: one that starts late and takes several times the mean
+      // number of steps otherwise sets the length of the launch).  (Raising the issue priority of
+      // chunks that fall behind was measured to slow the launch down and is not done.)
+      const int nt = E.n_traj, ch = E.chunk;
+      const int items = (E.n_intervals + ch - 1) / ch * nt;
+      for (int item; (item = claim()) < items;) {
+        const int c = item / nt, traj = item - c * nt;
+        if (c > 0 && !chunk_wait(E.prog + traj, c, E.err_flag)) continue;
+        advance_traj<Eng, kPcLds, GSL>(S, E, eng, traj, pc, rts, cnt, c * ch, min(E.n_intervals, (c + 1) * ch));
+        chunk_publish(E.prog + traj, c + 1);
+      }
+    } else {
+      for (int traj; (traj = claim()) < E.n_traj;) advance_traj<Eng, kPcLds, GSL>(S, E, eng, traj, pc, rts, cnt);
+    }
   }
 }
 

@@ -95,6 +95,12 @@ struct EnsDev {
   double t_end;
   double* rows_out;             // [n_traj][n_intervals][2 + no] raw rows (time, dtDid, averages) or null
   int draws_strict;             // 1: draws are host-supplied, running out of them fails the launch (bit 3)
+  // interleaved scheduling (one-wavefront engines): the work items are (chunk c, trajectory) in
+  // chunk-major order, chunk c = stream intervals [c chunk, (c + 1) chunk); prog[traj] counts the
+  // trajectory's finished chunks, and the claimer of chunk c waits until chunk c - 1 is done.
+  // chunk = 0: one item per trajectory covering every interval of the launch
+  int chunk;
+  int* prog;                    // [n_traj]
 };
 
 // Mixed batch: up to kMaxMembers ensembles of different systems advanced by one launch.

@@ -176,6 +176,11 @@ typedef struct {
   int layout_rows, layout_waves;  /* 0 = pick automatically (autotuned table) */
   int layout_occupancy;      /* minimum waves per SIMD asked of the register allocator; 0 = table */
   int layout_group_lanes;    /* 16 | 32 | 64: several trajectories per wavefront (G lanes each); 0 = table */
+  int sched_chunk;           /* stream intervals per scheduling chunk of the one-wavefront layouts: with
+                                more trajectories than resident wavefronts a launch runs every
+                                trajectory's chunk c before any chunk c+1.  0 = automatic (about
+                                n_intervals/8), < 0 = whole trajectories.  Results are the same
+                                either way. */
 } qt_mcwf_params;
 
 /* nTraj copies of psi0 (host) on one system: mcwf::makeEnsemble, MCWF_Trajectory.h:213-234,
