@@ -108,6 +108,7 @@ def lib():
         l.qt_mcwf_reset.argtypes = [vp, vp]
         l.qt_mcwf_set_graph.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         l.qt_mcwf_traj_averages.argtypes = [vp, vp]
+        l.qt_mcwf_rho.argtypes = [vp, vp]
         l.qt_mcwf_state.argtypes = [vp, ip, vp, vp]
         l.qt_mcwf_jumplog.argtypes = [vp, ip, vp, vp, ip, C.POINTER(C.c_int)]
         l.qt_mcwf_counters.argtypes = [vp, vp, vp]
@@ -415,6 +416,13 @@ class Ensemble:
     def traj_averages(self):
         out = np.zeros((self.n, self.system.n_avr))
         check(lib().qt_mcwf_traj_averages(self.h, out.ctypes.data))
+        return out
+
+    def rho(self):
+        """The ensemble's density operator (1/N) sum_k |psi_k><psi_k| of the current states (qt_mcwf_rho)."""
+        d = self.system.dim
+        out = np.zeros((d, d), np.complex128)
+        check(lib().qt_mcwf_rho(self.h, out.ctypes.data))
         return out
 
     def state(self, i):

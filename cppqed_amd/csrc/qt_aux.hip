@@ -87,6 +87,84 @@ __global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const dou
 }
 
 
+// The ensemble's density operator on request (AverageTrajectories<MCWF_Trajectory>,
+// MCWF_Trajectory.h:406-420, with StateVector::addTo, StateVector.h:173-180):
+// rho_ij = (1/N) sum_k psi_k[i] conj(psi_k[j]).  A work group owns a 64 x 64 tile of rho and a
+// range of `per` trajectories (split blockIdx.z); 16 trajectories' tile segments are staged in LDS
+// at a time and each thread accumulates a 4 x 4 block (rows ty + 16 r, columns tx + 16 c).  With one
+// split the scaled sums go straight to rho; otherwise each split writes its partial sums to
+// part[split] and rho_reduce_kernel adds them in split order (the result does not depend on
+// scheduling).
+__global__ void __launch_bounds__(256) ensemble_rho_kernel(const double2* __restrict__ psi, int n, int d, int per,
+                                                           double scale, double2* __restrict__ out) {
+  __shared__ double2 A[16][64], B[16][64];
+  const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;
+  const int kBeg = blockIdx.z * per, kEnd = min(n, kBeg + per);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double2 acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = make_double2(0., 0.);
+  for (int k0 = kBeg; k0 < kEnd; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, ii = e & 63, k = k0 + kk;
+      const bool in = k < kEnd;
+      A[kk][ii] = in && ti + ii < d ? psi[(size_t)k * d + ti + ii] : make_double2(0., 0.);
+      B[kk][ii] = in && tj + ii < d ? psi[(size_t)k * d + tj + ii] : make_double2(0., 0.);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      double2 a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { a[r] = A[kk][ty + 16 * r]; b[r] = B[kk][tx + 16 * r]; }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {  // a conj(b)
+          acc[r][c].x = fma(a[r].x, b[c].x, fma(a[r].y, b[c].y, acc[r][c].x));
+          acc[r][c].y = fma(a[r].y, b[c].x, fma(-a[r].x, b[c].y, acc[r][c].y));
+        }
+    }
+    __syncthreads();
+  }
+  double2* o = out + (size_t)blockIdx.z * d * d;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = ti + ty + 16 * r, j = tj + tx + 16 * c;
+      if (i < d && j < d) o[(size_t)i * d + j] = make_double2(acc[r][c].x * scale, acc[r][c].y * scale);
+    }
+}
+
+__global__ void rho_reduce_kernel(const double2* __restrict__ part, int splits, size_t dd, double scale,
+                                  double2* __restrict__ rho) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < dd; i += (size_t)gridDim.x * blockDim.x) {
+    double2 s = part[i];
+    for (int q = 1; q < splits; ++q) { s.x += part[q * dd + i].x; s.y += part[q * dd + i].y; }
+    rho[i] = make_double2(s.x * scale, s.y * scale);
+  }
+}
+
+// rho (d x d, row-major) of the n states psi[n][d]; part: splits * d * d scratch when splits > 1
+hipError_t launch_ensemble_rho(const double2* psi, int n, int d, int splits, double2* part, double2* rho,
+                               hipStream_t st) {
+  const int per = ((n + splits - 1) / splits + 15) / 16 * 16;
+  splits = (n + per - 1) / per;
+  const int tiles = (d + 63) / 64;
+  if (splits <= 1) {
+    hipLaunchKernelGGL(ensemble_rho_kernel, dim3(tiles, tiles, 1), dim3(256), 0, st, psi, n, d, n, 1.0 / n, rho);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(ensemble_rho_kernel, dim3(tiles, tiles, splits), dim3(256), 0, st, psi, n, d, per, 1.0, part);
+  const size_t dd = (size_t)d * d;
+  const int grid = (int)std::min<size_t>((dd + 255) / 256, 4096);
+  hipLaunchKernelGGL(rho_reduce_kernel, dim3(grid), dim3(256), 0, st, part, splits, dd, 1.0 / n, rho);
+  return hipGetLastError();
+}
+
 // p[0..n) = v  (a kernel rather than a memset node, so that the launch sequence can be captured)
 __global__ void fill_int_kernel(int* p, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;

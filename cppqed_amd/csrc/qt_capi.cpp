@@ -37,6 +37,8 @@ int mixed_variants(int* table, int cap);
 hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
                         const MixDev* M, hipStream_t st);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
+hipError_t launch_ensemble_rho(const double2* psi, int n, int d, int splits, double2* part, double2* rho,
+                               hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, size_t n, hipStream_t st);
 hipError_t launch_stream_rows(const unsigned long long* acc, int n_int, int no, int n_traj, double t_start,
                               const double* intervals, const int* pvar, const int* psqrt, double* rows, hipStream_t st);
@@ -377,6 +379,7 @@ struct qt_ens {
   DevBuf<long long> cnt;
   DevBuf<int> jch, work, errflag;
   DevBuf<int> prog;                    // finished chunks per trajectory (interleaved scheduling)
+  DevBuf<double2> rhoDev, rhoPart;     // the ensemble's density operator on request (qt_mcwf_rho)
   int lastIntervals = 0;
   int grid = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1211,6 +1214,20 @@ int qt_mcwf_traj_averages(qt_ens* e, double* out) {
     const int no = (int)s.O.size();
     if (!e->lastIntervals) throw Fail(QT_EINVAL, "no interval advanced yet");
     if (no) hipCheck(hipMemcpy(out, e->lastObs.p, (size_t)e->n * no * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+  });
+}
+
+int qt_mcwf_rho(qt_ens* e, double* rho) {
+  return guard([&] {
+    if (!e || !rho) throw Fail(QT_EINVAL, "null argument");
+    const int d = (int)e->sys->h->dim, n = e->n;
+    // split the trajectories so that about 1024 work groups share the sum (fixed split order)
+    const int tiles = (d + 63) / 64;
+    const int splits = std::max(1, std::min((1024 + tiles * tiles - 1) / (tiles * tiles), (n + 15) / 16));
+    e->rhoDev.alloc((size_t)d * d);
+    if (splits > 1) e->rhoPart.alloc((size_t)splits * d * d);
+    hipCheck(qt::launch_ensemble_rho(e->psi.p, n, d, splits, e->rhoPart.p, e->rhoDev.p, nullptr), "ensemble rho");
+    hipCheck(hipMemcpy(rho, e->rhoDev.p, (size_t)d * d * sizeof(double2), hipMemcpyDeviceToHost), "D2H");
   });
 }
 

@@ -245,6 +245,12 @@ int qt_mcwf_set_graph(qt_ens* ens, int enable, long long* captures, long long* r
 
 /* Per-trajectory raw (unprocessed) averages at the last streamed point: out[n_traj*n_avr]. */
 int qt_mcwf_traj_averages(qt_ens* ens, double* out);
+/* The ensemble's density operator, on request (AverageTrajectories<MCWF_Trajectory>,
+ * MCWF_Trajectory.h:406-420, with StateVector::addTo, StateVector.h:173-180):
+ * rho = (1/N) sum_k |psi_k><psi_k| of the trajectories' current states, formed on the device in FP64
+ * (a fixed-order sum, so repeated calls agree bit for bit; the order differs from the reference's
+ * sequential addTo at rounding level).  rho_host: d*d complex, row-major.  Synchronising. */
+int qt_mcwf_rho(qt_ens* ens, double* rho_host);
 /* State of trajectory i (host copy) and its time. */
 int qt_mcwf_state(qt_ens* ens, int i, double* psi_host, double* t);
 /* Jump record of trajectory i (MCWF_TrajectoryLogger jump list). */

@@ -226,6 +226,7 @@ def test_null_handles_are_refused():
         ("qt_mcwf_jumplog", (None, 0, None, None, 0, ctypes.byref(i))),
         ("qt_mcwf_counters", (None, None, ll)),
         ("qt_mcwf_traj_averages", (None, None)),
+        ("qt_mcwf_rho", (None, None)),
         ("qt_mcwf_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
         ("qt_master_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
         ("qt_batch_kernel_ms", (None, ctypes.byref(d), ctypes.byref(i))),
