@@ -225,6 +225,11 @@ struct qt_sys {
   DevBuf<double> jwts;   // rate weights [nj][dim] (SysDev::jw)
   int nfreq = 0;          // distinct interaction-picture frequencies of H (0: Schroedinger picture)
   DevBuf<float2> hval32;
+  // dictionary form of the constant-offset table (SysDev::hdict)
+  DevBuf<double2> hdict;
+  DevBuf<float2> hdict32;
+  DevBuf<int> hdbase;
+  DevBuf<unsigned char> hidx;
   qt::SysDev dev{};
 
   void upload() {
@@ -261,6 +266,42 @@ struct qt_sys {
     std::vector<float2> v32;
     for (auto& z : v) v32.push_back(make_float2((float)z.x, (float)z.y));
     hval32.upload(v32);
+    // dictionary form of the constant-offset table: when every slot's values take at most 256
+    // distinct bit patterns (kMaxDict in all, at most kMaxDictSlots slots), the distinct values go
+    // to a dictionary, slot by slot, and each entry becomes a byte index into its slot's part
+    // (slot-major like the table); the team engines copy both into LDS, so that a gather reads
+    // its coefficients from LDS instead of global memory
+    int ndict = 0;
+    if (!offs.empty() && s.H.width <= qt::kMaxDictSlots) {
+      const int w0 = s.H.width;
+      std::vector<double2> dict;
+      std::vector<int> base(qt::kMaxDictSlots, 0);
+      std::vector<unsigned char> idx((size_t)w0 * d, 0);
+      bool ok = true;
+      for (int k = 0; k < w0 && ok; ++k) {
+        std::map<std::pair<unsigned long long, unsigned long long>, int> seen;
+        base[k] = (int)dict.size();
+        for (int r = 0; r < d && ok; ++r) {
+          const double2 z = v[(size_t)k * d + r];
+          std::pair<unsigned long long, unsigned long long> key;
+          std::memcpy(&key.first, &z.x, 8);
+          std::memcpy(&key.second, &z.y, 8);
+          auto it = seen.find(key);
+          if (it == seen.end()) {
+            if (seen.size() == 256 || dict.size() == (size_t)qt::kMaxDict) { ok = false; break; }
+            it = seen.emplace(key, (int)seen.size()).first;
+            dict.push_back(z);
+          }
+          idx[(size_t)k * d + r] = (unsigned char)it->second;
+        }
+      }
+      if (ok && !dict.empty()) {
+        std::vector<float2> d32;
+        for (auto& z : dict) d32.push_back(make_float2((float)z.x, (float)z.y));  // as hval32
+        hdict.upload(dict); hdict32.upload(d32); hdbase.upload(base); hidx.upload(idx);
+        ndict = (int)dict.size();
+      }
+    }
     int wj = 1;
     for (auto& j : s.J) wj = std::max(wj, j.width);
     c.clear(); v.clear();
@@ -354,6 +395,9 @@ struct qt_sys {
     dev.dim = d; dev.wh = s.H.width;
     dev.hcol = hcol.p; dev.hval = hval.p; dev.hval32 = hval32.p;
     dev.hoff = offs.empty() ? nullptr : hoff.p;
+    dev.ndict = ndict;
+    dev.hdict = ndict ? hdict.p : nullptr; dev.hdict32 = ndict ? hdict32.p : nullptr;
+    dev.hdbase = ndict ? hdbase.p : nullptr; dev.hidx = ndict ? hidx.p : nullptr;
     dev.nj = (int)s.J.size(); dev.wj = wj; dev.jcol = jcol.p; dev.jval = jval.p;
     dev.no = (int)s.O.size(); dev.wo = wo; dev.ocol = ocol.p; dev.oval = oval.p; dev.oimag = oimag.p; dev.owid = owid.p;
     dev.pvar = pvar.p; dev.psqrt = psqrt.p;

@@ -68,6 +68,11 @@ struct Engine {
   const T* hvalU;
   const int* hoffU;  // constant-offset form of the table (SysDev::hoff), or null
   int whU, dimU;
+  // dictionary form of the table (SysDev::hdict): the team's LDS copies of the dictionary and of the
+  // slots' bases (loadDict), and the rows' index words; null without one
+  const T* dictL = nullptr;
+  const unsigned char* idxL = nullptr;
+  const int* dbL = nullptr;
   // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
   // launch's error flag (set when a wait times out)
   unsigned* bar = nullptr;
@@ -121,6 +126,31 @@ struct Engine {
           }
         }
       }
+    }
+  }
+
+  // Copies the system's coefficient dictionary into the team's LDS buffers -- dict [kMaxDict],
+  // base [kMaxDictSlots], idx [idxCap bytes] -- and switches the gathers to it; ends with a team
+  // barrier.  Without a dictionary, when its index does not fit idxCap, or for an engine without
+  // the offset form, nothing changes.
+  __device__ __forceinline__ void loadDict(T* dict, int* base, unsigned char* idx, int idxCap) {
+    if constexpr (kRegs || !(OPT & kOffsets)) {
+      (void)dict; (void)base; (void)idx; (void)idxCap;
+    } else {
+      const int n = uniform(S.ndict);
+      if (!hoffU || n <= 0 || n > kMaxDict || whU * dimU > idxCap) return;
+      const T* src;
+      if constexpr (F32) src = uniform(S.hdict32);
+      else src = uniform(S.hdict);
+      for (int i = lane; i < n; i += G) dict[i] = src[i];
+      if (lane < kMaxDictSlots) base[lane] = S.hdbase[lane];
+      // the index bytes in 4-byte words (whU * dimU is a multiple of 4: dim is, for every
+      // layout with a dictionary)
+      const unsigned* isrc = reinterpret_cast<const unsigned*>(uniform(S.hidx));
+      unsigned* idst = reinterpret_cast<unsigned*>(idx);
+      for (int i = lane; i < whU * dimU / 4; i += G) idst[i] = isrc[i];
+      dictL = dict; dbL = base; idxL = idx;
+      sync();
     }
   }
 
@@ -202,7 +232,15 @@ struct Engine {
         for (int w = 0; w < WH; ++w) acc = cfma(hval[r][w], xs[hcol[r][w]], acc);
       } else {
         const int row = r * G + lane;
-        if ((OPT & kOffsets) && row < dimU && hoffU) {
+        if ((OPT & kOffsets) && row < dimU && hoffU && dictL) {
+          // dictionary form: the coefficient is the entry of slot w's part of the dictionary named
+          // by the row's index byte, both read from the team's LDS copies
+#pragma unroll 4
+          for (int w = 0; w < whU; ++w) {
+            const int c = min(max(row + gbl(hoffU)[w], 0), dimU - 1);
+            acc = cfma(dictL[dbL[w] + idxL[(unsigned)(w * dimU + row)]], xs[c], acc);
+          }
+        } else if ((OPT & kOffsets) && row < dimU && hoffU) {
           // constant offsets: the column is computed, only the coefficient is loaded
 #pragma unroll 4
           for (int w = 0; w < whU; ++w) {
@@ -566,8 +604,9 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
     return err;
   }
 
-  // <X|O_q|X>, real or imaginary part as the observable asks
-  __device__ double observableX(int q) const {
+  // <X|O_q|X>, real or imaginary part as the observable asks (inlined: an out-of-line member call
+  // would put the engine object in private memory, behind a scratch load on every member access)
+  __device__ __forceinline__ double observableX(int q) const {
     double s = 0.;
     const bool im = S.oimag[q] != 0;
 #pragma unroll
@@ -672,7 +711,10 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
   };
 
   const double* jw = S.jw;
+  // every channel's rate in one team hand-over: each wavefront's partial sums go to red[m][wave],
+  // then lane m adds channel m's partials in wave order (the sums of separate gsum calls)
   auto computeRates = [&]() {
+    eng.sync();  // previous readers of red are done
     for (int m = 0; m < nj; ++m) {
       double sacc = 0.;
       if (jw) {  // rate weights: no gather
@@ -683,8 +725,14 @@ __device__ __forceinline__ void advance_traj_lds(const SysDev& S, const EnsDev& 
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) if (eng.valid(r)) sacc += cabs2(eng.jumpRow(m, r));
       }
-      const double v = eng.gsum(sacc);
-      if (lane == 0) rts[m] = v;
+      sacc = wave_sum(sacc);
+      if ((lane & 63) == 0) eng.red[m * NW + (lane >> 6)] = sacc;
+    }
+    eng.sync();
+    if (lane < nj) {
+      double sum = 0.;
+      for (int k = 0; k < NW; ++k) sum += eng.red[lane * NW + k];
+      rts[lane] = sum;
     }
     eng.sync();
   };
@@ -1104,7 +1152,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
   // kernel arguments live in LDS: read where used instead of pinning ~80 scalar registers
   __shared__ SysDev S;
   __shared__ EnsDev E;
-  __shared__ double red[NW > 1 ? NW : 1];
+  __shared__ double red[NW > 1 ? NW * kMaxJumps : 1];
   __shared__ double rts[kMaxJumps];        // jump rates of the current step
   __shared__ long long cnt[kCounters];     // this trajectory's counters (lane 0 updates)
   __shared__ T ph[kMaxFreqs];              // interaction-picture phasors of the current stage
@@ -1123,11 +1171,21 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // multi-wavefront team: psi in LDS
     __shared__ double2 Xs[ROWS * G];
     __shared__ T xs[ROWS * G];
-    // a second stage buffer when it fits beside the state (saves a barrier per derivative)
-    constexpr bool kTwo = (size_t)ROWS * G * (sizeof(double2) + 2 * sizeof(T)) <= 150 * 1024;
+    // the H table's coefficient dictionary with its index bytes (Schroedinger picture), when LDS
+    // allows, and a second stage buffer when it fits beside the state (saves a barrier per
+    // derivative); the dictionary goes first (dim 4096: 297 -> 209 ms without the second buffer)
+    constexpr size_t kBase = (size_t)ROWS * G * (sizeof(double2) + sizeof(T));
+    constexpr size_t kIdx = (size_t)kMaxDictSlots * ROWS * G;
+    constexpr bool kDict = !TD && kBase + kIdx + kMaxDict * sizeof(T) + 8 * 1024 <= 160 * 1024;
+    constexpr size_t kDictBytes = kDict ? kIdx + kMaxDict * sizeof(T) : 0;
+    constexpr bool kTwo = kBase + ROWS * G * sizeof(T) + kDictBytes <= 150 * 1024;
     __shared__ T xs2[kTwo ? ROWS * G : 1];
     LdsEngine<ROWS, NW, WH, F32, false, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
                                                                                  kTwo ? xs2 : nullptr);
+    __shared__ T dict[kDict ? kMaxDict : 1];
+    __shared__ int dbase[kMaxDictSlots];
+    __shared__ __align__(16) unsigned char didx[kDict ? kIdx : 4];
+    if constexpr (kDict) eng.loadDict(dict, dbase, didx, (int)kIdx);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds<decltype(eng), GSL>(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
@@ -1218,13 +1276,24 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   constexpr size_t kPh = TDC ? kMaxFreqs * sizeof(TB) + 16 : 0;
   constexpr size_t kBlockBytes = (size_t)RB * GB * (sizeof(double2) + sizeof(TB));
   constexpr size_t kWaveEach = ((size_t)RW * 64 * sizeof(double2) * 2 + (kMaxJumps + kCounters) * 8 + kPh + 15) / 16 * 16;
-  constexpr size_t kQuarterOwn = (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ + kMaxJumps + kCounters) * 8 + kPh;
+  constexpr size_t kQuarterOwn =
+      (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ * kMaxJumps + kMaxJumps + kCounters) * 8 + kPh;
   constexpr size_t kQuarterRaw = kQuarterOwn > kWaveEach * NWQ ? kQuarterOwn : kWaveEach * NWQ;
   constexpr size_t kQuarter = (kQuarterRaw + 15) / 16 * 16;
   constexpr size_t kBytes = kBlockBytes > 4 * kQuarter ? kBlockBytes : 4 * kQuarter;
+  // the work-group team's coefficient dictionary with its index bytes, where LDS allows
+  // (the index bytes get what LDS has left, up to kMaxDictSlots slots; loadDict checks the fit)
+  constexpr long kIdxAvail = 160L * 1024 - (long)kBytes - kMaxDict * (long)sizeof(TB) - (long)sizeof(MixDev) - 6 * 1024;
+  constexpr size_t kIdxB = kIdxAvail < 4L * RB * GB ? 0
+                           : (size_t)(kIdxAvail / 1024 * 1024 < (long)kMaxDictSlots * RB * GB ? kIdxAvail / 1024 * 1024
+                                                                                              : (long)kMaxDictSlots * RB * GB);
+  constexpr bool kDictB = kIdxB > 0;
   __shared__ MixDev M;
   __shared__ __align__(16) unsigned char pool[kBytes];
-  __shared__ double red[NWB];
+  __shared__ TB dictS[kDictB ? kMaxDict : 1];
+  __shared__ int dbaseS[kMaxDictSlots];
+  __shared__ __align__(16) unsigned char didxS[kDictB ? kIdxB : 4];
+  __shared__ double red[NWB * kMaxJumps];
   __shared__ double rtsB[kMaxJumps];
   __shared__ long long cntB[kCounters];
   __shared__ int shItem;
@@ -1263,6 +1332,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       if (item >= M.items[0]) break;
       const int m = member(0, item);
       EngB eng(M.S[m], xs, X, red, (int)threadIdx.x, TDC ? phB : nullptr);
+      if constexpr (kDictB) eng.loadDict(dictS, dbaseS, didxS, (int)kIdxB);
       advance_member_lds<GSLC>(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
     }
     __syncthreads();
@@ -1277,7 +1347,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
     double2* X = reinterpret_cast<double2*>(qbase);
     TB* xs = reinterpret_cast<TB*>(X + RB * GQ);
     double* qred = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(xs) + RB * GQ * sizeof(TB));
-    double* rts = qred + NWQ;
+    double* rts = qred + NWQ * kMaxJumps;
     long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
     TB* qph = TDC ? align16(cnt + kCounters) : nullptr;
     int* slot = qslot + q;

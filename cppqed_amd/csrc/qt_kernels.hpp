@@ -10,6 +10,8 @@ constexpr int kMaxJumps = 8;      // jump channels handled by the batched kernel
 constexpr int kCounters = 8;      // per-trajectory counters, see Counter
 constexpr int kMaxFreqs = 32;     // distinct interaction-picture frequencies of one H_nH/i
 constexpr int kAccStripes = 16;   // stream accumulators are striped by trajectory index (less contention)
+constexpr int kMaxDict = 512;     // entries of the H table's coefficient dictionary (SysDev::hdict)
+constexpr int kMaxDictSlots = 12; // ELL slots of a table in dictionary form
 
 enum Counter {
   kMcwfSteps = 0,   // MCWF_Trajectory::step calls
@@ -55,6 +57,15 @@ struct SysDev {
   // row holds the entry at column row + hoff[w] (zero where the row has none, its hcol clamped
   // into range), so a gather may compute the column instead of loading it
   const int* hoff;         // [wh]
+  // dictionary form of the constant-offset table (with hoff, when the table has at most
+  // kMaxDictSlots slots, each slot takes at most 256 distinct values and all slots together at most
+  // kMaxDict): slot w of row r holds hdict[hdbase[w] + hidx[w][r]] (hdict32: the same values in
+  // FP32).  Null without one.
+  int ndict;
+  const double2* hdict;
+  const float2* hdict32;
+  const int* hdbase;           // [kMaxDictSlots]
+  const unsigned char* hidx;   // [wh][dim]
 };
 
 // Device view of an MCWF ensemble and the per-launch work description.
