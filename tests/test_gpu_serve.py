@@ -117,6 +117,33 @@ def test_sweep_requests_same_alone_and_in_mix():
     np.testing.assert_array_equal(alone.reshape(len(pts), len(iv), -1), rows)
 
 
+def test_several_sweeps_in_flight():
+    """Three sweep requests in flight at once (each on its own stream with its own staging),
+    different point counts and lengths so that last chunks are shorter (chunks of 4 intervals:
+    5 = 4 + 1, 7 = 4 + 3, 13 = 4 + 4 + 4 + 1), beside ensemble requests: each sweep's rows equal
+    the same sweep driven directly."""
+    from sysdefs import jc_master_spec
+    print("[gpu] serve: three sweeps in flight", flush=True)
+    rho0 = np.zeros((80, 80), complex)
+    rho0[0, 0] = 1
+    cases = [([0.5], 0.5), ([2.0, 7.5], 0.7), ([1.0, 4.0, 12.0], 1.3)]
+    jc = Q.System(JC60)
+    srv = Q.Server(chunk_intervals=4, pars=Q.mcwf_pars(precision=Q.QT_FP32_DERIV))
+    ens_ids = [srv.submit(jc, 300, ground(60), T=1.1, seed=5)]
+    sweep_ids = []
+    for etas, T in cases:
+        pts = [Q.System(jc_master_spec(40, eta=e, g=10.0)) for e in etas]
+        sweep_ids.append((srv.submit_sweep(pts, rho0, T=T, epsRel=1e-8, epsAbs=1e-8), pts, T))
+    ens_ids.append(srv.submit(jc, 200, ground(60), T=0.6, seed=6))
+    res = _drain(srv, ens_ids + [r for r, _, _ in sweep_ids])
+    assert all(res[r] is not None for r in ens_ids)
+    for rid, pts, T in sweep_ids:
+        iv = Q.run_intervals(T=T, Dt=0.1)
+        m = Q.MasterSweep(pts, rho0, epsRel=1e-8, epsAbs=1e-8)
+        rows, _ = m.advance(iv)
+        np.testing.assert_array_equal(res[rid][0].reshape(len(pts), len(iv), -1), rows)
+
+
 def test_sweep_request_refusals():
     """Sweep points must share dimension and jumps, have a unitary interaction picture and fit the
     requested Master kernel (the MFMA one: d <= 96); refused at submission, so the loop itself
