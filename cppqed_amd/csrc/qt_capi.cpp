@@ -677,7 +677,8 @@ qt::Term coo_term(const qt::System& s, const qt_coo& c) {
   for (long long e = 0; e < c.nnz; ++e) {
     if (c.rows[e] < 0 || c.rows[e] >= size || c.cols[e] < 0 || c.cols[e] >= size)
       throw Fail(QT_EINVAL, "sparse term index out of range");
-    term.op.entries.push_back({(int)c.rows[e], (int)c.cols[e], qt::cd(c.vals[2 * e], c.vals[2 * e + 1]), qt::cd(0.)});
+    const qt::cd f = c.freqs ? qt::cd(c.freqs[2 * e], c.freqs[2 * e + 1]) : qt::cd(0.);
+    term.op.entries.push_back({(int)c.rows[e], (int)c.cols[e], qt::cd(c.vals[2 * e], c.vals[2 * e + 1]), f});
   }
   return term;
 }

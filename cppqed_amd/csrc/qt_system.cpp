@@ -2,6 +2,7 @@
 #include "qt_system.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <memory>
 #include <sstream>
@@ -41,6 +42,24 @@ LocalOp plus(const LocalOp& a, const LocalOp& b) {
   if (a.dims != b.dims) throw std::invalid_argument("operator dimension mismatch");
   LocalOp r = a;
   r.entries.insert(r.entries.end(), b.entries.begin(), b.entries.end());
+  return r;
+}
+
+// the product a b of two operators on the same axes (entries summed per row and column; frequencies
+// of the factors' entries add, as the entries' phases multiply)
+static LocalOp matmul(const LocalOp& a, const LocalOp& b) {
+  if (a.dims != b.dims) throw std::invalid_argument("operator dimension mismatch");
+  std::map<std::pair<int, int>, std::pair<cd, cd>> acc;
+  std::vector<std::vector<const LocalOp::Entry*>> bRows(b.size());
+  for (auto& y : b.entries) bRows[y.row].push_back(&y);
+  for (auto& x : a.entries)
+    for (const LocalOp::Entry* y : bRows[x.col]) {
+      auto& e = acc[{x.row, y->col}];
+      e.first += x.val * y->val;
+      e.second = x.freq + y->freq;
+    }
+  LocalOp r; r.dims = a.dims;
+  for (auto& kv : acc) r.entries.push_back({kv.first.first, kv.first.second, kv.second.first, kv.second.second});
   return r;
 }
 
@@ -389,8 +408,14 @@ struct Builder {
   // mode detuning.  The interference term with etaeff = sign(u0) sqrt(u0 v): for a particle
   // without its own pump, or with vclass= given (v = vclass), etaeff (a (x) mf(kcav x) + h.c.) / i;
   // for a pumped particle (v = its vClass) and kcav = kPart, etaeff (a^dagger (x)
-  // mfComposition(mf, mfPart) + h.c.) / i.  A pumped particle of another wavenumber takes a
-  // product of two Tridiagonals (ParticleCavity.cc:113-140) that is not built.
+  // mfComposition(mf, mfPart) + h.c.) / i.  A pumped particle of another wavenumber takes the
+  // chained product of ParticleCavity.cc:102-140: secondH (firstH psi) + secondHT (firstHT psi) with
+  // firstH = etaeff a (x) mf(kcav x) / i, firstHT = -firstH^dagger, secondH = mfPart(kPart x)^dagger
+  // on the particle, i.e. etaeff (a (x) mfPart^dagger mf + a^dagger (x) mfPart mf^dagger) / i, each
+  // product formed on the (truncated) momentum grid.  Its entries shift the particle by
+  // +-kPart +- kcav, never 0, so in the interaction picture each rotates as one entry when both
+  // frees are Exact; with only the particle Exact the reference rotates the second factor alone
+  // (per intermediate momentum), which one ELL entry cannot hold: refused.
   void pcav(int m, int p, const KV& kv) {
     const FreeInfo& fm = freeAt(m); const FreeInfo& fp = particleAt(p);
     if (fm.kind != "mode") throw std::invalid_argument("pcav needs a mode as its first free");
@@ -404,8 +429,10 @@ struct Builder {
     const double v = pumpedCtor ? fp.vclass : num(kv, "vclass", 0.);
     if (u0 * v < 0.) throw std::invalid_argument("Unot VClass sign discrepancy");
     const double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * v);
-    if (pumpedCtor && kcav != std::abs(fp.kpart))
-      throw std::invalid_argument("pcav with a pumped particle of another wavenumber is not built");
+    const bool special = !pumpedCtor || kcav == std::abs(fp.kpart);
+    if (!special && etaeff != 0. && fp.hasEx && !fm.hasEx)
+      throw std::invalid_argument("pcav with a pumped particle of another wavenumber needs the mode in the interaction "
+                                  "picture when the particle is");
     const double sgn = mfs == "sin" ? -1. : 1.;
     const int K = 2 * kcav;
     LocalOp t; t.dims = {fm.dim, fp.dim};
@@ -420,10 +447,16 @@ struct Builder {
       }
       t = scale(kron(n, c), cd(u0 * sgn, 0.) / cd(0., 2.));
     }
-    if (etaeff != 0.) {
+    if (etaeff != 0. && special) {
       LocalOp x = pumpedCtor ? kron(adjoint(destroy(fm.dim)), mfComposition(fp.dim, mfs, fp.mf, kcav))
                              : kron(destroy(fm.dim), modeFunction(fp.dim, mfs, kcav));
       t = plus(t, scale(plus(x, adjoint(x)), cd(etaeff, 0.) / cd(0., 1.)));
+    }
+    if (etaeff != 0. && !special) {
+      const LocalOp mc = modeFunction(fp.dim, mfs, kcav), mp = modeFunction(fp.dim, fp.mf, fp.kpart);
+      const LocalOp x = plus(kron(destroy(fm.dim), matmul(adjoint(mp), mc)),
+                             kron(adjoint(destroy(fm.dim)), matmul(mp, adjoint(mc))));
+      t = plus(t, scale(x, cd(etaeff, 0.) / cd(0., 1.)));
     }
     s.hTerms.push_back({{m, p}, t});
     // RF{"Unot", uNot, mode dim}, RF{"etaeff", etaeff, sqrt(mode dim)}

@@ -151,13 +151,15 @@ def _outer(a, b, f):
 
 
 class Sparse:
-    """An operator that is no Tridiagonal (qt_coo): entries (row, col, value) over its dims."""
+    """An operator that is no Tridiagonal (qt_coo): entries (row, col, value) over its dims, each
+    rotating with its frequency in the interaction picture when freqs is given."""
 
-    def __init__(self, dims, rows, cols, vals):
+    def __init__(self, dims, rows, cols, vals, freqs=None):
         self.dims = tuple(dims)
         self.rows = np.asarray(rows, np.int64)
         self.cols = np.asarray(cols, np.int64)
         self.vals = np.asarray(vals, np.complex128)
+        self.freqs = None if freqs is None else np.asarray(freqs, np.complex128)
 
 
 class Free:
@@ -459,13 +461,56 @@ def particle_along_cavity(m, p, im, ip, u0=0.0, kcav=1, mf="cos", vclass=None):
         c = Tridiagonal.unary(k=K, minus=[0.5] * (p.dim - K), plus=[0.5] * (p.dim - K))
         c = c if p.exact is None else c.furnish_with_freqs(p.exact)
         h.append((n * c).scaled(complex(u0 * sgn, 0.0) / complex(0.0, 2.0)))
-    if ee != 0.0:
+    special = not pumped_ctor or kcav == abs(p.kpart)
+    if ee != 0.0 and special:
         if pumped_ctor:
             x = aop(m.dim, m).dagger() * mf_composition(p.dim, mf, p.mf, kcav, p)
         else:
             x = aop(m.dim, m) * mode_function(p.dim, mf, kcav, p)
         h.append((x + x.dagger()).scaled(complex(ee, 0.0) / complex(0.0, 1.0)))
+    if ee != 0.0 and not special:
+        h.append(_chained_pcav(m, p, ee, mf, kcav))
     return Interaction((im, ip), h, max(abs(u0) * m.dim, abs(ee) * math.sqrt(float(m.dim))))
+
+
+def _mf_matrix(dim, mf, nK):
+    """mfNKX on the truncated momentum grid as a dense matrix (e^{iKx}: |q> -> |q+K>)"""
+    e = np.zeros((dim, dim), complex)
+    for q in range(dim - abs(nK)):
+        if nK >= 0:
+            e[q + nK, q] = 1.0
+        else:
+            e[q, q - nK] = 1.0
+    if mf == "sin":
+        return (e + e.conj().T * -1.0) * (1.0 / complex(0.0, 2.0))
+    if mf == "cos":
+        return (e + e.conj().T) * 0.5
+    return e if mf == "plus" else e.conj().T
+
+
+def _chained_pcav(m, p, ee, mf, kcav):
+    """ParticleAlongCavity with a pumped particle of another wavenumber (ParticleCavity.cc:102-140):
+    secondH (firstH psi) + secondHT (firstHT psi), firstH = ee aop(mode) mfNKX(particle, (mf, kcav)) / i,
+    secondH = mfNKX(particle, particle's mf)^dagger, i.e. ee (a (x) mfPart^dagger mf + a^dagger (x)
+    mfPart mf^dagger) / i: a product of Tridiagonals, handed over as a sparse term.  Its entries
+    shift the particle by +-kPart +- kcav (never 0), so with both frees Exact each rotates with
+    exact_r - exact_c over both axes; with only the particle Exact it cannot be one entry."""
+    if p.exact is not None and m.exact is None:
+        raise ValueError("the chained particle-cavity term needs the mode in the interaction picture when the particle is")
+    mc, mp = _mf_matrix(p.dim, mf, kcav), _mf_matrix(p.dim, p.mf, p.kpart)
+    f = complex(ee, 0.0) / complex(0.0, 1.0)
+    rows, cols, vals, freqs = [], [], [], []
+    both = p.exact is not None and m.exact is not None
+    for P, up in ((mp.conj().T @ mc, False), (mp @ mc.conj().T, True)):
+        for n in range(m.dim - 1):
+            rm, cm = (n + 1, n) if up else (n, n + 1)    # a: |n><n+1|, a^dagger: |n+1><n|
+            av = complex(math.sqrt(n + 1.0), 0.0)
+            for r, c in zip(*np.nonzero(P)):
+                rows.append(rm * p.dim + r)
+                cols.append(cm * p.dim + c)
+                vals.append((av * P[r, c]) * f)
+                freqs.append(((0.0 + (m.exact[rm] - m.exact[cm])) + (p.exact[r] - p.exact[c])) if both else 0.0)
+    return Sparse((m.dim, p.dim), rows, cols, vals, freqs if both else None)
 
 
 def particle_two_modes(m0, m1, p, i0, i1, ip, u0=-1.0, u1=-1.0, phi=0.0, k=1, mf0="plus", mf1="plus"):
@@ -487,7 +532,7 @@ class _Tri(C.Structure):
 
 class _Coo(C.Structure):
     _fields_ = [("rank", C.c_int), ("axes", C.c_void_p), ("nnz", C.c_longlong), ("rows", C.c_void_p),
-                ("cols", C.c_void_p), ("vals", C.c_void_p)]
+                ("cols", C.c_void_p), ("vals", C.c_void_p), ("freqs", C.c_void_p)]
 
 
 class _Op(C.Structure):
@@ -521,7 +566,8 @@ def build_system(frees, interactions=(), binary=None):
         for t in terms:
             if isinstance(t, Sparse):
                 coos.append(_Coo(len(axes), arr(axes, np.int32), len(t.vals), arr(t.rows, np.int64),
-                                 arr(t.cols, np.int64), arr(t.vals.view(np.float64), np.float64)))
+                                 arr(t.cols, np.int64), arr(t.vals.view(np.float64), np.float64),
+                                 None if t.freqs is None else arr(t.freqs.view(np.float64), np.float64)))
                 continue
             if len(t.dims) != len(axes):
                 raise ValueError("term rank does not match its axes")

@@ -45,7 +45,9 @@ const char* qt_last_error(void);
  *   mode::make  CPPQEDelements/frees/Mode.tcc:40-60,   qbit::make  Qbit.cc:217-246,
  *   PumpedTwoLevelAtomSch  PumpedTwoLevelAtom.cc:35-49, (Pumped)Particle  Particle.cc:79-110, 210-252,
  *   jaynescummings::make  JaynesCummings_.h:85-90, CoupledModes  CoupledModes.cc:36-41,
- *   ParticleAlongCavity / ParticleOrthogonalToCavity  ParticleCavity.cc:26-110,
+ *   ParticleAlongCavity / ParticleOrthogonalToCavity  ParticleCavity.cc:26-140 (a pumped particle of
+ *   another wavenumber than the cavity's: the chained product term, refused with only the particle
+ *   in the interaction picture),
  *   ParticleTwoModes  ParticleTwoModes.cc:35-46 (equal wavenumbers); zz has no reference
  *   counterpart (Qbit_.h:68-70 declares sigmazop without defining it);
  *   binary::make  CPPQEDcore/composites/BinarySystem.cc:323-333, composite::make  Composite.h:363-367.
@@ -68,8 +70,10 @@ int qt_sys_destroy(qt_sys* sys);
  * holds Tridiagonal::getFreqs(): an entry's value at picture time tau is diag exp(freq tau)
  * (Tridiagonal::propagate, Tridiagonal.tcc:154-163); the main diagonal never rotates.
  * qt_coo is a general sparse operator on axes (an observable that is no Tridiagonal, e.g. the
- * particle's position operator): entry e is vals[e] |rows[e]><cols[e]| with row-major local
- * indices over the axes' dimensions. */
+ * particle's position operator, or a product of Tridiagonals such as ParticleAlongCavity's chained
+ * term, ParticleCavity.cc:113-140): entry e is vals[e] |rows[e]><cols[e]| with row-major local
+ * indices over the axes' dimensions, rotating as vals[e] exp(freqs[e] tau) when freqs (interleaved
+ * (re, im) [2 nnz]) is given, not at all when it is NULL. */
 typedef struct {
   int rank;
   const int* axes;              /* [rank] */
@@ -85,6 +89,7 @@ typedef struct {
   const long long* rows;        /* [nnz] */
   const long long* cols;        /* [nnz] */
   const double* vals;           /* [2 nnz] */
+  const double* freqs;          /* [2 nnz] or NULL */
 } qt_coo;
 
 /* an operator: the sum of its terms (Tridiagonals first, in order, then the sparse ones) */

@@ -332,6 +332,9 @@ struct Interaction {
   std::vector<int> axes;       // the V of Act<...> (BinarySystem: <0,1>)
   bool hasHa = false, timeDep = false;
   std::vector<Tridiag> hOverIs;
+  // chained terms (first, second): dpsidt += (1 (x) second) (first psi), second acting on the
+  // interaction's last axis (ParticleAlongCavity::addContribution_v, ParticleCavity.cc:113-140)
+  std::vector<std::pair<Tridiag, Tridiag>> chained;
   std::vector<std::pair<cd, double>> complexFreqs;
   std::vector<std::pair<double, double>> realFreqs;
   double highestFrequency() const {
@@ -666,7 +669,11 @@ Tridiag mfNKXgiven(const Free& fp, const std::string& mf, int nK) {
 //     interfericOneModeFN = etaeff tridiagPlusHC_overI(aop(mode) mfNKX(particle, (mf, kcav)));
 //   pumped particle (1p1mconf 3): etaeff = etaEff(u0, particle's vClass); with kcav = |kPart|
 //     (isSpecialH_) interfericTwoModeFN = etaeff tridiagPlusHC_overI(aop(mode)^dagger
-//     mfComposition(particle, (mf, kcav), (mfPart, kPart))); other wavenumbers are not restated.
+//     mfComposition(particle, (mf, kcav), (mfPart, kPart)));
+//   pumped particle of another wavenumber (ParticleCavity.cc:102-140): firstH = etaeff aop(mode)
+//     mfNKX(particle, (mf, kcav)) / i, firstHT = -firstH^dagger, secondH = mfNKX(particle,
+//     particle's mf)^dagger, secondHT = secondH^dagger, applied as secondH (firstH psi) +
+//     secondHT (firstHT psi), the second factor on the particle axis.
 // Frequencies RF{"Unot", u0, mode dim}, RF{"etaeff", etaeff, sqrt(mode dim)} (ParticleCavity.cc:62-66)
 Interaction makePcav(const System& s, int m, int p, const std::map<std::string, std::string>& kv) {
   Interaction ia; ia.kind = "pcav"; ia.axes = {m, p};
@@ -681,7 +688,7 @@ Interaction makePcav(const System& s, int m, int p, const std::map<std::string, 
   const double v = pumpedCtor ? fp.vClass : num(kv, "vclass", 0.);
   if (u0 * v < 0.) throw std::domain_error("Unot VClass sign discrepancy");
   const double etaeff = (u0 > 0 ? 1. : u0 < 0 ? -1. : 0.) * std::sqrt(u0 * v);
-  if (pumpedCtor && kcav != std::abs(fp.kPart)) throw std::invalid_argument("oracle: pcav with kcav != kPart");
+  const bool special = !pumpedCtor || kcav == std::abs(fp.kPart);
   double sgn = mfs == "sin" ? -1. : 1.;
   // dispersive(): only for a real mode function (ParticleCavity.cc:26-32)
   if (u0 != 0. && 2 * kcav < fp.dim && (mfs == "cos" || mfs == "sin")) {
@@ -690,11 +697,20 @@ Interaction makePcav(const System& s, int m, int p, const std::map<std::string, 
     ia.hOverIs.push_back(scaled(t, 1. / cd(0., 2.)));
     ia.hasHa = true; ia.timeDep = true;
   }
-  if (etaeff != 0.) {
+  if (etaeff != 0. && special) {
     Tridiag a = aop(fm.dim);
     if (fm.hasEx) furnishWithFreqs(a, fm.exd);
     Tridiag t = pumpedCtor ? product(hc(a), mfComposition(fp, mfs, fp.mf, kcav)) : product(a, mfNKXgiven(fp, mfs, kcav));
     ia.hOverIs.push_back(scaled(sum(t, hc(t)), cd(etaeff, 0.) / cd(0., 1.)));
+    ia.hasHa = true; ia.timeDep = true;
+  }
+  if (etaeff != 0. && !special) {
+    Tridiag a = aop(fm.dim);
+    if (fm.hasEx) furnishWithFreqs(a, fm.exd);
+    Tridiag first = scaled(product(a, mfNKXgiven(fp, mfs, kcav)), cd(etaeff, 0.) / cd(0., 1.));
+    Tridiag second = hc(mfNKXof(fp));
+    ia.chained.push_back({first, second});
+    ia.chained.push_back({negated(hc(first)), hc(second)});
     ia.hasHa = true; ia.timeDep = true;
   }
   ia.realFreqs.push_back({u0, (double)fm.dim});
@@ -823,6 +839,13 @@ void addContribution(System& s, double t, const cd* psi, cd* dpsidt, double t0) 
     if (!ia.hasHa) continue;
     forSlices(s.L, ia.axes, [&](long off, const std::vector<long>& vs) {
       for (auto& h : ia.hOverIs) { if (ia.timeDep) propagate(h, t - t0); apply(h, psi + off, dpsidt + off, vs); }
+      for (auto& c : ia.chained) {  // rank-2 interactions: the second factor on axis 1
+        if (ia.timeDep) { propagate(c.first, t - t0); propagate(c.second, t - t0); }
+        const int d0 = s.frees[ia.axes[0]].dim, d1 = s.frees[ia.axes[1]].dim;
+        std::vector<cd> tmp((size_t)d0 * d1, cd(0.));
+        apply(c.first, psi + off, tmp.data(), {(long)d1, 1L});
+        for (int r = 0; r < d0; ++r) apply(c.second, tmp.data() + (size_t)r * d1, dpsidt + off + r * vs[0], {vs[1]});
+      }
     });
   }
 }

@@ -137,3 +137,11 @@ def coherent(alpha, dim):
     n = np.arange(dim)
     amp = np.array([alpha ** k / sqrt(factorial(k)) for k in n], np.complex128)
     return np.exp(-abs(alpha) ** 2 / 2) * amp
+
+
+# ParticleAlongCavity with a pumped particle of another wavenumber than the cavity's (the chained
+# product term of ParticleCavity.cc:102-140): mode cutoff 3 (x) particle with 8 momentum states (in
+# the interaction picture its entries take 31 distinct frequencies, within the device's 32)
+PCAV_OTHER_K = ("free mode cutoff=3 delta=-1 kappa=1 eta=0.5\n"
+                "free particle fin=3 omrec=0.7 vclass=-2 kpart=1 mf=sin\nia pcav 0 1 u0=-1.5 kcav=2 mf=cos")
+PCAV_OTHER_K_IP = PCAV_OTHER_K.replace("eta=0.5", "eta=0.5 picture=IP").replace("mf=sin", "mf=sin picture=IP")

@@ -11,6 +11,14 @@ single-mode / qubit / particle operators, with the definitions the reference sta
                 tridiagMinusHC(conj(u) aop(m1) aop(m2).dagger()))
   dispersive    H = (u0/2) n (x) (+-cos(2 kcav x)), + for a cos, - for a sin mode function,
                 cos(Kx) = (e^{iKx} + e^{-iKx})/2 with e^{iKx}|k> = |k+K>         (ParticleCavity.cc:26-32)
+  pumped particle  H += (vClass/2) (+-cos(2 kPart x)), + for cos, - for sin (sin^2 = (1 - cos 2kx)/2
+                with the constant dropped; none for e^{+-ikx})                  (Particle.cc:84-87)
+  pumped particle along a cavity of another wavenumber (ParticleCavity.cc:102-140):
+                H += etaeff (a (x) f_p^dagger f_c + a^dagger (x) f_p f_c^dagger), etaeff = sign(u0)
+                sqrt(u0 vClass), f = cos(kx), sin(kx), e^{ikx}, e^{-ikx} as operators on the truncated
+                momentum grid, multiplied in the order the reference applies them (secondH after
+                firstH, secondHT after firstHT): at the grid's edges f_p f_c^dagger differs from
+                (f_p^dagger f_c)^dagger, since truncated shifts do not commute there
   zz            H = J sz (x) sz with sz = [s+, s] = diag(-1, +1)  (no reference element: qbit::sigmazop is
                 declared, never defined, Qbit_.h:68-70; this build defines the coupling so)
 and H_nH/i = -i (H - (i/2) sum_m J_m+ J_m).
@@ -76,6 +84,11 @@ def textbook(frees, ias):
         elif f["kind"] == "particle":
             k = np.arange(f["dim"]) - f["dim"] // 2
             H = H + _embed(dims, {i: sp.diags(f["omrec"] * k.astype(float) ** 2, 0, dtype=complex, format="csr")})
+            if f.get("vclass") and f["mf"] in ("sin", "cos"):  # |e^{+-ikx}|^2 = 1: no pump term
+                K = 2 * f["kpart"]
+                cos = 0.5 * (_shift(f["dim"], K) + _shift(f["dim"], K).getH())
+                sgn = -1.0 if f["mf"] == "sin" else 1.0
+                H = H + _embed(dims, {i: 0.5 * f["vclass"] * sgn * cos})
         else:
             raise ValueError(f["kind"])
     for kind, ax, p in ias:
@@ -96,10 +109,27 @@ def textbook(frees, ias):
             cos = 0.5 * (_shift(dims[q], K) + _shift(dims[q], K).getH())
             sgn = -1.0 if p["mf"] == "sin" else 1.0
             H = H + 0.5 * p["u0"] * sgn * _embed(dims, {m: n, q: cos})
+        elif kind == "pcav_other_k":
+            m, q = ax
+            fp = frees[q]
+            etaeff = np.sign(p["u0"]) * np.sqrt(p["u0"] * fp["vclass"])
+            fc, fpart = _mode_function(dims[q], p["mf"], p["kcav"]), _mode_function(dims[q], fp["mf"], fp["kpart"])
+            a = _a(dims[m])
+            H = H + etaeff * (_embed(dims, {m: a, q: fpart.getH() @ fc}) + _embed(dims, {m: a.getH(), q: fpart @ fc.getH()}))
         else:
             raise ValueError(kind)
     Hnh = H - 0.5j * sum((J.getH() @ J for J in Js), sp.csr_matrix((D, D), dtype=complex))
     return (-1j * Hnh).tocsr(), Js
+
+
+def _mode_function(n, mf, k):
+    """cos(kx), sin(kx), e^{ikx}, e^{-ikx} on the momentum grid (e^{ikx} shifts |q> to |q+k>, truncated)"""
+    e = _shift(n, k)
+    if mf == "cos":
+        return 0.5 * (e + e.getH())
+    if mf == "sin":
+        return (e - e.getH()) / 2j
+    return e if mf == "plus" else e.getH()
 
 
 def _table_sparse(sq, which):
@@ -250,3 +280,33 @@ def test_dispersive_mutation_is_caught():
     psi = np.random.default_rng(5).normal(size=32) + 0j
     so = O.System(spec.replace("mf=cos", "mf=sin"))
     assert np.abs(so.apply_heff(psi) - Hs @ psi).max() > 0.1
+
+
+# ---------------------------------------------------------------- pumped particle along a cavity of another wavenumber
+
+def _pumped_particle(fin, omrec, vclass, kpart, mf):
+    return {"kind": "particle", "dim": 1 << fin, "omrec": omrec, "vclass": vclass, "kpart": kpart, "mf": mf}
+
+
+@pytest.mark.parametrize("pmf,cmf,kpart,kcav", [("sin", "cos", 1, 2), ("cos", "sin", 2, 1), ("plus", "minus", 3, 1),
+                                                ("sin", "plus", 1, 3)])
+def test_pumped_particle_other_wavenumber_textbook(pmf, cmf, kpart, kcav):
+    spec = (f"free mode cutoff=3 delta=-1 kappa=1 eta=0.5\n"
+            f"free particle fin=4 omrec=0.7 vclass=-2 kpart={kpart} mf={pmf}\n"
+            f"ia pcav 0 1 u0=-1.5 kcav={kcav} mf={cmf}")
+    ias = [("pcav_other_k", (0, 1), {"u0": -1.5, "kcav": kcav, "mf": cmf})]
+    if cmf in ("cos", "sin"):
+        ias.append(("disp", (0, 1), {"u0": -1.5, "kcav": kcav, "mf": cmf}))
+    check_against_textbook(spec, [_mode(3, -1, 1, 0.5), _pumped_particle(4, 0.7, -2, kpart, pmf)], ias)
+
+
+def test_pumped_particle_other_wavenumber_mutation_is_caught():
+    """The particle's mode function enters conjugated (secondH = mfNKX(particle)^dagger,
+    ParticleCavity.cc:107): the oracle given the conjugate function (minus for plus) must not match."""
+    frees = [_mode(3, -1, 1, 0.5), _pumped_particle(4, 0.7, -2, 3, "plus")]
+    Hs, _ = textbook(frees, [("pcav_other_k", (0, 1), {"u0": -1.5, "kcav": 1, "mf": "cos"}),
+                             ("disp", (0, 1), {"u0": -1.5, "kcav": 1, "mf": "cos"})])
+    spec = ("free mode cutoff=3 delta=-1 kappa=1 eta=0.5\nfree particle fin=4 omrec=0.7 vclass=-2 kpart=3 mf=minus\n"
+            "ia pcav 0 1 u0=-1.5 kcav=1 mf=cos")
+    psi = np.random.default_rng(6).normal(size=48) + 0j
+    assert np.abs(O.System(spec).apply_heff(psi) - Hs @ psi).max() > 0.1

@@ -6,7 +6,8 @@ import pytest
 
 import cppqed_amd as Q
 import oracle_lib as O
-from sysdefs import P1M1, P1M1_CONF, RING, coherent, ground, p1m1_initial, ring_initial
+from sysdefs import (P1M1, P1M1_CONF, PCAV_OTHER_K, PCAV_OTHER_K_IP, RING, basis, coherent, ground, p1m1_initial,
+                     ring_initial)
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +53,9 @@ def test_apply_heff_interaction_picture(spec, prec, tol):
     # ParticleAlongCavity with the interference terms (1particle1mode --1p1mconf 2, 3)
     ("p1m1_conf2", P1M1_CONF[2], p1m1_initial(), 0.5, 16, 10.0),
     ("p1m1_conf3", P1M1_CONF[3], p1m1_initial(), 0.5, 16, 10.0),
+    # a pumped particle of another wavenumber (the chained product term), both pictures
+    ("pcav_other_k", PCAV_OTHER_K, basis(24, 4), 1.0, 16, 10.0),
+    ("pcav_other_k_ip", PCAV_OTHER_K_IP, basis(24, 4), 1.0, 16, 10.0),
 ])
 def test_ip_trajectories_match_oracle(name, spec, psi0, T, ntraj, ovt, layout):
     """With overshootTol = 1 dpLimit step-backs are frequent, and the redone step runs from the

@@ -26,6 +26,11 @@ SPECS = {
                       "ia pcav 0 1 u0=-1 kcav=1 mf=sin",
     "pcav_sch_conf2": "free mode cutoff=3 delta=-9.5 kappa=10 eta=10\nfree particle fin=3 omrec=1\n"
                       "ia pcav 0 1 u0=-1 kcav=1 mf=cos vclass=-10",
+    # a pumped particle of another wavenumber than the cavity's: the chained product term
+    "pcav_pumped_other_k": "free mode cutoff=3 delta=-1 kappa=1 eta=0.5\n"
+                           "free particle fin=4 omrec=0.7 vclass=-2 kpart=1 mf=sin\nia pcav 0 1 u0=-1.5 kcav=2 mf=cos",
+    "pcav_pumped_other_k_plus": "free mode cutoff=4 delta=0.5 kappa=0.5 eta=1\n"
+                                "free particle fin=4 omrec=0.3 vclass=1.2 kpart=3 mf=plus\nia pcav 0 1 u0=0.8 kcav=1 mf=minus",
     "pumped_particle_sch": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree particle fin=3 omrec=1 vclass=-10 kpart=1 mf=sin\n"
                            "ia pcav_orth 0 1 u0=-1",
     "bs_chain": "free mode cutoff=3 delta=-1 kappa=1 eta=1\nfree mode cutoff=4 delta=0.5 kappa=0.5 eta=0\n"
@@ -55,7 +60,13 @@ def test_tables_match_oracle(name):
     sq, so = Q.System(spec), O.System(spec)
     assert sq.dim == so.dim and sq.n_jumps == so.njumps and sq.n_avr == so.navr
     assert sq.highest_frequency == so.highest_frequency
-    np.testing.assert_array_equal(sq.dense(0), so.dense_heff())
+    if "other_k" in name:
+        # the chained term: the oracle applies firstH, then secondH to the result (as the reference);
+        # the table holds their product, summed in another order -- equal to a few ulp
+        H = so.dense_heff()
+        np.testing.assert_allclose(sq.dense(0), H, rtol=0, atol=8 * 2.2e-16 * np.abs(H).max())
+    else:
+        np.testing.assert_array_equal(sq.dense(0), so.dense_heff())
     rng = np.random.default_rng(3)
     psi = rng.normal(size=sq.dim) + 1j * rng.normal(size=sq.dim)
     psi /= np.linalg.norm(psi)
@@ -89,6 +100,9 @@ IP_SPECS = {
     "pcav_pumped_cos": "free mode cutoff=4 delta=-2 kappa=1 eta=2 picture=IP\n"
                        "free particle fin=4 omrec=0.5 vclass=3 kpart=2 mf=cos picture=IP\n"
                        "ia pcav 0 1 u0=2 kcav=2 mf=sin",
+    "pcav_pumped_other_k_ip": "free mode cutoff=3 delta=-1 kappa=1 eta=0.5 picture=IP\n"
+                              "free particle fin=4 omrec=0.7 vclass=-2 kpart=1 mf=sin picture=IP\n"
+                              "ia pcav 0 1 u0=-1.5 kcav=2 mf=cos",
     "pcav_plus_vclass": "free mode cutoff=4 delta=-2 kappa=1 eta=2 picture=IP\n"
                         "free particle fin=4 omrec=0.5 picture=IP\nia pcav 0 1 u0=2 kcav=1 mf=plus vclass=1.5",
     "ring": __import__("sysdefs").RING,
@@ -159,7 +173,11 @@ def test_jc_ell_width_and_nnz():
 
 def test_bad_specs_raise():
     for bad in ["", "free blob x=1", "free mode cutoff=3\nia jc 0 1 g=1", "free mode cutoff=3 picture=XY",
-                "free qbit deltaA=1 gamma=1 eta=1\nfree qbit deltaA=1 gamma=1 eta=1\nia jc 0 0 g=1"]:
+                "free qbit deltaA=1 gamma=1 eta=1\nfree qbit deltaA=1 gamma=1 eta=1\nia jc 0 0 g=1",
+                # the chained pcav term with only the particle in the interaction picture (its
+                # entries would rotate per intermediate momentum)
+                "free mode cutoff=3 delta=-1 kappa=1 eta=0.5\nfree particle fin=4 omrec=0.7 vclass=-2 kpart=1 "
+                "mf=sin picture=IP\nia pcav 0 1 u0=-1.5 kcav=2 mf=cos"]:
         with pytest.raises(Q.QtError):
             Q.System(bad)
 
