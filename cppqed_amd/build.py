@@ -17,10 +17,14 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC"]
 
-# heaviest first: the kernel templates are instantiated in several translation units, compiled in parallel
-SOURCES = [("qt_batch.hip", "hip"), ("qt_batch_td.hip", "hip"), ("qt_batch_gsl.hip", "hip"), ("qt_mixed.hip", "hip"),
-           ("qt_mixed_td.hip", "hip"), ("qt_mixed_gsl.hip", "hip"), ("qt_mixed_td_gsl.hip", "hip"), ("qt_aux.hip", "hip"), ("qt_plain.hip", "hip"),
-           ("qt_subgroup.hip", "hip"), ("qt_master.hip", "hip"), ("qt_master_sparse.hip", "hip"), ("qt_capi.cpp", "hip"), ("qt_system.cpp", "cxx")]
+# heaviest first: the kernel templates are instantiated in one translation unit per (picture,
+# controller, precision) table, compiled in parallel
+_TABLES = [f"qt_mixed_{v}_{p}.hip" for v in ("td_gsl", "gsl", "td", "plain") for p in ("f32", "f64")] + \
+          [f"qt_batch_{v}_{p}.hip" for v in ("gsl_td", "gsl", "td", "sch") for p in ("f32", "f64")]
+SOURCES = [(f, "hip") for f in _TABLES] + \
+          [("qt_plain.hip", "hip"), ("qt_master.hip", "hip"), ("qt_subgroup.hip", "hip"), ("qt_capi.cpp", "hip"),
+           ("qt_aux.hip", "hip"), ("qt_batch.hip", "hip"), ("qt_mixed.hip", "hip"), ("qt_master_sparse.hip", "hip"),
+           ("qt_system.cpp", "cxx")]
 
 
 def _run(cmd):

@@ -1,0 +1,11 @@
+// qt_batch_gsl_td_f32.hip -- launch table of the batched MCWF kernel instances:
+// interaction picture capable, GSL's controller, FP32 derivatives.
+// One unit per (picture, controller, precision), so that the instantiations compile in parallel.
+#define QT_TBL_F32 true
+#define QT_TBL_TD true
+#define QT_TBL_GSL true
+#include "qt_engine.hpp"
+
+namespace qt {
+QT_BATCH_TABLE(batch_gsl_td_f32, QT_BATCH_GSL)
+}  // namespace qt

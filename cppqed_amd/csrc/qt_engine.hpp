@@ -1428,50 +1428,43 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   X(1, 1, 4, 1) X(1, 1, 8, 1) X(1, 1, 16, 1) X(2, 1, 8, 1) X(2, 1, 16, 1) X(4, 1, 16, 1) \
   X(1, 4, 16, 1) X(2, 4, 16, 1) X(4, 4, 16, 1) X(4, 8, 16, 1) X(2, 16, 16, 1) X(4, 16, 16, 1)
 
-// One launch table over a list of batched variants: NAME##_blocks (resident work groups per
-// compute unit, registers and LDS permitting) and NAME##_launch.  Each list is instantiated in
-// one translation unit.
-#define QT_BATCH_TABLE(NAME, LIST, TDV, GV)                                                                       \
-  int NAME##_blocks(int rows, int nw, int wh, int minw, bool f32) {                                               \
-    int n = 0;                                                                                                    \
-    QT_TABLE_CASES_B(LIST, TDV, GV)                                                                               \
-    return 0;                                                                                                     \
-  }                                                                                                               \
-  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E, \
-                           hipStream_t st) {                                                                      \
-    QT_TABLE_CASES_L(LIST, TDV, GV)                                                                               \
-    return hipErrorInvalidValue;                                                                                  \
+// One launch table over a list of batched variants for one precision: NAME##_blocks (resident work
+// groups per compute unit, registers and LDS permitting) and NAME##_launch.  The translation unit
+// that invokes it defines QT_TBL_F32, QT_TBL_TD and QT_TBL_GSL (the kernels' F32, TD and GSL
+// arguments); each (picture, controller, precision) table has a unit of its own, so that the
+// heavy instantiations compile in parallel.
+#define QT_BATCH_TABLE(NAME, LIST)                                                                                 \
+  int NAME##_blocks(int rows, int nw, int wh, int minw) {                                                          \
+    int n = 0;                                                                                                     \
+    LIST(QT_TABLE_B)                                                                                               \
+    return 0;                                                                                                      \
+  }                                                                                                                \
+  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, int grid, const SysDev& S, const EnsDev& E,         \
+                           hipStream_t st) {                                                                       \
+    LIST(QT_TABLE_L)                                                                                               \
+    return hipErrorInvalidValue;                                                                                   \
   }
-#define QT_TABLE_CASES_B(LIST, TDV, GV) LIST(QT_TABLE_B_##TDV##_##GV)
-#define QT_TABLE_CASES_L(LIST, TDV, GV) LIST(QT_TABLE_L_##TDV##_##GV)
-#define QT_TABLE_B(R, N, W, M, TDV, GV)                                                                           \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                             \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, true, M, TDV, GV>, 64 * N, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_batch_kernel<R, N, W, false, M, TDV, GV>, 64 * N, 0);    \
-    return n;                                                                                                     \
+#define QT_TABLE_B(R, N, W, M)                                                                                     \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                              \
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                            \
+        &n, mcwf_batch_kernel<R, N, W, QT_TBL_F32, M, QT_TBL_TD, QT_TBL_GSL>, 64 * N, 0);                          \
+    return n;                                                                                                      \
   }
-#define QT_TABLE_L(R, N, W, M, TDV, GV)                                                                           \
-  if (rows == R && nw == N && wh == W && minw == M) {                                                             \
-    if (f32) hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, true, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E); \
-    else hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, false, M, TDV, GV>), dim3(grid), dim3(64 * N), 0, st, S, E);    \
-    return hipGetLastError();                                                                                     \
+#define QT_TABLE_L(R, N, W, M)                                                                                     \
+  if (rows == R && nw == N && wh == W && minw == M) {                                                              \
+    hipLaunchKernelGGL((mcwf_batch_kernel<R, N, W, QT_TBL_F32, M, QT_TBL_TD, QT_TBL_GSL>), dim3(grid), dim3(64 * N), \
+                       0, st, S, E);                                                                               \
+    return hipGetLastError();                                                                                      \
   }
-#define QT_TABLE_B_false_false(R, N, W, M) QT_TABLE_B(R, N, W, M, false, false)
-#define QT_TABLE_B_true_false(R, N, W, M) QT_TABLE_B(R, N, W, M, true, false)
-#define QT_TABLE_B_false_true(R, N, W, M) QT_TABLE_B(R, N, W, M, false, true)
-#define QT_TABLE_B_true_true(R, N, W, M) QT_TABLE_B(R, N, W, M, true, true)
-#define QT_TABLE_L_false_false(R, N, W, M) QT_TABLE_L(R, N, W, M, false, false)
-#define QT_TABLE_L_true_false(R, N, W, M) QT_TABLE_L(R, N, W, M, true, false)
-#define QT_TABLE_L_false_true(R, N, W, M) QT_TABLE_L(R, N, W, M, false, true)
-#define QT_TABLE_L_true_true(R, N, W, M) QT_TABLE_L(R, N, W, M, true, true)
 
 #define QT_TABLE_DECL(NAME)                                                                                        \
-  int NAME##_blocks(int rows, int nw, int wh, int minw, bool f32);                                                 \
-  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, bool f32, int grid, const SysDev& S, const EnsDev& E, \
+  int NAME##_blocks(int rows, int nw, int wh, int minw);                                                           \
+  hipError_t NAME##_launch(int rows, int nw, int wh, int minw, int grid, const SysDev& S, const EnsDev& E,         \
                            hipStream_t st);
-QT_TABLE_DECL(batch_sch)
-QT_TABLE_DECL(batch_td)
-QT_TABLE_DECL(batch_gsl)
-QT_TABLE_DECL(batch_gsl_td)
+#define QT_TABLE_DECL2(NAME) QT_TABLE_DECL(NAME##_f32) QT_TABLE_DECL(NAME##_f64)
+QT_TABLE_DECL2(batch_sch)
+QT_TABLE_DECL2(batch_td)
+QT_TABLE_DECL2(batch_gsl)
+QT_TABLE_DECL2(batch_gsl_td)
 
 }  // namespace qt

@@ -1,11 +1,9 @@
-// qt_mixed.hip -- launch table of the mixed-batch kernel (several ensembles in one launch).
-// The interaction-picture and GSL-capable instances are compiled in their own translation
-// units (qt_mixed_td.hip, qt_mixed_gsl.hip, qt_mixed_td_gsl.hip) by the same table macro.
+// qt_mixed.hip -- the dispatch over the mixed-batch kernel's instances (several ensembles in one
+// launch); the launch tables are compiled in their own units
+// (qt_mixed_{plain,td,gsl,td_gsl}_{f32,f64}.hip) by one table macro.
 #include "qt_mixed.hpp"
 
 namespace qt {
-
-QT_MIXED_TABLE(plain, false, false)
 
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw) {
 #define QT_CASE(A, B, C, D, F) if (nwb == A && rb == B && whb == C && rw == D && whw == F) return true;
@@ -25,18 +23,23 @@ int mixed_variants(int* table, int cap) {
 }
 
 int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl) {
-  if (td && gsl) return mixed_blocks_td_gsl(nwb, rb, whb, rw, whw, f32);
-  if (td) return mixed_blocks_td(nwb, rb, whb, rw, whw, f32);
-  if (gsl) return mixed_blocks_gsl(nwb, rb, whb, rw, whw, f32);
-  return mixed_blocks_plain(nwb, rb, whb, rw, whw, f32);
+  if (td && gsl) return f32 ? mixed_blocks_td_gsl_f32(nwb, rb, whb, rw, whw) : mixed_blocks_td_gsl_f64(nwb, rb, whb, rw, whw);
+  if (td) return f32 ? mixed_blocks_td_f32(nwb, rb, whb, rw, whw) : mixed_blocks_td_f64(nwb, rb, whb, rw, whw);
+  if (gsl) return f32 ? mixed_blocks_gsl_f32(nwb, rb, whb, rw, whw) : mixed_blocks_gsl_f64(nwb, rb, whb, rw, whw);
+  return f32 ? mixed_blocks_plain_f32(nwb, rb, whb, rw, whw) : mixed_blocks_plain_f64(nwb, rb, whb, rw, whw);
 }
 
 hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
                         const MixDev* M, hipStream_t st) {
-  if (td && gsl) return launch_mixed_td_gsl(nwb, rb, whb, rw, whw, f32, grid, M, st);
-  if (td) return launch_mixed_td(nwb, rb, whb, rw, whw, f32, grid, M, st);
-  if (gsl) return launch_mixed_gsl(nwb, rb, whb, rw, whw, f32, grid, M, st);
-  return launch_mixed_plain(nwb, rb, whb, rw, whw, f32, grid, M, st);
+  if (td && gsl)
+    return f32 ? launch_mixed_td_gsl_f32(nwb, rb, whb, rw, whw, grid, M, st)
+               : launch_mixed_td_gsl_f64(nwb, rb, whb, rw, whw, grid, M, st);
+  if (td)
+    return f32 ? launch_mixed_td_f32(nwb, rb, whb, rw, whw, grid, M, st) : launch_mixed_td_f64(nwb, rb, whb, rw, whw, grid, M, st);
+  if (gsl)
+    return f32 ? launch_mixed_gsl_f32(nwb, rb, whb, rw, whw, grid, M, st) : launch_mixed_gsl_f64(nwb, rb, whb, rw, whw, grid, M, st);
+  return f32 ? launch_mixed_plain_f32(nwb, rb, whb, rw, whw, grid, M, st)
+             : launch_mixed_plain_f64(nwb, rb, whb, rw, whw, grid, M, st);
 }
 
 }  // namespace qt

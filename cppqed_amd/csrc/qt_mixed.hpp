@@ -10,46 +10,41 @@ namespace qt {
   X(16, 4, 16, 1, 8) X(16, 4, 16, 2, 8) X(16, 2, 16, 1, 8) X(16, 1, 16, 1, 8)
 
 // resident work groups per compute unit and the launch of every variant, for one choice of
-// (interaction-picture capable, GSL capable) instances
-#define QT_MIXED_TABLE(NAME, TDC, GSLC)                                                                         \
-  int mixed_blocks_##NAME(int nwb, int rb, int whb, int rw, int whw, bool f32) {                               \
+// (interaction-picture capable, GSL capable, precision) instances: the translation unit that
+// invokes it defines QT_TBL_F32, QT_TBL_TD and QT_TBL_GSL (each choice has a unit of its own, so
+// that the heavy instantiations compile in parallel)
+#define QT_MIXED_TABLE(NAME)                                                                                    \
+  int mixed_blocks_##NAME(int nwb, int rb, int whb, int rw, int whw) {                                          \
     int n = 0;                                                                                                  \
-    QT_MIXED(QT_MIXED_B_##TDC##_##GSLC)                                                                         \
+    QT_MIXED(QT_MIXED_B)                                                                                        \
     return 0;                                                                                                   \
   }                                                                                                             \
-  hipError_t launch_mixed_##NAME(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, \
+  hipError_t launch_mixed_##NAME(int nwb, int rb, int whb, int rw, int whw, int grid, const MixDev* M,          \
                                  hipStream_t st) {                                                              \
-    QT_MIXED(QT_MIXED_L_##TDC##_##GSLC)                                                                         \
+    QT_MIXED(QT_MIXED_L)                                                                                        \
     return hipErrorInvalidValue;                                                                                \
   }
-#define QT_MIXED_B(A, B, C, D, F, TDC, GSLC)                                                                    \
+#define QT_MIXED_B(A, B, C, D, F)                                                                               \
   if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                 \
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, true, TDC, GSLC>, 64 * A, 0); \
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mcwf_mixed_kernel<A, B, C, D, F, false, TDC, GSLC>, 64 * A, 0);   \
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(                                                         \
+        &n, mcwf_mixed_kernel<A, B, C, D, F, QT_TBL_F32, QT_TBL_TD, QT_TBL_GSL>, 64 * A, 0);                    \
     return n;                                                                                                   \
   }
-#define QT_MIXED_L(A, B, C, D, F, TDC, GSLC)                                                                    \
+#define QT_MIXED_L(A, B, C, D, F)                                                                               \
   if (nwb == A && rb == B && whb == C && rw == D && whw == F) {                                                 \
-    if (f32) hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, true, TDC, GSLC>), dim3(grid), dim3(64 * A), 0, st, M); \
-    else hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, false, TDC, GSLC>), dim3(grid), dim3(64 * A), 0, st, M);    \
+    hipLaunchKernelGGL((mcwf_mixed_kernel<A, B, C, D, F, QT_TBL_F32, QT_TBL_TD, QT_TBL_GSL>), dim3(grid),       \
+                       dim3(64 * A), 0, st, M);                                                                 \
     return hipGetLastError();                                                                                   \
   }
-#define QT_MIXED_B_false_false(A, B, C, D, F) QT_MIXED_B(A, B, C, D, F, false, false)
-#define QT_MIXED_B_true_false(A, B, C, D, F) QT_MIXED_B(A, B, C, D, F, true, false)
-#define QT_MIXED_B_false_true(A, B, C, D, F) QT_MIXED_B(A, B, C, D, F, false, true)
-#define QT_MIXED_B_true_true(A, B, C, D, F) QT_MIXED_B(A, B, C, D, F, true, true)
-#define QT_MIXED_L_false_false(A, B, C, D, F) QT_MIXED_L(A, B, C, D, F, false, false)
-#define QT_MIXED_L_true_false(A, B, C, D, F) QT_MIXED_L(A, B, C, D, F, true, false)
-#define QT_MIXED_L_false_true(A, B, C, D, F) QT_MIXED_L(A, B, C, D, F, false, true)
-#define QT_MIXED_L_true_true(A, B, C, D, F) QT_MIXED_L(A, B, C, D, F, true, true)
 
 #define QT_MIXED_DECL(NAME)                                                                                     \
-  int mixed_blocks_##NAME(int nwb, int rb, int whb, int rw, int whw, bool f32);                                 \
-  hipError_t launch_mixed_##NAME(int nwb, int rb, int whb, int rw, int whw, bool f32, int grid, const MixDev* M, \
+  int mixed_blocks_##NAME(int nwb, int rb, int whb, int rw, int whw);                                           \
+  hipError_t launch_mixed_##NAME(int nwb, int rb, int whb, int rw, int whw, int grid, const MixDev* M,          \
                                  hipStream_t st);
-QT_MIXED_DECL(plain)
-QT_MIXED_DECL(td)
-QT_MIXED_DECL(gsl)
-QT_MIXED_DECL(td_gsl)
+#define QT_MIXED_DECL2(NAME) QT_MIXED_DECL(NAME##_f32) QT_MIXED_DECL(NAME##_f64)
+QT_MIXED_DECL2(plain)
+QT_MIXED_DECL2(td)
+QT_MIXED_DECL2(gsl)
+QT_MIXED_DECL2(td_gsl)
 
 }  // namespace qt

@@ -1,0 +1,11 @@
+// qt_mixed_td_gsl_f32.hip -- launch table of the mixed-batch kernel instances:
+// interaction picture capable, GSL's controller, FP32 derivatives.
+// One unit per (picture, controller, precision), so that the instantiations compile in parallel.
+#define QT_TBL_F32 true
+#define QT_TBL_TD true
+#define QT_TBL_GSL true
+#include "qt_mixed.hpp"
+
+namespace qt {
+QT_MIXED_TABLE(td_gsl_f32)
+}  // namespace qt
