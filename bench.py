@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-chunk", type=int, default=20,
+                    help="stream intervals a request advances per loop step (20: a whole T=2 request at Dt=0.1)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the copy of known size that calibrates FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
@@ -361,7 +363,7 @@ def serve(Q, args):
     jc80 = [Q.System(jc_master_spec(40, eta=float(e), g=10.0)) for e in np.linspace(0.5, 15, 8)]
     rho80 = np.zeros((80, 80), complex)
     rho80[0, 0] = 1
-    srv = Q.Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_active=16, chunk_intervals=10,
+    srv = Q.Server(hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_active=16, chunk_intervals=args.serve_chunk,
                    pars=Q.mcwf_pars(precision=Q.QT_FP32_DERIV))
     t_start = time.perf_counter()
     t_next = t_start + rng.exponential(1.0 / args.serve_rate)
@@ -415,7 +417,7 @@ def serve(Q, args):
     return {"workload": f"Poisson arrivals at {args.serve_rate}/s for {args.serve_seconds} s; ensembles of 256-4096 "
                         "trajectories (log-uniform) of PLM-20, JC-60, particle-cavity 1024, four modes 4096, six qubits 64 "
                         "or JC-80 (eta in [0.5, 15]), or Master-equation sweeps of 4-16 JC-80 points; T=2, Dt=0.1; "
-                        "FP32 derivatives; up to 16 requests per step, "
+                        f"FP32 derivatives; up to 16 requests per step, {args.serve_chunk} intervals per step, "
                         f"{args.serve_budget_gib} GiB admission budget",
             "requests": int(len(lat)), "submitted": submitted, "seconds": el, "tries": tries, "tries_per_s": tries / el,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),

@@ -1472,8 +1472,20 @@ struct qt_batch_s {
       if (!assign_teams(V[v], mem, req, team, policy)) throw Fail(QT_ENOTSUP, "the requested layout does not fit every member");
       variant = v;
     } else {
+      // the fitting variant that occupies the fewest wavefronts per launch: sum over members of
+      // trajectories x wavefronts of their team (a dim-80 member takes one wavefront with two rows
+      // per lane, against a four-wavefront quarter team with one row per lane); ties: the lowest
+      // index
       int found = -1;
-      for (size_t i = 0; i < V.size() && found < 0; ++i) if (assign_teams(V[i], mem, req, team, policy)) found = (int)i;
+      double best = 0.;
+      std::vector<int> t;
+      for (size_t i = 0; i < V.size(); ++i) {
+        if (!assign_teams(V[i], mem, req, t, policy)) continue;
+        double cost = 0.;
+        for (size_t m = 0; m < mem.size(); ++m)
+          cost += (double)mem[m]->n * (t[m] == 1 ? 1 : t[m] == 2 ? V[i].nwb / 4 : V[i].nwb);
+        if (found < 0 || cost < best) { found = (int)i; best = cost; team = t; }
+      }
       if (found < 0) throw Fail(QT_ENOTSUP, "no compiled mixed-batch layout fits every member");
       variant = found;
     }
