@@ -89,16 +89,33 @@ __device__ __forceinline__ void team_sync() {
   else __syncthreads();
 }
 
-// Sum / max over the 64 lanes of a wavefront; every lane gets the bitwise-identical result
-// (each butterfly step adds two equal-order partial sums, addition being commutative).
-// All 64 lanes must be active.
+// Row broadcasts across the 16-lane rows (GFX9 DPP): row_bcast:15 (0x142) hands lane 15 of each
+// row to the next row, row_bcast:31 (0x143) hands lane 31 to rows 2 and 3.  Only the rows in
+// ROWMASK are written; the others get 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_bcast(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum / max over the 64 lanes of a wavefront; every lane gets the bitwise-identical result.
+// Four butterfly steps leave every lane of row i with the row's sum r_i (each step adds two
+// equal-order partial sums, addition being commutative); two row broadcasts then form
+// (r3 + r2) + (r1 + r0) in lane 63, which is read out as a wavefront-uniform value (the same bits
+// as (r0 + r1) + (r2 + r3), again by commutativity).  All 64 lanes must be active.
 __device__ __forceinline__ double wave_sum(double v) {
   v += dpp<0xB1>(v); v += dpp<0x4E>(v); v += dpp<0x141>(v); v += dpp<0x140>(v);
-  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+  v += dpp_bcast<0x142, 0xA>(v);
+  v += dpp_bcast<0x143, 0xC>(v);
+  return readlane(v, 63);
 }
 __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp<0xB1>(v)); v = fmax(v, dpp<0x4E>(v)); v = fmax(v, dpp<0x141>(v)); v = fmax(v, dpp<0x140>(v));
-  return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+  // rows outside the mask receive 0 and are not read
+  v = fmax(v, dpp_bcast<0x142, 0xA>(v));
+  v = fmax(v, dpp_bcast<0x143, 0xC>(v));
+  return readlane(v, 63);
 }
 
 // three sums over the 64 lanes at once (interleaved butterflies: one latency for all three)
@@ -112,7 +129,11 @@ __device__ __forceinline__ void wave_sum3(double v[3]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) v[i] += dpp<0x140>(v[i]);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) v[i] = (readlane(v[i], 0) + readlane(v[i], 16)) + (readlane(v[i], 32) + readlane(v[i], 48));
+  for (int i = 0; i < 3; ++i) v[i] += dpp_bcast<0x142, 0xA>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] += dpp_bcast<0x143, 0xC>(v[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = readlane(v[i], 63);
 }
 
 template <int NW> struct GroupReduce {
@@ -249,8 +270,10 @@ __device__ __forceinline__ void acc_add(unsigned long long* p, double v, double 
   const double r = v - h * 0x1p-20;  // exact: h 2^-20 is a double within 2^-20 of v
   const unsigned long long H = (unsigned long long)(long long)h;
   const unsigned long long L = (unsigned long long)rint(r * 0x1p60);
-  __hip_atomic_fetch_add(p, H, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_fetch_add(p + 1, L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // through the global path (the sums live in device memory), not the flat one
+  __attribute__((address_space(1))) unsigned long long* g = (__attribute__((address_space(1))) unsigned long long*)p;
+  __hip_atomic_fetch_add(g, H, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(g + 1, L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // the mean of n values accumulated by acc_add (sums of H and of L, two's complement)

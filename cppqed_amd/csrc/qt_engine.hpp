@@ -73,6 +73,10 @@ struct Engine {
   const T* dictL = nullptr;
   const unsigned char* idxL = nullptr;
   const int* dbL = nullptr;
+  // the slots' column offsets (LDS copy of SysDev::hoff) and whether the slot-major gather applies:
+  // the state fills the team (dim == ROWS * G) and no |offset| exceeds G (gatherDictC)
+  const int* doffL = nullptr;
+  bool dictC = false;
   // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
   // launch's error flag (set when a wait times out)
   unsigned* bar = nullptr;
@@ -132,10 +136,11 @@ struct Engine {
   // Copies the system's coefficient dictionary into the team's LDS buffers -- dict [kMaxDict],
   // base [kMaxDictSlots], idx [idxCap bytes] -- and switches the gathers to it; ends with a team
   // barrier.  Without a dictionary, when its index does not fit idxCap, or for an engine without
-  // the offset form, nothing changes.
-  __device__ __forceinline__ void loadDict(T* dict, int* base, unsigned char* idx, int idxCap) {
+  // the offset form, nothing changes.  off [kMaxDictSlots] (may be null): LDS for the slots'
+  // column offsets, which the slot-major gather needs.
+  __device__ __forceinline__ void loadDict(T* dict, int* base, unsigned char* idx, int idxCap, int* off = nullptr) {
     if constexpr (kRegs || !(OPT & kOffsets)) {
-      (void)dict; (void)base; (void)idx; (void)idxCap;
+      (void)dict; (void)base; (void)idx; (void)idxCap; (void)off;
     } else {
       const int n = uniform(S.ndict);
       if (!hoffU || n <= 0 || n > kMaxDict || whU * dimU > idxCap) return;
@@ -150,6 +155,13 @@ struct Engine {
       unsigned* idst = reinterpret_cast<unsigned*>(idx);
       for (int i = lane; i < whU * dimU / 4; i += G) idst[i] = isrc[i];
       dictL = dict; dbL = base; idxL = idx;
+      if (off) {
+        int mx = 0;
+        for (int w = 0; w < whU; ++w) mx = max(mx, abs(gbl(hoffU)[w]));
+        if (lane < whU) off[lane] = gbl(hoffU)[lane];
+        doffL = off;
+        dictC = mx <= G && dimU == ROWS * G;
+      }
       sync();
     }
   }
@@ -220,9 +232,45 @@ struct Engine {
     }
   }
 
+  // Dictionary form when the state fills the team and no |offset| exceeds G, slot-major: a slot's
+  // offset, part of the dictionary and row of index bytes are set up once for all of the lane's
+  // rows, and the column row + offset needs the clamp of the general path only in the first row
+  // block (it can fall below 0) and the last (beyond dim - 1); in between it stays inside the
+  // state.  Each row still sums its slots in order with the same operands, so k is bitwise the
+  // general path's.
+  __device__ __forceinline__ void gatherDictC(T k[ROWS]) const {
+    T acc[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r] = from_d<T>(make_double2(0., 0.));
+    const int wh = whU, dim = dimU;
+#pragma unroll 3
+    for (int w = 0; w < wh; ++w) {
+      const int off = doffL[w];
+      const T* dw = dictL + dbL[w];
+      const unsigned char* iw = idxL + (unsigned)(w * dim + lane);
+      const T* xw = xs + (lane + off);
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        T xv;
+        if (r == 0 || r == ROWS - 1) {
+          int c = r * G + lane + off;
+          if (r == 0) c = max(c, 0);
+          if (r == ROWS - 1) c = min(c, dim - 1);
+          xv = xs[c];
+        } else {
+          xv = xw[r * G];
+        }
+        acc[r] = cfma(dw[iw[r * G]], xv, acc[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) k[r] = acc[r];
+  }
+
   // k = (H_nH/i) v, with v already staged in xs
   __device__ __forceinline__ void gather(T k[ROWS]) const {
     if constexpr (TD) if (td) { gatherTD(k); return; }
+    if constexpr (!kRegs && (OPT & kOffsets) != 0) if (dictC) { gatherDictC(k); return; }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       T acc = from_d<T>(make_double2(0., 0.));
@@ -1183,9 +1231,9 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     LdsEngine<ROWS, NW, WH, F32, false, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
                                                                                  kTwo ? xs2 : nullptr);
     __shared__ T dict[kDict ? kMaxDict : 1];
-    __shared__ int dbase[kMaxDictSlots];
+    __shared__ int dbase[kMaxDictSlots], doff[kMaxDictSlots];
     __shared__ __align__(16) unsigned char didx[kDict ? kIdx : 4];
-    if constexpr (kDict) eng.loadDict(dict, dbase, didx, (int)kIdx);
+    if constexpr (kDict) eng.loadDict(dict, dbase, didx, (int)kIdx, doff);
     for (int traj; (traj = claim()) < E.n_traj;) advance_traj_lds<decltype(eng), GSL>(S, E, eng, traj, rts, cnt);
   } else {
     // one wavefront: psi in registers; the RK stage vector (derivative precision) and the FP64
@@ -1291,7 +1339,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   __shared__ MixDev M;
   __shared__ __align__(16) unsigned char pool[kBytes];
   __shared__ TB dictS[kDictB ? kMaxDict : 1];
-  __shared__ int dbaseS[kMaxDictSlots];
+  __shared__ int dbaseS[kMaxDictSlots], doffS[kMaxDictSlots];
   __shared__ __align__(16) unsigned char didxS[kDictB ? kIdxB : 4];
   __shared__ double red[NWB * kMaxJumps];
   __shared__ double rtsB[kMaxJumps];
@@ -1332,7 +1380,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       if (item >= M.items[0]) break;
       const int m = member(0, item);
       EngB eng(M.S[m], xs, X, red, (int)threadIdx.x, TDC ? phB : nullptr);
-      if constexpr (kDictB) eng.loadDict(dictS, dbaseS, didxS, (int)kIdxB);
+      if constexpr (kDictB) eng.loadDict(dictS, dbaseS, didxS, (int)kIdxB, doffS);
       advance_member_lds<GSLC>(M.S[m], M.E[m], eng, item - M.first[m], rtsB, cntB);
     }
     __syncthreads();
