@@ -32,9 +32,9 @@ hipError_t launch_batch(int rows, int nw, int wh, int minw, bool f32, bool td, b
 hipError_t launch_jump_draws(const unsigned long long* ordinal, unsigned long long seed, unsigned long long traj0,
                              int n_traj, int cap, double* draws, hipStream_t st, unsigned long long* base);
 bool mixed_supported(int nwb, int rb, int whb, int rw, int whw);
-int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl);
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, int ph, bool f32, bool td, bool gsl);
 int mixed_variants(int* table, int cap);
-hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, int ph, bool f32, bool td, bool gsl, int grid,
                         const MixDev* M, hipStream_t st);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t st);
 hipError_t launch_ensemble_rho(const double2* psi, int n, int d, int splits, double2* part, double2* rho,
@@ -1450,15 +1450,22 @@ struct qt_batch_s {
   DevBuf<qt::MixDev> mix;
   DevBuf<int> queue;            // [3] queue heads + [1] barrier error flag
   qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
-  int grid = 0;
+  // the launch is two kernels: work-group teams on the caller's stream (grid1), quarter and
+  // wavefront teams on a stream of the batch's own (grid6), joined by events
+  int grid1 = 0, grid6 = 0;
   int cus = 0;                  // compute units of the device (cached)
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   double kernelMs = 0.;
   int launches = 0;
   ~qt_batch_s() {
     if (host) (void)hipHostFree(host);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
   }
   void choose(int v, const int* req, int policy = 0) {
     td = gsl = false;
@@ -1497,8 +1504,8 @@ struct qt_batch_s {
       hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
       cus = prop.multiProcessorCount;
     }
-    int per = qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, f32, td, gsl);
-    grid = cus * std::max(1, per);
+    grid1 = cus * std::max(1, qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, 1, f32, td, gsl));
+    grid6 = cus * std::max(1, qt::mixed_blocks_per_cu(L.nwb, L.rb, L.whb, L.rw, L.whw, 6, f32, td, gsl));
   }
 };
 
@@ -1572,11 +1579,31 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   for (int k = 0; k < 3; ++k) M.items[k] = items[k];
   hipCheck(hipMemcpyAsync(b->mix.p, &M, sizeof(M), hipMemcpyHostToDevice, st), "batch descriptor H2D");
   hipCheck(qt::launch_fill_int(b->queue.p, 4, 0, st), "reset queues");
-  // every resident work group, also when the work would fit fewer: wavefront and quarter
-  // items then spread over all compute units instead of filling a few
-  const int grid = std::max(1, b->grid);
+  // every resident work group of each kernel, also when the work would fit fewer: wavefront and
+  // quarter items then spread over all compute units instead of filling a few.  The work-group
+  // teams go first on st; the smaller teams run beside them on the batch's own stream (default
+  // priority) and take compute units as the long-running work-group items drain.
   hipCheck(hipEventRecord(b->e0, st), "hipEventRecord");
-  hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, b->f32, b->td, b->gsl, grid, b->mix.p, st), "mixed launch");
+  const bool big = items[0] > 0, small = items[1] + items[2] > 0;
+  if (small && big) {  // the side stream starts once the descriptor and the queues are in place
+    if (!b->side) hipCheck(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    if (!b->fork) hipCheck(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    if (!b->join) hipCheck(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
+    hipCheck(hipEventRecord(b->fork, st), "hipEventRecord");
+  }
+  if (big)
+    hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, 1, b->f32, b->td, b->gsl, std::max(1, b->grid1), b->mix.p,
+                              st), "mixed launch (work-group teams)");
+  if (small && big) {
+    hipCheck(hipStreamWaitEvent(b->side, b->fork, 0), "hipStreamWaitEvent");
+    hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, 6, b->f32, b->td, b->gsl, std::max(1, b->grid6), b->mix.p,
+                              b->side), "mixed launch (quarter and wavefront teams)");
+    hipCheck(hipEventRecord(b->join, b->side), "hipEventRecord");
+    hipCheck(hipStreamWaitEvent(st, b->join, 0), "hipStreamWaitEvent");
+  } else if (small) {
+    hipCheck(qt::launch_mixed(L.nwb, L.rb, L.whb, L.rw, L.whw, 6, b->f32, b->td, b->gsl, std::max(1, b->grid6), b->mix.p,
+                              st), "mixed launch (quarter and wavefront teams)");
+  }
   hipCheck(hipEventRecord(b->e1, st), "hipEventRecord");
   for (size_t m = 0; m < b->mem.size(); ++m) {
     b->mem[m]->lastIntervals = n_int[m];

@@ -73,10 +73,11 @@ struct Engine {
   const T* dictL = nullptr;
   const unsigned char* idxL = nullptr;
   const int* dbL = nullptr;
-  // the slots' column offsets (LDS copy of SysDev::hoff) and whether the slot-major gather applies:
-  // the state fills the team (dim == ROWS * G) and no |offset| exceeds G (gatherDictC)
+  // the slots' column offsets (LDS copy of SysDev::hoff, with a dictionary) and whether the
+  // slot-major gather applies: constant offsets, the state fills the team (dim == ROWS * G) and
+  // no |offset| exceeds G (gatherC)
   const int* doffL = nullptr;
-  bool dictC = false;
+  bool offC = false;
   // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
   // launch's error flag (set when a wait times out)
   unsigned* bar = nullptr;
@@ -98,11 +99,19 @@ struct Engine {
 
   // lane_ = index of this thread within its team of NW wavefronts; ph_ = LDS phasor table
   // [kMaxFreqs] (needed for interaction-picture systems only)
-  __device__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
+  __device__ __forceinline__ Engine(const SysDev& s, T* xs_, double2* ps_, double* red_, int lane_, T* ph_ = nullptr)
       : S(s), xs(xs_), xsA(xs_), ps(ps_), red(red_), lane(lane_) {
     hcolU = uniform(s.hcol);
-    if constexpr (kRegs || !(OPT & kOffsets)) hoffU = nullptr;  // (register-cached columns otherwise)
-    else hoffU = uniform(s.hoff);
+    if constexpr (kRegs || !(OPT & kOffsets)) {
+      hoffU = nullptr;  // (register-cached columns otherwise)
+    } else {
+      hoffU = uniform(s.hoff);
+      if (hoffU && uniform(s.dim) == ROWS * G) {
+        int mx = 0;
+        for (int w = 0; w < uniform(s.wh); ++w) mx = max(mx, abs(gbl(hoffU)[w]));
+        offC = mx <= G;
+      }
+    }
     if constexpr (F32) hvalU = uniform(s.hval32);
     else hvalU = uniform(s.hval);
     whU = uniform(s.wh);
@@ -156,11 +165,8 @@ struct Engine {
       for (int i = lane; i < whU * dimU / 4; i += G) idst[i] = isrc[i];
       dictL = dict; dbL = base; idxL = idx;
       if (off) {
-        int mx = 0;
-        for (int w = 0; w < whU; ++w) mx = max(mx, abs(gbl(hoffU)[w]));
         if (lane < whU) off[lane] = gbl(hoffU)[lane];
         doffL = off;
-        dictC = mx <= G && dimU == ROWS * G;
       }
       sync();
     }
@@ -232,22 +238,25 @@ struct Engine {
     }
   }
 
-  // Dictionary form when the state fills the team and no |offset| exceeds G, slot-major: a slot's
-  // offset, part of the dictionary and row of index bytes are set up once for all of the lane's
+  // Constant-offset table when the state fills the team and no |offset| exceeds G, slot-major: a
+  // slot's offset and coefficient source -- its part of the dictionary and row of index bytes in
+  // LDS (DICT), else its column of the global table -- are set up once for all of the lane's
   // rows, and the column row + offset needs the clamp of the general path only in the first row
   // block (it can fall below 0) and the last (beyond dim - 1); in between it stays inside the
   // state.  Each row still sums its slots in order with the same operands, so k is bitwise the
   // general path's.
-  __device__ __forceinline__ void gatherDictC(T k[ROWS]) const {
+  template <bool DICT>
+  __device__ __forceinline__ void gatherC(T k[ROWS]) const {
     T acc[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) acc[r] = from_d<T>(make_double2(0., 0.));
     const int wh = whU, dim = dimU;
 #pragma unroll 3
     for (int w = 0; w < wh; ++w) {
-      const int off = doffL[w];
-      const T* dw = dictL + dbL[w];
-      const unsigned char* iw = idxL + (unsigned)(w * dim + lane);
+      const int off = DICT && doffL ? doffL[w] : gbl(hoffU)[w];
+      const T* dw = DICT ? dictL + dbL[w] : nullptr;
+      const unsigned char* iw = DICT ? idxL + (unsigned)(w * dim + lane) : nullptr;
+      const unsigned cw = (unsigned)(w * dim + lane);
       const T* xw = xs + (lane + off);
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
@@ -260,7 +269,8 @@ struct Engine {
         } else {
           xv = xw[r * G];
         }
-        acc[r] = cfma(dw[iw[r * G]], xv, acc[r]);
+        const T cf = DICT ? dw[iw[r * G]] : gld2(hvalU, cw + (unsigned)(r * G));
+        acc[r] = cfma(cf, xv, acc[r]);
       }
     }
 #pragma unroll
@@ -270,7 +280,12 @@ struct Engine {
   // k = (H_nH/i) v, with v already staged in xs
   __device__ __forceinline__ void gather(T k[ROWS]) const {
     if constexpr (TD) if (td) { gatherTD(k); return; }
-    if constexpr (!kRegs && (OPT & kOffsets) != 0) if (dictC) { gatherDictC(k); return; }
+    if constexpr (!kRegs && (OPT & kOffsets) != 0)
+      if (offC) {
+        if (dictL) gatherC<true>(k);
+        else gatherC<false>(k);
+        return;
+      }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       T acc = from_d<T>(make_double2(0., 0.));
@@ -537,7 +552,7 @@ struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
 
   // xs2: optional second stage buffer (see Engine::xsB)
-  __device__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_, T* ph_ = nullptr,
+  __device__ __forceinline__ LdsEngine(const SysDev& s, T* xs_, double2* X_, double* red_, int lane_, T* ph_ = nullptr,
                        T* xs2 = nullptr)
       : B(s, xs_, X_, red_, lane_, ph_), X(X_) { B::xsB = xs2; }
 
@@ -1307,15 +1322,26 @@ __device__ __forceinline__ void advance_member_lds(const SysDev& S, const EnsDev
 // TDC: members may run in the interaction picture (engines with phasor tables; a member without
 // frequencies skips them at run time); GSLC: members may use GSL's controller.  The host picks
 // the plain variant unless some member needs one of them.
+// PH: the team kinds this kernel runs, bit 0 kind 0 (phase 1), bit 1 kind 2 (phase 2), bit 2
+// kind 1 (phase 3); bit 3: a work group is one quarter team of NWB / 4 wavefronts (phases 2 and 3
+// only), so the team's lanes get the registers of a 64 NWB / 4 thread group.  The host launches
+// PH = 1 and PH = 14 as two kernels on two streams: each gets a register allocation of its own
+// (the work-group engine's state and the smaller engines' no longer spill into each other), and
+// the second fills compute units as the first drains.
 // ---------------------------------------------------------------------------
 
-template <int NWB, int RB, int WHB, int RW, int WHW, bool F32, bool TDC = false, bool GSLC = false>
-__global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
+template <int NWB, int RB, int WHB, int RW, int WHW, bool F32, bool TDC = false, bool GSLC = false, int PH = 7>
+__global__ void __launch_bounds__(64 * ((PH & 8) ? NWB / 4 : NWB), (PH & 8) ? 2 : 1)
+    mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
+  constexpr bool kP1 = (PH & 1) != 0, kP2 = (PH & 2) != 0, kP3 = (PH & 4) != 0, kQG = (PH & 8) != 0;
+  static_assert(!(kQG && kP1), "a quarter-team work group runs phases 2 and 3 only");
+  constexpr int kThreads = 64 * (kQG ? NWB / 4 : NWB);
   constexpr int NWQ = NWB / 4;
   // (a second stage buffer for the work-group team measured 5% slower here: 404 vs 384 ms on
   // the 8192-trajectory four-system batch, same GPU and layout)
   using EngB = LdsEngine<RB, NWB, WHB, F32, false, TDC, kOffsets>;
-  using EngQ = LdsEngine<RB, NWQ, WHB, F32, true, TDC>;
+  // a quarter team that is a whole work group (kQG) synchronises with the hardware barrier
+  using EngQ = LdsEngine<RB, NWQ, WHB, F32, !kQG, TDC, kOffsets>;
   using EngW = Engine<RW, 1, WHW, F32, false, TDC>;
   using TB = typename EngB::T;
   using TW = typename EngW::T;
@@ -1328,20 +1354,21 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
       (size_t)RB * GQ * (sizeof(double2) + sizeof(TB)) + (NWQ * kMaxJumps + kMaxJumps + kCounters) * 8 + kPh;
   constexpr size_t kQuarterRaw = kQuarterOwn > kWaveEach * NWQ ? kQuarterOwn : kWaveEach * NWQ;
   constexpr size_t kQuarter = (kQuarterRaw + 15) / 16 * 16;
-  constexpr size_t kBytes = kBlockBytes > 4 * kQuarter ? kBlockBytes : 4 * kQuarter;
+  constexpr size_t kTeams = kP2 || kP3 ? (kQG ? 1 : 4) * kQuarter : 0;
+  constexpr size_t kBytes = kP1 && kBlockBytes > kTeams ? kBlockBytes : kTeams;
   // the work-group team's coefficient dictionary with its index bytes, where LDS allows
   // (the index bytes get what LDS has left, up to kMaxDictSlots slots; loadDict checks the fit)
   constexpr long kIdxAvail = 160L * 1024 - (long)kBytes - kMaxDict * (long)sizeof(TB) - (long)sizeof(MixDev) - 6 * 1024;
   constexpr size_t kIdxB = kIdxAvail < 4L * RB * GB ? 0
                            : (size_t)(kIdxAvail / 1024 * 1024 < (long)kMaxDictSlots * RB * GB ? kIdxAvail / 1024 * 1024
                                                                                               : (long)kMaxDictSlots * RB * GB);
-  constexpr bool kDictB = kIdxB > 0;
+  constexpr bool kDictB = kP1 && kIdxB > 0;
   __shared__ MixDev M;
   __shared__ __align__(16) unsigned char pool[kBytes];
   __shared__ TB dictS[kDictB ? kMaxDict : 1];
   __shared__ int dbaseS[kMaxDictSlots], doffS[kMaxDictSlots];
   __shared__ __align__(16) unsigned char didxS[kDictB ? kIdxB : 4];
-  __shared__ double red[NWB * kMaxJumps];
+  __shared__ double red[kP1 ? NWB * kMaxJumps : 1];
   __shared__ double rtsB[kMaxJumps];
   __shared__ long long cntB[kCounters];
   __shared__ int shItem;
@@ -1357,7 +1384,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   {  // copy the member descriptors to LDS (16-byte words)
     const int4* src = reinterpret_cast<const int4*>(Mg);
     int4* dst = reinterpret_cast<int4*>(&M);
-    for (int i = threadIdx.x; i < (int)(sizeof(MixDev) / sizeof(int4)); i += 64 * NWB) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(sizeof(MixDev) / sizeof(int4)); i += kThreads) dst[i] = src[i];
   }
   __syncthreads();
 
@@ -1369,7 +1396,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   };
 
   // phase 1: one trajectory per work group
-  if (M.items[0] > 0) {
+  if (kP1 && M.items[0] > 0) {
     double2* X = reinterpret_cast<double2*>(pool);
     TB* xs = reinterpret_cast<TB*>(X + RB * GB);
     for (;;) {
@@ -1390,7 +1417,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   unsigned char* qbase = pool + kQuarter * q;
 
   // phase 2: one trajectory per quarter of the work group
-  if (M.items[2] > 0) {
+  if (kP2 && M.items[2] > 0) {
     const int qlane = threadIdx.x % GQ;
     double2* X = reinterpret_cast<double2*>(qbase);
     TB* xs = reinterpret_cast<TB*>(X + RB * GQ);
@@ -1421,7 +1448,7 @@ __global__ void __launch_bounds__(64 * NWB, 1) mcwf_mixed_kernel(const MixDev* _
   }
 
   // phase 3: one trajectory per wavefront (inside its quarter's region)
-  {
+  if constexpr (kP3) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned char* mine = qbase + kWaveEach * (w % NWQ);
     double2* ps = reinterpret_cast<double2*>(mine);

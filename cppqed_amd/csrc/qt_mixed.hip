@@ -22,24 +22,28 @@ int mixed_variants(int* table, int cap) {
   return n;
 }
 
-int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl) {
-  if (td && gsl) return f32 ? mixed_blocks_td_gsl_f32(nwb, rb, whb, rw, whw) : mixed_blocks_td_gsl_f64(nwb, rb, whb, rw, whw);
-  if (td) return f32 ? mixed_blocks_td_f32(nwb, rb, whb, rw, whw) : mixed_blocks_td_f64(nwb, rb, whb, rw, whw);
-  if (gsl) return f32 ? mixed_blocks_gsl_f32(nwb, rb, whb, rw, whw) : mixed_blocks_gsl_f64(nwb, rb, whb, rw, whw);
-  return f32 ? mixed_blocks_plain_f32(nwb, rb, whb, rw, whw) : mixed_blocks_plain_f64(nwb, rb, whb, rw, whw);
+// ph: 1 for the work-group-team kernel, 6 for the quarter- and wavefront-team kernel
+int mixed_blocks_per_cu(int nwb, int rb, int whb, int rw, int whw, int ph, bool f32, bool td, bool gsl) {
+  if (td && gsl)
+    return f32 ? mixed_blocks_td_gsl_f32(nwb, rb, whb, rw, whw, ph) : mixed_blocks_td_gsl_f64(nwb, rb, whb, rw, whw, ph);
+  if (td) return f32 ? mixed_blocks_td_f32(nwb, rb, whb, rw, whw, ph) : mixed_blocks_td_f64(nwb, rb, whb, rw, whw, ph);
+  if (gsl) return f32 ? mixed_blocks_gsl_f32(nwb, rb, whb, rw, whw, ph) : mixed_blocks_gsl_f64(nwb, rb, whb, rw, whw, ph);
+  return f32 ? mixed_blocks_plain_f32(nwb, rb, whb, rw, whw, ph) : mixed_blocks_plain_f64(nwb, rb, whb, rw, whw, ph);
 }
 
-hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, bool f32, bool td, bool gsl, int grid,
+hipError_t launch_mixed(int nwb, int rb, int whb, int rw, int whw, int ph, bool f32, bool td, bool gsl, int grid,
                         const MixDev* M, hipStream_t st) {
   if (td && gsl)
-    return f32 ? launch_mixed_td_gsl_f32(nwb, rb, whb, rw, whw, grid, M, st)
-               : launch_mixed_td_gsl_f64(nwb, rb, whb, rw, whw, grid, M, st);
+    return f32 ? launch_mixed_td_gsl_f32(nwb, rb, whb, rw, whw, ph, grid, M, st)
+               : launch_mixed_td_gsl_f64(nwb, rb, whb, rw, whw, ph, grid, M, st);
   if (td)
-    return f32 ? launch_mixed_td_f32(nwb, rb, whb, rw, whw, grid, M, st) : launch_mixed_td_f64(nwb, rb, whb, rw, whw, grid, M, st);
+    return f32 ? launch_mixed_td_f32(nwb, rb, whb, rw, whw, ph, grid, M, st)
+               : launch_mixed_td_f64(nwb, rb, whb, rw, whw, ph, grid, M, st);
   if (gsl)
-    return f32 ? launch_mixed_gsl_f32(nwb, rb, whb, rw, whw, grid, M, st) : launch_mixed_gsl_f64(nwb, rb, whb, rw, whw, grid, M, st);
-  return f32 ? launch_mixed_plain_f32(nwb, rb, whb, rw, whw, grid, M, st)
-             : launch_mixed_plain_f64(nwb, rb, whb, rw, whw, grid, M, st);
+    return f32 ? launch_mixed_gsl_f32(nwb, rb, whb, rw, whw, ph, grid, M, st)
+               : launch_mixed_gsl_f64(nwb, rb, whb, rw, whw, ph, grid, M, st);
+  return f32 ? launch_mixed_plain_f32(nwb, rb, whb, rw, whw, ph, grid, M, st)
+             : launch_mixed_plain_f64(nwb, rb, whb, rw, whw, ph, grid, M, st);
 }
 
 }  // namespace qt
