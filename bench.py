@@ -303,7 +303,8 @@ def master_traffic(args, tries, launches):
 
 def mixed_batch(Q, args):
     """Uneven batch: 8192 trajectories drawn uniformly from four builds (dims 1024, 4096, 64, 60),
-    FP32 derivatives, one persistent launch, layout chosen by the autotuner."""
+    FP32 derivatives, one batched launch (two persistent kernels on two streams: work-group teams,
+    and quarter plus wavefront teams), layout chosen by the autotuner."""
     from sysdefs import MIXED, basis, mixed_counts
     counts = mixed_counts()
     iv = Q.run_intervals(T=args.mixed_T, Dt=0.1)
@@ -341,7 +342,7 @@ def mixed_batch(Q, args):
         seq += time.perf_counter() - t1
     return {"workload": f"{int(counts.sum())} trajectories drawn uniformly from 4 builds "
                         f"({', '.join(f'{n} {c}' for (n, _, _), c in zip(MIXED, counts))}), T={args.mixed_T}, "
-                        "Dt=0.1, FP32 derivatives, one persistent launch",
+                        "Dt=0.1, FP32 derivatives, one batched launch (two persistent kernels)",
             "tries": sum(tries), "tries_per_member": tries, "seconds": el, "tries_per_s": sum(tries) / el,
             "kernel_ms": kms, "variant": Q.batch_variants()[variant], "teams": b.layout()[1],
             "autotune_s": tune_s, "autotune_probe_ms": [float(x) for x in probe_ms],

@@ -556,7 +556,9 @@ def batch_variants():
 
 
 class Batch:
-    """Several ensembles (different systems) advanced by one persistent launch (qt_batch_*)."""
+    """Several ensembles (different systems) advanced by one batched launch (qt_batch_*): the
+    work-group-team members in one persistent kernel, the quarter- and wavefront-team members in a
+    second beside it."""
 
     def __init__(self, members, variant=-1, team=None):
         self.members = list(members)

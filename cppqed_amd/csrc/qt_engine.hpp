@@ -489,7 +489,10 @@ struct Engine {
     }
   }
 
-  // LDS scratch [8][64] for the batched stream-point reduction (one-wavefront teams; may be null)
+  // LDS scratch [8][kOredRow] for the batched stream-point reduction (one-wavefront teams; may be
+  // null).  Rows of 65 doubles: the transposed reads (lane 8j + c reads row j, doubles 8c .. 8c + 7)
+  // then fall on all 32 banks instead of two bank pairs.
+  static constexpr int kOredRow = 65;
   double* ored = nullptr;
   // LDS copy of the first two channels' rate weights, [2][ROWS * G] (SysDev::jw; may be null)
   const double* jwl = nullptr;
@@ -502,12 +505,12 @@ struct Engine {
   __device__ void observablesBatched(const double2 x[ROWS], int no, Store store) const {
     for (int q0 = 0; q0 < no; q0 += 8) {
       const int nq = no - q0 < 8 ? no - q0 : 8;
-      for (int j = 0; j < 8; ++j) ored[j * 64 + lane] = j < nq ? obsPartial(q0 + j, x) : 0.;
+      for (int j = 0; j < 8; ++j) ored[j * kOredRow + lane] = j < nq ? obsPartial(q0 + j, x) : 0.;
       wave_sync();
       const int j = lane >> 3, c = lane & 7;
       double v = 0.;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v += ored[j * 64 + c * 8 + i];
+      for (int i = 0; i < 8; ++i) v += ored[j * kOredRow + c * 8 + i];
       v += dpp<0xB1>(v); v += dpp<0x4E>(v); v += dpp<0x141>(v);
       if (c == 0 && j < nq) store(q0 + j, v);
       wave_sync();
@@ -1255,7 +1258,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     // state copy share storage
     __shared__ double2 ps[ROWS * G];
     __shared__ double2 pc[kPcLds ? ROWS * G : 1];
-    __shared__ double ored[8 * 64];
+    __shared__ double ored[8 * Eng::kOredRow];
     __shared__ double jwl[2 * ROWS * G];  // rate weights of the system (the same for every trajectory)
     Eng eng(S, reinterpret_cast<T*>(ps), ps, red, lane, ph);
     eng.ored = ored;

@@ -312,7 +312,9 @@ int qt_batch_destroy(qt_batch* b);
 /* Chosen layout: *variant and team[n_members]. */
 int qt_batch_layout(const qt_batch* b, int* variant, int* team);
 /* Advances member m through intervals[m][0..n_intervals[m]) (each member as qt_mcwf_advance),
- * all members in one launch; rows[m] (may be NULL) receives member m's ensemble rows. */
+ * all members together: the work-group-team members in one persistent kernel on `stream`, the
+ * quarter- and wavefront-team members in a second one on a stream of the batch's own that
+ * `stream` then waits for; rows[m] (may be NULL) receives member m's ensemble rows. */
 int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_intervals, double* const* rows,
                      qt_stream stream);
 /* Times every layout that fits the members on a probe (each member advanced by probe_span on a

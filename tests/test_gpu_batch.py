@@ -1,6 +1,7 @@
 """GPU parity of the mixed batch: ensembles of four different systems (dims 1024, 4096, 64, 60)
-advanced by one persistent launch, against the CPU oracle (FP64) and against each member run
-alone (FP32 derivatives)."""
+advanced by one batched launch (work-group teams in one persistent kernel, quarter and wavefront
+teams in a second beside it), against the CPU oracle (FP64) and against each member run alone
+(FP32 derivatives)."""
 import numpy as np
 import pytest
 
