@@ -490,9 +490,10 @@ struct Engine {
   }
 
   // LDS scratch [8][kOredRow] for the batched stream-point reduction (one-wavefront teams; may be
-  // null).  Rows of 65 doubles: the transposed reads (lane 8j + c reads row j, doubles 8c .. 8c + 7)
-  // then fall on all 32 banks instead of two bank pairs.
-  static constexpr int kOredRow = 65;
+  // null).  (Rows padded to 65 doubles measured more bank-conflict cycles, 5.8e8 vs 4.3e8 per
+  // JC-60 launch: lane 8j + c reads doubles 8c .. 8c + 7 of row j, which 16-byte-aligned rows
+  // let the compiler fetch as four 16-byte reads.)
+  static constexpr int kOredRow = 64;
   double* ored = nullptr;
   // LDS copy of the first two channels' rate weights, [2][ROWS * G] (SysDev::jw; may be null)
   const double* jwl = nullptr;

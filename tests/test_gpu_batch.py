@@ -50,6 +50,48 @@ def test_mixed_fp64_trajectories_match_oracle():
         np.testing.assert_allclose(rows[i], ro[1:], rtol=1e-9, atol=1e-9, err_msg=name)
 
 
+def _check_against_oracle(entries, mem, rows, nint, seed0=SEED):
+    """Every trajectory of every member against the oracle (FP64), as in the test above."""
+    for i, ((name, spec, idx), e) in enumerate(zip(entries, mem)):
+        so = O.System(spec)
+        psi0 = basis(so.dim, idx)
+        per, _ = e.counters()
+        for k in range(e.n):
+            res = O.mcwf_run(so, psi0, O.params(rng="philox", seed=seed0 + i, stream=k), O.run_spec("NDt", Dt=0.1, NDt=nint))
+            psi_g, _ = e.state(k)
+            jt, jch, nj = e.jumplog(k)
+            assert nj == len(res["jump_t"]), (name, k)
+            np.testing.assert_array_equal(jch, res["jump_ch"])
+            err = np.linalg.norm(psi_g - res["psi"]) / np.linalg.norm(res["psi"])
+            assert err <= 1e-9, (name, k, err)
+            for c in (0, 1, 2, 4, 5, 7):
+                assert per[k][c] == res["counters"][c], (name, k, c)
+        ro, _ = O.ensemble_run(so, psi0, O.params(rng="philox", seed=seed0 + i, stream=0), O.run_spec("NDt", Dt=0.1, NDt=nint), e.n)
+        np.testing.assert_allclose(rows[i], ro[1:], rtol=1e-9, atol=1e-9, err_msg=name)
+
+
+def test_mixed_single_kernel_batches_match_oracle():
+    """The batched launch is two kernels (work-group teams; quarter and wavefront teams on the
+    batch's side stream).  A batch with every member on work groups runs only the first, one
+    without a work-group member only the second; both match the oracle trajectory by trajectory."""
+    iv = np.full(6, 0.1)
+    counts = [3, 2, 5, 5]
+    print(f"[gpu] mixed batch, every member on work groups, counts={counts}", flush=True)
+    mem = _members(counts, Q.QT_FP64)
+    b = Q.Batch(mem, team=[0, 0, 0, 0])
+    assert b.layout()[1] == [0, 0, 0, 0]
+    _check_against_oracle(MIXED, mem, b.advance(iv), len(iv))
+    sub = [MIXED[0], MIXED[2], MIXED[3]]
+    print(f"[gpu] mixed batch without a work-group member: {[e[0] for e in sub]}", flush=True)
+    mem2 = []
+    for i, ((name, spec, idx), n) in enumerate(zip(sub, [4, 6, 6])):
+        s = Q.System(spec)
+        mem2.append(Q.Ensemble(s, n, basis(s.dim, idx), Q.mcwf_pars(seed=SEED + i, precision=Q.QT_FP64, jump_log_cap=256)))
+    b2 = Q.Batch(mem2)
+    assert 0 not in b2.layout()[1], b2.layout()
+    _check_against_oracle(sub, mem2, b2.advance(iv), len(iv))
+
+
 def test_mixed_equals_members_alone():
     """Each member's rows from the mixed launch equal the member run alone with the same engine."""
     counts = [64, 32, 256, 256]
