@@ -490,10 +490,11 @@ struct Engine {
   }
 
   // LDS scratch [8][kOredRow] for the batched stream-point reduction (one-wavefront teams; may be
-  // null).  (Rows padded to 65 doubles measured more bank-conflict cycles, 5.8e8 vs 4.3e8 per
-  // JC-60 launch: lane 8j + c reads doubles 8c .. 8c + 7 of row j, which 16-byte-aligned rows
-  // let the compiler fetch as four 16-byte reads.)
-  static constexpr int kOredRow = 64;
+  // null).  Lane 8j + c reads doubles 8c .. 8c + 7 of row j as four 16-byte reads; rows of 66
+  // doubles keep them 16-byte aligned and start each row 4 banks on, so the 64 lanes spread over
+  // all banks (bank-conflict cycles per JC-60 launch: 4.3e8 with 64-double rows, 5.8e8 with 65,
+  // whose misaligned rows split the reads, 2.7e8 with 66).
+  static constexpr int kOredRow = 66;
   double* ored = nullptr;
   // LDS copy of the first two channels' rate weights, [2][ROWS * G] (SysDev::jw; may be null)
   const double* jwl = nullptr;
