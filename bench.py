@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ntraj", type=int, default=16384)
     ap.add_argument("--T", type=float, default=20.0)
+    ap.add_argument("--Dt", type=float, default=0.1,
+                    help="stream spacing of the JC-60 ensemble (counter runs at other spacings separate the "
+                         "cost of a stream point from that of an RK try)")
     ap.add_argument("--plain-traj", type=int, default=4)
     ap.add_argument("--cpu-traj", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
@@ -105,7 +108,7 @@ def main():
 
     sysq = Q.System(JC60)
     psi0 = ground(sysq.dim)
-    iv = Q.run_intervals(T=args.T, Dt=0.1)
+    iv = Q.run_intervals(T=args.T, Dt=args.Dt)
 
     def run_batched(precision, steps, warmup):
         ens = Q.Ensemble(sysq, args.ntraj, psi0, Q.mcwf_pars(seed=1001, precision=precision, jump_log_cap=64))
@@ -156,7 +159,7 @@ def main():
     traffic = None
     issue = None
     tpath = os.path.join(ROOT, "profiles", "mcwf_batch_jc60_16k_traffic.json")
-    if os.path.exists(tpath) and args.ntraj == 16384 and args.T == 20.0:
+    if os.path.exists(tpath) and args.ntraj == 16384 and args.T == 20.0 and args.Dt == 0.1:
         with open(tpath) as f:
             tj = json.load(f)
         traffic = tj["traffic_bytes_per_launch"]
@@ -224,7 +227,7 @@ def main():
         "dtype": "fp32-deriv/fp64-control",
         "data": "synthetic",
         "config": {"workload": f"driven Jaynes-Cummings dim 60, {args.ntraj} MCWF trajectories to T={args.T}, "
-                               "Dt=0.1 streams, FP32 derivatives + FP64 control",
+                               f"Dt={args.Dt} streams, FP32 derivatives + FP64 control",
                    "n_traj": args.ntraj, "T": args.T, "dim": 60, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "valu_issue_frac": issue,
@@ -237,6 +240,10 @@ def main():
         "batched_fp64_over_plain": (v64 / plain["value"]) if plain else None,
         "kernel": {"launches": b32["launches"], "avg_ms": 1e3 * b32["kernel_s"] / max(1, b32["launches"]),
                    "graph": b32["graph"],
+                   # one step's totals: MCWF steps, RK tries, stream points (n_traj x intervals)
+                   "per_step": {"mcwf_steps": int(b32["last_totals"][0]),
+                                "tries": b32["tries"] // max(1, args.steps),
+                                "stream_points": args.ntraj * len(iv)},
                    "note": "each step replays one captured hipGraph (jump draws, queue reset, batched kernel); "
                            "avg_ms is the graph's duration from HIP events on its stream"},
         "cpu_baseline": cpu,
@@ -266,7 +273,7 @@ def cpu_baseline(spec, psi0, args):
     from concurrent.futures import ThreadPoolExecutor
     import oracle_lib as O
     so = O.System(spec)
-    run = O.run_spec("Dt", Dt=0.1, T=args.T)
+    run = O.run_spec("Dt", Dt=args.Dt, T=args.T)
 
     def chunk(first, n):
         return O.ensemble_run(so, psi0, O.params(rng="philox", seed=1001, stream=first), run, n)[1]
