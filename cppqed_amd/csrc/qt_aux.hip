@@ -62,7 +62,7 @@ __global__ void stream_rows_kernel(const unsigned long long* acc, int n_int, int
 template <int ROWS, int NW, int WH, bool F32, bool TD>
 __global__ void __launch_bounds__(64 * NW) apply_heff_kernel(SysDev S, const double2* psi, double2* dpsidt, int n,
                                                              double tau) {
-  using Eng = Engine<ROWS, NW, WH, F32, false, TD>;
+  using Eng = Engine<ROWS, NW, WH, F32, TD>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
   __shared__ T xs[ROWS * G];

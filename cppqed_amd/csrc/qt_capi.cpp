@@ -1448,7 +1448,7 @@ struct qt_batch_s {
   bool td = false;              // some member runs in the interaction picture
   bool gsl = false;             // some member uses GSL's controller
   DevBuf<qt::MixDev> mix;
-  DevBuf<int> queue;            // [3] queue heads + [1] barrier error flag
+  DevBuf<int> queue;            // [3] queue heads, one per team kind
   qt::MixDev* host = nullptr;   // pinned staging for the descriptor upload
   // the launch is two kernels: work-group teams on the caller's stream (grid1), quarter and
   // wavefront teams on a stream of the batch's own (grid6), joined by events
@@ -1527,7 +1527,7 @@ int qt_batch_create(qt_ens* const* members, int n_members, const qt_batch_params
       if ((e->p.precision == QT_FP32_DERIV) != b->f32) throw Fail(QT_EINVAL, "members must share one precision");
     b->choose(p ? p->variant : -1, p ? p->team : nullptr);
     b->mix.alloc(1);
-    b->queue.alloc(4);
+    b->queue.alloc(3);
     hipCheck(hipHostMalloc((void**)&b->host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
     hipCheck(hipEventCreate(&b->e0), "hipEventCreate");
     hipCheck(hipEventCreate(&b->e1), "hipEventCreate");
@@ -1563,7 +1563,6 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   std::memset(&M, 0, sizeof(M));
   M.n_members = (int)b->mem.size();
   M.queue = b->queue.p;
-  M.err = b->queue.p + 3;
   int items[3] = {0, 0, 0};
   for (size_t m = 0; m < b->mem.size(); ++m) {
     qt_ens* e = b->mem[m];
@@ -1578,7 +1577,7 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   }
   for (int k = 0; k < 3; ++k) M.items[k] = items[k];
   hipCheck(hipMemcpyAsync(b->mix.p, &M, sizeof(M), hipMemcpyHostToDevice, st), "batch descriptor H2D");
-  hipCheck(qt::launch_fill_int(b->queue.p, 4, 0, st), "reset queues");
+  hipCheck(qt::launch_fill_int(b->queue.p, 3, 0, st), "reset queues");
   // every resident work group of each kernel, also when the work would fit fewer: wavefront and
   // quarter items then spread over all compute units instead of filling a few.  The work-group
   // teams go first on st; the smaller teams run beside them on the batch's own stream (default
@@ -1610,12 +1609,9 @@ static void batch_launch(qt_batch* b, const double* const* intervals, const int*
   }
 }
 
-// a quarter team's counter barrier that timed out leaves 2 in the batch's error flag
-static void batch_check(qt_batch* b, hipStream_t st) {
-  int flag = 0;
-  hipCheck(hipMemcpyAsync(&flag, b->queue.p + 3, sizeof(int), hipMemcpyDeviceToHost, st), "D2H");
+// waits for a batched launch (both of its kernels: the side stream joins st)
+static void batch_sync(qt_batch*, hipStream_t st) {
   hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-  if (flag) throw Fail(QT_EDEVICE, "mixed launch: a team barrier timed out");
 }
 
 extern "C" {
@@ -1628,7 +1624,7 @@ int qt_batch_advance(qt_batch* b, const double* const* intervals, const int* n_i
     std::vector<double> t0(b->mem.size());
     for (size_t m = 0; m < b->mem.size(); ++m) t0[m] = b->mem[m]->time;
     batch_launch(b, intervals, n_int, st);
-    batch_check(b, st);
+    batch_sync(b, st);
     for (size_t m = 0; m < b->mem.size(); ++m) {
       qt_ens* e = b->mem[m];
       for (int k = 0; k < n_int[m]; ++k) e->time += intervals[m][k];
@@ -1682,13 +1678,13 @@ int qt_batch_autotune(qt_batch* b, double probe_span, int probe_traj, int* varia
         qt_batch probe;
         probe.mem = pm; probe.f32 = b->f32;
         probe.choose((int)v, tm.data());
-        probe.mix.alloc(1); probe.queue.alloc(4);
+        probe.mix.alloc(1); probe.queue.alloc(3);
         hipCheck(hipHostMalloc((void**)&probe.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
         hipCheck(hipEventCreate(&probe.e0), "hipEventCreate");
         hipCheck(hipEventCreate(&probe.e1), "hipEventCreate");
         for (auto* e : pm) { int rc = qt_mcwf_reset(e, nullptr); if (rc) throw Fail(rc, g_err); }
         batch_launch(&probe, ivs.data(), nis.data(), nullptr);
-        batch_check(&probe, nullptr);
+        batch_sync(&probe, nullptr);
         float ms = 0.f;
         hipCheck(hipEventElapsedTime(&ms, probe.e0, probe.e1), "hipEventElapsedTime");
         if (ms_out && nc < ms_cap) ms_out[nc] = ms;
@@ -2283,7 +2279,7 @@ int qt_server_create(const qt_server_params* p, qt_server** out) {
     s->P = *p;
     s->batch.f32 = p->mcwf.precision == QT_FP32_DERIV;
     s->batch.mix.alloc(1);
-    s->batch.queue.alloc(4);
+    s->batch.queue.alloc(3);
     hipCheck(hipHostMalloc((void**)&s->batch.host, sizeof(qt::MixDev), hipHostMallocDefault), "hipHostMalloc");
     hipCheck(hipEventCreate(&s->batch.e0), "hipEventCreate");
     hipCheck(hipEventCreate(&s->batch.e1), "hipEventCreate");
@@ -2439,8 +2435,8 @@ int qt_server_step(qt_server* s, int* n_active, int* n_waiting) {
       if (launched) {
         bool ok = true;
         try {
-          batch_check(&s->batch, nullptr);
-        } catch (std::exception& ex) {  // a team barrier of the launch timed out: every member is suspect
+          batch_sync(&s->batch, nullptr);
+        } catch (std::exception& ex) {  // the launch failed: every member is suspect
           for (auto* r : ens) { s->fail(r, code_of(ex), ex.what()); dead.push_back(r); }
           ok = false;
         }

@@ -34,14 +34,14 @@ namespace qt {
 // the per-group trajectory engine
 // ---------------------------------------------------------------------------
 
-// SUB = the team is NW wavefronts of a larger work group: its hand-overs use a counter
-// barrier in LDS instead of the work-group barrier.  TD = interaction-picture system (H entries
+// A team of NW wavefronts is a whole work group (its hand-overs use the work-group barrier) or,
+// for NW = 1, one wavefront.  TD = interaction-picture system (H entries
 // rotate with the picture, psi gets the Exact U after each ODE step); the Schroedinger-picture
 // instantiations carry none of that code.  OPT: optional code paths, compiled only where they
 // pay (each adds registers to a kernel): kOffsets = gathers use the constant-offset form of the
 // H table when the system has one; kTwoStage = two stage buffers (multi-wavefront teams).
 enum { kOffsets = 1, kTwoStage = 2 };
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false, int OPT = 0>
+template <int ROWS, int NW, int WH, bool F32, bool TD = false, int OPT = 0>
 struct Engine {
   static constexpr int kRows = ROWS, kNW = NW;
   static constexpr bool kTD = TD;
@@ -78,12 +78,6 @@ struct Engine {
   // no |offset| exceeds G (gatherC)
   const int* doffL = nullptr;
   bool offC = false;
-  // team barrier of a SUB team: arrival counter in LDS, this wavefront's next target, the
-  // launch's error flag (set when a wait times out)
-  unsigned* bar = nullptr;
-  mutable unsigned barTarget = 0;
-  mutable bool barBroken = false;
-  int* barErr = nullptr;
   // interaction picture: the H entry (w, row) at picture time tau is hval * ph[f] with
   // ph[f] = exp(freq[f] tau), the phasor table in LDS rebuilt for each stage time; the
   // register-cached column carries f in its top byte
@@ -185,29 +179,7 @@ struct Engine {
   __device__ __forceinline__ bool live(int r) const { return r == 0 || r * G < dimU; }
   __device__ __forceinline__ void sync() const {
     if constexpr (NW == 1) wave_sync();
-    else if constexpr (SUB) sub_sync();
     else __syncthreads();
-  }
-
-  // Counter barrier of a SUB team.  Every wavefront adds one arrival and waits until the count
-  // reaches NW per barrier passed.  The wait is bounded: after ~0.1 s the wavefront flags the
-  // launch as failed and stops waiting at later barriers, so the kernel always ends.
-  __device__ __forceinline__ void sub_sync() const {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    barTarget += NW;
-    if ((lane & 63) == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (!barBroken) {
-      unsigned spins = 0;
-      while ((int)(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) - barTarget) < 0) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1u << 22)) {
-          barBroken = true;
-          if ((lane & 63) == 0) atomicExch(barErr, 2);
-          break;
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
 
   // sum / max over the team's lanes; every lane gets the bitwise-same result
@@ -549,9 +521,9 @@ struct Engine {
 // 16-wavefront team) free of register spills.
 // ---------------------------------------------------------------------------
 
-template <int ROWS, int NW, int WH, bool F32, bool SUB = false, bool TD = false, int OPT = 0>
-struct LdsEngine : Engine<ROWS, NW, WH, F32, SUB, TD, OPT> {
-  using B = Engine<ROWS, NW, WH, F32, SUB, TD, OPT>;
+template <int ROWS, int NW, int WH, bool F32, bool TD = false, int OPT = 0>
+struct LdsEngine : Engine<ROWS, NW, WH, F32, TD, OPT> {
+  using B = Engine<ROWS, NW, WH, F32, TD, OPT>;
   using T = typename B::T;
   using B::G; using B::S; using B::xs; using B::red; using B::lane; using B::kd; using B::gsum; using B::gmax;
   double2* X;  // LDS state [ROWS*G] (== B::ps: jumpRow / observable gather from it)
@@ -1213,7 +1185,7 @@ constexpr bool pc_in_lds() { return (size_t)ROWS * 64 * NW * 2 * sizeof(double2)
 
 template <int ROWS, int NW, int WH, bool F32, int MINW, bool TD, bool GSL = false>
 __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, EnsDev Earg) {
-  using Eng = Engine<ROWS, NW, WH, F32, false, TD, kOffsets>;
+  using Eng = Engine<ROWS, NW, WH, F32, TD, kOffsets>;
   using T = typename Eng::T;
   constexpr int G = Eng::G;
   constexpr bool kPcLds = pc_in_lds<ROWS, NW>();
@@ -1248,7 +1220,7 @@ __global__ void __launch_bounds__(64 * NW, MINW) mcwf_batch_kernel(SysDev Sarg, 
     constexpr size_t kDictBytes = kDict ? kIdx + kMaxDict * sizeof(T) : 0;
     constexpr bool kTwo = kBase + ROWS * G * sizeof(T) + kDictBytes <= 150 * 1024;
     __shared__ T xs2[kTwo ? ROWS * G : 1];
-    LdsEngine<ROWS, NW, WH, F32, false, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
+    LdsEngine<ROWS, NW, WH, F32, TD, kOffsets | (kTwo ? kTwoStage : 0)> eng(S, xs, Xs, red, lane, ph,
                                                                                  kTwo ? xs2 : nullptr);
     __shared__ T dict[kDict ? kMaxDict : 1];
     __shared__ int dbase[kMaxDictSlots], doff[kMaxDictSlots];
@@ -1314,40 +1286,36 @@ __device__ __forceinline__ void advance_member_lds(const SysDev& S, const EnsDev
 }
 
 // ---------------------------------------------------------------------------
-// mixed batch: trajectories of up to kMaxMembers ensembles (different systems) in one
-// persistent launch.  Three team kinds share one work group of NWB wavefronts:
-//   kind 0: the whole work group owns a trajectory (RB rows per lane, psi in LDS);
-//   kind 2: each quarter (NWB/4 wavefronts) owns a trajectory (RB rows per lane, psi in LDS,
-//           counter barriers in LDS);
-//   kind 1: each wavefront owns a trajectory (RW rows per lane, psi in registers).
-// Work groups first drain the kind-0 queue together, then each quarter drains the kind-2
-// queue on its own, then each wavefront the kind-1 queue.  A quarter's LDS region holds its
-// four wavefronts' kind-1 regions, so a quarter that moves on never touches LDS another
-// quarter still uses.
+// mixed batch: trajectories of up to kMaxMembers ensembles (different systems) in one batched
+// launch of two persistent kernels.  Three team kinds:
+//   kind 0: a work group of NWB wavefronts owns a trajectory (RB rows per lane, psi in LDS);
+//   kind 2: a quarter team of NWB/4 wavefronts owns a trajectory (RB rows per lane, psi in LDS);
+//   kind 1: a wavefront owns a trajectory (RW rows per lane, psi in registers).
+// PH selects the kinds a kernel runs: PH = 1, kind 0 on NWB-wavefront work groups; PH = 14,
+// kinds 2 and 1 on work groups of one quarter team each (bit 3), which first drain the kind-2
+// queue together (work-group barrier) and then split into wavefronts for the kind-1 queue; the
+// team's LDS region holds its wavefronts' kind-1 regions.  The host launches the two on two
+// streams: each gets a register allocation of its own (the work-group engine's state and the
+// smaller engines' do not spill into each other; a quarter team's lanes get the registers of a
+// 64 NWB / 4 thread group), and the second fills compute units as the first drains.
 // TDC: members may run in the interaction picture (engines with phasor tables; a member without
 // frequencies skips them at run time); GSLC: members may use GSL's controller.  The host picks
 // the plain variant unless some member needs one of them.
-// PH: the team kinds this kernel runs, bit 0 kind 0 (phase 1), bit 1 kind 2 (phase 2), bit 2
-// kind 1 (phase 3); bit 3: a work group is one quarter team of NWB / 4 wavefronts (phases 2 and 3
-// only), so the team's lanes get the registers of a 64 NWB / 4 thread group.  The host launches
-// PH = 1 and PH = 14 as two kernels on two streams: each gets a register allocation of its own
-// (the work-group engine's state and the smaller engines' no longer spill into each other), and
-// the second fills compute units as the first drains.
 // ---------------------------------------------------------------------------
 
-template <int NWB, int RB, int WHB, int RW, int WHW, bool F32, bool TDC = false, bool GSLC = false, int PH = 7>
+template <int NWB, int RB, int WHB, int RW, int WHW, bool F32, bool TDC, bool GSLC, int PH>
 __global__ void __launch_bounds__(64 * ((PH & 8) ? NWB / 4 : NWB), (PH & 8) ? 2 : 1)
     mcwf_mixed_kernel(const MixDev* __restrict__ Mg) {
   constexpr bool kP1 = (PH & 1) != 0, kP2 = (PH & 2) != 0, kP3 = (PH & 4) != 0, kQG = (PH & 8) != 0;
   static_assert(!(kQG && kP1), "a quarter-team work group runs phases 2 and 3 only");
+  static_assert(kQG || !(kP2 || kP3), "quarter and wavefront teams run on quarter-team work groups");
   constexpr int kThreads = 64 * (kQG ? NWB / 4 : NWB);
   constexpr int NWQ = NWB / 4;
   // (a second stage buffer for the work-group team measured 5% slower here: 404 vs 384 ms on
   // the 8192-trajectory four-system batch, same GPU and layout)
-  using EngB = LdsEngine<RB, NWB, WHB, F32, false, TDC, kOffsets>;
-  // a quarter team that is a whole work group (kQG) synchronises with the hardware barrier
-  using EngQ = LdsEngine<RB, NWQ, WHB, F32, !kQG, TDC, kOffsets>;
-  using EngW = Engine<RW, 1, WHW, F32, false, TDC>;
+  using EngB = LdsEngine<RB, NWB, WHB, F32, TDC, kOffsets>;
+  using EngQ = LdsEngine<RB, NWQ, WHB, F32, TDC, kOffsets>;
+  using EngW = Engine<RW, 1, WHW, F32, TDC>;
   using TB = typename EngB::T;
   using TW = typename EngW::T;
   constexpr int GB = 64 * NWB, GQ = 64 * NWQ;
@@ -1377,10 +1345,8 @@ __global__ void __launch_bounds__(64 * ((PH & 8) ? NWB / 4 : NWB), (PH & 8) ? 2 
   __shared__ double rtsB[kMaxJumps];
   __shared__ long long cntB[kCounters];
   __shared__ int shItem;
-  // quarter teams' claimed items and barrier counters, outside the pool: a wavefront that has
-  // passed its quarter's last barrier reuses the pool while the others may still read them
-  __shared__ int qslot[4];
-  __shared__ unsigned qbar[4];
+  // the quarter team's claimed item, outside the pool (phase 3 reuses the pool)
+  __shared__ int qslot[1];
   __shared__ TB phB[TDC ? kMaxFreqs : 1];
   auto align16 = [](void* p) {
     return reinterpret_cast<TB*>((reinterpret_cast<size_t>(p) + 15) / 16 * 16);
@@ -1418,10 +1384,9 @@ __global__ void __launch_bounds__(64 * ((PH & 8) ? NWB / 4 : NWB), (PH & 8) ? 2 
     __syncthreads();
   }
 
-  const int q = threadIdx.x / GQ;
-  unsigned char* qbase = pool + kQuarter * q;
+  unsigned char* qbase = pool;  // the quarter team's region (the whole pool of a PH = 14 work group)
 
-  // phase 2: one trajectory per quarter of the work group
+  // phase 2: one trajectory per quarter team
   if (kP2 && M.items[2] > 0) {
     const int qlane = threadIdx.x % GQ;
     double2* X = reinterpret_cast<double2*>(qbase);
@@ -1430,26 +1395,18 @@ __global__ void __launch_bounds__(64 * ((PH & 8) ? NWB / 4 : NWB), (PH & 8) ? 2 
     double* rts = qred + NWQ * kMaxJumps;
     long long* cnt = reinterpret_cast<long long*>(rts + kMaxJumps);
     TB* qph = TDC ? align16(cnt + kCounters) : nullptr;
-    int* slot = qslot + q;
-    unsigned* bar = qbar + q;
-    if (qlane == 0) *bar = 0u;
-    __syncthreads();  // every quarter starts phase 2 together, with its counter at zero
-    EngQ eng(M.S[0], xs, X, qred, qlane);  // re-targeted below; holds the barrier state
-    eng.bar = bar;
-    eng.barErr = M.err;
+    int* slot = qslot;
     for (;;) {
-      eng.sync();
+      __syncthreads();
       if (qlane == 0) *slot = atomicAdd(M.queue + 2, 1);
-      eng.sync();
+      __syncthreads();
       const int item = *slot;
       if (item >= M.items[2]) break;
       const int m = member(2, item);
-      EngQ e2(M.S[m], xs, X, qred, qlane, qph);
-      e2.bar = bar; e2.barErr = M.err; e2.barTarget = eng.barTarget; e2.barBroken = eng.barBroken;
-      advance_member_lds<GSLC>(M.S[m], M.E[m], e2, item - M.first[m], rts, cnt);
-      eng.barTarget = e2.barTarget; eng.barBroken = e2.barBroken;
+      EngQ eng(M.S[m], xs, X, qred, qlane, qph);
+      advance_member_lds<GSLC>(M.S[m], M.E[m], eng, item - M.first[m], rts, cnt);
     }
-    eng.sync();  // the quarter's LDS is free only when all of its wavefronts are done with it
+    __syncthreads();  // the team's LDS is free only when all of its wavefronts are done with it
   }
 
   // phase 3: one trajectory per wavefront (inside its quarter's region)

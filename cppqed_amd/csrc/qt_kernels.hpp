@@ -121,7 +121,6 @@ struct alignas(16) MixDev {
   int n_members;
   int items[3];                 // trajectories per team kind: 0 work group, 1 wavefront, 2 quarter work group
   int* queue;                   // [3] queue heads, one per team kind
-  int* err;                     // set to 2 when a quarter's counter barrier timed out
   int team[kMaxMembers];        // team kind of member m
   int first[kMaxMembers];       // first item of member m in its kind's queue
   SysDev S[kMaxMembers];

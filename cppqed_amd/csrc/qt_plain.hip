@@ -9,7 +9,7 @@ namespace qt {
 // ratio with GSL (ODE_GSL.h:44-84).  Interaction picture (TD): stage times t + c dt - t0.
 template <int ROWS, int NW, int WH, bool TD, bool GSL>
 __global__ void __launch_bounds__(64 * NW) plain_try_kernel(SysDev S, PlainTry P) {
-  using Eng = Engine<ROWS, NW, WH, false, false, TD>;
+  using Eng = Engine<ROWS, NW, WH, false, TD>;
   constexpr int G = Eng::G;
   __shared__ double2 xs[ROWS * G];
   __shared__ double2 ps[ROWS * G];
