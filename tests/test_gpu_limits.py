@@ -101,6 +101,31 @@ def test_large_states_with_offset_gathers_match_oracle(name):
             assert per[k][c] == res["counters"][c], (k, c, per[k][c], res["counters"][c])
 
 
+@pytest.mark.parametrize("cutoff", [1024, 1000])
+def test_offset_gathers_in_a_team_match_oracle(cutoff):
+    """A pumped lossy mode in a 4-wavefront team (4 rows per lane, 1024 lanes' worth of rows):
+    at cutoff 1024 the state fills the team and the slot-major gather runs; at 1000 it does not
+    and the general constant-offset gather does.  FP64, each trajectory against the oracle."""
+    spec = f"free mode cutoff={cutoff} delta=-1 kappa=1 eta=2"
+    print(f"[gpu] offset gathers in a team, mode cutoff {cutoff}", flush=True)
+    s, so = Q.System(spec), O.System(spec)
+    assert s.dim == cutoff and s.info.h_offsets > 0
+    psi0 = ground(s.dim)
+    seed, ntraj, iv = 41, 4, np.full(10, 0.1)
+    e = Q.Ensemble(s, ntraj, psi0, Q.mcwf_pars(seed=seed, jump_log_cap=256, layout=(4, 4)))
+    e.advance(iv)
+    per, _ = e.counters()
+    for k in range(ntraj):
+        res = O.mcwf_run(so, psi0, O.params(rng="philox", seed=seed, stream=k), O.run_spec("NDt", Dt=0.1, NDt=len(iv)))
+        psi, _ = e.state(k)
+        jt, jch, nj = e.jumplog(k)
+        assert nj == len(res["jump_t"]), (k, nj, len(res["jump_t"]))
+        np.testing.assert_array_equal(jch, res["jump_ch"])
+        assert np.linalg.norm(psi - res["psi"]) <= 1e-9 * np.linalg.norm(res["psi"]), k
+        for c in (0, 1, 2, 4, 5, 7):
+            assert per[k][c] == res["counters"][c], (k, c, per[k][c], res["counters"][c])
+
+
 def test_non_finite_state_is_reported():
     """The reference's numeric-failure exceptions.  A non-finite initial state is refused when the
     ensemble is made (its rates: LiouvilleanAveragedCommon.h:82-93 via the MCWF_Trajectory
