@@ -198,15 +198,19 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
     }
   }
   __syncthreads();
-  // drho_ij = N_ij + conj(N_ji)
+  // drho_ij = N_ij + conj(N_ji), kept for the lower tiles r >= c only (drho is Hermitian: the
+  // mirrored entry is the exact conjugate) at lower-tile index u = r (r + 1) / 2 + c
 #pragma unroll
   for (int tw = 0; tw < TW; ++tw) {
     const int t = w + kMasterWaves * tw;
     if (t >= NT) continue;
+    const int r = t / NB, c = t % NB;
+    if (r < c) continue;
+    const int u = r * (r + 1) / 2 + c;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int o = 16 * (t % NB) * LD + 16 * (t / NB) + 4 * v;
-      Kout[(4 * t + v) * 64 + lane] = make_double2(cr[tw][v] + SrT[o], ci[tw][v] - SiT[o]);
+      const int o = 16 * c * LD + 16 * r + 4 * v;
+      Kout[(4 * u + v) * 64 + lane] = make_double2(cr[tw][v] + SrT[o], ci[tw][v] - SiT[o]);
     }
   }
 }
@@ -239,6 +243,14 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
     remL = remS;  // ordered before its first use by the derivative's barrier
   }
 
+  // rho (X) is kept whole, tile by tile; the RK slots K1..K6 and the step's result XN hold only
+  // the NL lower tiles (r >= c), in lower-tile order u = r (r + 1) / 2 + c: they are exactly
+  // Hermitian (drho = N + N^dagger entry by entry, and the stage and step sums act entry by entry
+  // with real weights), so the upper tiles are the mirrored conjugates and need not be stored or
+  // read -- 40% less slot traffic per try.  rho must be Hermitian (the caller's rho0 and every
+  // state the kernel writes are).
+  constexpr int NL = NB * (NB + 1) / 2;
+  constexpr int UW = (NL + kMasterWaves - 1) / kMasterWaves;  // lower tiles per wavefront (at most)
   double2* const X = M.x + (size_t)p * NN;
   double2* const K = M.k + (size_t)p * 7 * NN;
   double2* const K1 = K;
@@ -261,6 +273,24 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   const double* const SiT = Si + (lane & 15) * LD + (lane >> 4);
   auto lofs = [&](int tw, int v) { const int t = tileOf(tw); return (16 * (t / NB) + 4 * v) * LD + 16 * (t % NB); };
   auto tofs = [&](int tw, int v) { const int t = tileOf(tw); return 16 * (t % NB) * LD + 16 * (t / NB) + 4 * v; };
+  // lower tile u = w + 8 uw of this wavefront, its row and column block (lr, lc)
+  int lr[UW], lc[UW];
+#pragma unroll
+  for (int uw = 0; uw < UW; ++uw) {
+    const int u = w + kMasterWaves * uw;
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= u) ++r;
+    lr[uw] = r;
+    lc[uw] = u - r * (r + 1) / 2;
+  }
+  auto hasL = [&](int uw) { return w + kMasterWaves * uw < NL; };
+  auto lidx = [&](int uw, int v) { return (4 * (w + kMasterWaves * uw) + v) * 64 + lane; };       // slots
+  auto xidx = [&](int uw, int v) { return (4 * (lr[uw] * NB + lc[uw]) + v) * 64 + lane; };     // rho
+  // LDS offsets of a lower element (i, j) (with SrL) and of its mirror (j, i) (with SrM)
+  double* const SrM = Sr + (lane & 15) * LD + (lane >> 4);
+  double* const SiM = Si + (lane & 15) * LD + (lane >> 4);
+  auto llofs = [&](int uw, int v) { return (16 * lr[uw] + 4 * v) * LD + 16 * lc[uw]; };
+  auto lmofs = [&](int uw, int v) { return 16 * lc[uw] * LD + 16 * lr[uw] + 4 * v; };
 
   auto bsum = [&](double v) {
     v = wave_sum(v);
@@ -311,22 +341,26 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       (void)tau;
     }
   };
-  // Kout = F(S) at picture time tau; S given element-wise by stage(tw, v)
+  // Kout = F(S) at picture time tau; S given on the lower tiles element-wise by stage(uw, v) and
+  // mirrored (conjugated) into the upper tiles
   auto derivative = [&](auto&& stage, double2* Kout, double tau) {
     __syncthreads();
     setPhases(tau);
-    for (int tw = 0; tw < TW; ++tw) {
-      if (!has(tw)) break;
+    for (int uw = 0; uw < UW; ++uw) {
+      if (!hasL(uw)) break;
+      const bool mirror = lr[uw] != lc[uw];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double2 sv = stage(tw, v);
-        SrL[lofs(tw, v)] = sv.x; SiL[lofs(tw, v)] = sv.y;
+        const double2 sv = stage(uw, v);
+        SrL[llofs(uw, v)] = sv.x; SiL[llofs(uw, v)] = sv.y;
+        if (mirror) { SrM[lmofs(uw, v)] = sv.x; SiM[lmofs(uw, v)] = -sv.y; }
       }
     }
     __syncthreads();
     master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph, remL);
   };
   auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
+  auto ldX = [&](int uw, int v) { return X[xidx(uw, v)]; };
 
   double t = M.t[p], dtDid = M.dtDid[p], dtTry = M.dtTry[p];
   if (tid < kCounters) cnt[tid] = M.counters[(size_t)p * kCounters + tid];
@@ -348,7 +382,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
         const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
         if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
         if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-        derivative([&](int tw, int v) { return ld(X, tw, v); }, K1, t - t0);  // k1 at (x, t); unchanged on retries
+        derivative([&](int uw, int v) { return ldX(uw, v); }, K1, t - t0);  // k1 at (x, t); unchanged on retries
         long long tries = 0, failed = 0;
         for (;;) {
           ++tries;
@@ -356,21 +390,25 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
           // stages 2..6: S = X + dt sum_{i<s} a_si K_i, accumulated in the tableau's order
 #pragma unroll 1
           for (int st = 1; st < 6; ++st) {
-            derivative([&](int tw, int v) {
-              double2 acc = ld(X, tw, v);
-              for (int i = 0; i < st; ++i) acc = axpy(ck_a(st, i) * dt, ld(K + (size_t)i * NN, tw, v), acc);
+            derivative([&](int uw, int v) {
+              double2 acc = ldX(uw, v);
+              for (int i = 0; i < st; ++i) acc = axpy(ck_a(st, i) * dt, K[(size_t)i * NN + lidx(uw, v)], acc);
               return acc;
             }, K + (size_t)st * NN, (t + ck_c(st) * dt) - t0);
           }
+          // the error's maximum over the lower tiles is the whole matrix's (mirrored entries have
+          // the same moduli).  The barrier orders K6's tiles, written by the wavefronts that own
+          // the full tiles, before these reads by the owners of the lower tiles.
+          __syncthreads();
           double emax = 0.;
 #pragma unroll 1
-          for (int tw = 0; tw < TW; ++tw) {
-            if (!has(tw)) break;
+          for (int uw = 0; uw < UW; ++uw) {
+            if (!hasL(uw)) break;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const double2 x = ld(X, tw, v), k1 = ld(K1, tw, v), k3 = ld(K3, tw, v), k4 = ld(K4, tw, v),
-                            k5 = ld(K5, tw, v), k6 = ld(K6, tw, v);
-              XN[gidx(tw, v)] = axpy(CK::b6 * dt, k6, axpy(CK::b4 * dt, k4, axpy(CK::b3 * dt, k3, axpy(CK::b1 * dt, k1, x))));
+              const int li = lidx(uw, v);
+              const double2 x = ldX(uw, v), k1 = K1[li], k3 = K3[li], k4 = K4[li], k5 = K5[li], k6 = K6[li];
+              XN[li] = axpy(CK::b6 * dt, k6, axpy(CK::b4 * dt, k4, axpy(CK::b3 * dt, k3, axpy(CK::b1 * dt, k1, x))));
               double2 xe = make_double2(CK::e1 * dt * k1.x, CK::e1 * dt * k1.y);
               xe = axpy(CK::e6 * dt, k6, axpy(CK::e5 * dt, k5, axpy(CK::e4 * dt, k4, axpy(CK::e3 * dt, k3, xe))));
               const double D = M.epsAbs + M.epsRel * (sqrt(cabs2(x)) + dt * sqrt(cabs2(k1)));
@@ -394,16 +432,23 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
         __syncthreads();
         setPhases(t - t0);  // ph = u
         __syncthreads();
-        for (int tw = 0; tw < TW; ++tw) {
-          if (!has(tw)) break;
+        for (int uw = 0; uw < UW; ++uw) {
+          if (!hasL(uw)) break;
+          const bool mirror = lr[uw] != lc[uw];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            double2 y = ld(XN, tw, v);
+            // entry (i, j) of XN and its mirror (j, i) = conj, each transformed as its own entry
+            const double2 xn = XN[lidx(uw, v)];
+            double2 y = xn, ym = make_double2(xn.x, -xn.y);
             if constexpr (TD) {
-              const int i = rowOf(tw, v), j = colOf(tw);
-              if (i < d && j < d) y = cmul(cmul(ph[i], y), make_double2(ph[j].x, -ph[j].y));
+              const int i = 16 * lr[uw] + 4 * v + (lane >> 4), j = 16 * lc[uw] + (lane & 15);
+              if (i < d && j < d) {
+                y = cmul(cmul(ph[i], y), make_double2(ph[j].x, -ph[j].y));
+                ym = cmul(cmul(ph[j], ym), make_double2(ph[i].x, -ph[i].y));
+              }
             }
-            SrL[lofs(tw, v)] = y.x; SiL[lofs(tw, v)] = y.y;
+            SrL[llofs(uw, v)] = y.x; SiL[llofs(uw, v)] = y.y;
+            if (mirror) { SrM[lmofs(uw, v)] = ym.x; SiM[lmofs(uw, v)] = ym.y; }
           }
         }
         __syncthreads();
@@ -459,12 +504,14 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       const double v = bsum(part);
       if (tid == 0) M.obs_out[((size_t)p * M.n_intervals + kI) * M.no + q] = v;
     }
-    derivative([&](int tw, int v) { return ld(X, tw, v); }, K1, 0.);  // t0 = t at a stream point
+    derivative([&](int uw, int v) { return ldX(uw, v); }, K1, 0.);  // t0 = t at a stream point
+    __syncthreads();  // K1's tiles come from other wavefronts (as K6's before the error estimate)
     double n2 = 0.;
-    for (int tw = 0; tw < TW; ++tw) {
-      if (!has(tw)) break;
+    for (int uw = 0; uw < UW; ++uw) {
+      if (!hasL(uw)) break;
+      const double f = lr[uw] == lc[uw] ? 1. : 2.;  // an off-diagonal tile stands for its mirror too
 #pragma unroll
-      for (int v = 0; v < 4; ++v) n2 += cabs2(ld(K1, tw, v));
+      for (int v = 0; v < 4; ++v) n2 += f * cabs2(K1[lidx(uw, v)]);
     }
     const double dn = sqrt(bsum(n2));
     if (tid == 0) {
