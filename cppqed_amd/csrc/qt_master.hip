@@ -18,7 +18,8 @@
 // MFMA lane layout (measured on gfx950): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
 // D[i][j] in lane 16 (i % 4) + j, component i / 4.  Lane l of the tile's wavefront owns the
 // tile elements (4v + l/16, l%16), v < 4; matrices in HBM are stored tile by tile in that
-// order: index (4 t + v) 64 + l.
+// order: index (4 t + v) 64 + l (rho), or (4 u + v) 64 + l over the lower tiles u = r (r + 1) / 2
+// + c, r >= c, only (the Hermitian RK slots).
 
 #include "qt_device.hpp"
 #include "qt_kernels.hpp"
