@@ -120,6 +120,35 @@ def test_master_sweep_matches_oracle(cutoff, etas, T, kernel):
         assert abs(np.trace(rho_g) - 1) < 1e-13
 
 
+@pytest.mark.parametrize("picture", ["", " picture=UIP"])
+@pytest.mark.parametrize("cutoff", [4, 12, 20, 30, 40, 48])
+def test_master_tile_counts_match_oracle(cutoff, picture):
+    """The MFMA kernel at each tile count NB = 1..6 (d = 8, 24, 40, 60, 80, 96), in the Schroedinger
+    and the unitary interaction picture, from a density operator with coherences everywhere: the
+    RK slots are kept as lower 16 x 16 tiles and mirrored into the upper ones, so every lower-tile
+    index and every mirrored entry takes part; rho must match the oracle's and stay Hermitian."""
+    print(f"[gpu] master tile counts cutoff={cutoff} picture={picture.strip() or 'S'}", flush=True)
+    spec = (f"free qbit deltaA=0.5 gamma=0.2 eta=0.3{picture}\n"
+            f"free mode cutoff={cutoff} delta=-1 kappa=1 eta=1.5{picture}\n"
+            "ia jc 0 1 g=3")
+    sq = Q.System(spec)
+    d = sq.dim
+    rng = np.random.default_rng(cutoff)
+    psi = rng.normal(size=d) + 1j * rng.normal(size=d)
+    psi /= np.linalg.norm(psi)
+    rho0 = np.outer(psi, psi.conj())
+    iv = np.full(4, 0.05)
+    m = Q.MasterSweep([sq], rho0, kernel="mfma")
+    rows, _ = m.advance(iv)
+    rows_o, rho_o = O.master_run(O.System(spec), rho0, O.params(), O.run_spec("NDt", Dt=0.05, NDt=len(iv)))
+    rho_g = m.rho(0)
+    err = np.linalg.norm(rho_g - rho_o) / np.linalg.norm(rho_o)
+    assert err < 1e-9, err
+    np.testing.assert_allclose(rows[0][:, 2:], rows_o[1:, 2:], rtol=1e-8, atol=1e-10)
+    assert np.allclose(rho_g, rho_g.conj().T, atol=0)
+    assert abs(np.trace(rho_g) - 1) < 1e-13
+
+
 def test_master_sweep_autostop_matches_oracle():
     """The reference's autostop criterion (AutostopHandlerGeneric, Trajectory.h:179-222) per sweep
     point: every point stops at the oracle's stop row, on the oracle's density operator, and the
