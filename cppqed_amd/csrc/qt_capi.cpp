@@ -471,6 +471,7 @@ struct qt_master {
   qt_sys* sys0 = nullptr;
   double time = 0.;
   DevBuf<double> hfrag, t, dtDid, dtTry, tss, intervals, obs, dtd, dn, tout;
+  DevBuf<unsigned> hmask;          // [point][NB]: non-zero A fragments per row block (bit per k-step)
   DevBuf<double2> exz;             // [point][d] (interaction picture only)
   bool td = false;
   int wr = 0, nrf = 0;             // remainder of H_nH/i outside the MFMA product (all of it: sparse)
@@ -1765,6 +1766,8 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
     m->sharedH = shared;
     const int nfrag = sparse ? 0 : shared ? 1 : n_points;
     std::vector<double> hf(std::max<size_t>(2, (size_t)nfrag * NB * KC * 64 * 2), 0.);
+    static_assert(qt::kMasterMaxNB * 4 <= 32, "a row block's k-steps fit the mask's bits");
+    std::vector<unsigned> hm(std::max<size_t>(1, (size_t)nfrag * NB), 0u);
     for (int k = 0; k < nfrag; ++k)
       for (int rb = 0; rb < NB; ++rb)
         for (int kk = 0; kk < KC; ++kk)
@@ -1772,8 +1775,9 @@ int qt_master_create(qt_sys* const* systems, int n_points, const qt_master_param
             const qt::cd z = Hrot[k][(size_t)(16 * rb + l % 16) * DP + 4 * kk + l / 16];
             const size_t o = ((((size_t)k * NB + rb) * KC + kk) * 64 + l) * 2;
             hf[o] = z.real(); hf[o + 1] = z.imag();
+            if (z != qt::cd(0., 0.)) hm[(size_t)k * NB + rb] |= 1u << kk;
           }
-    m->hfrag.upload(hf); m->jcol.upload(jc); m->jval.upload(jv);
+    m->hfrag.upload(hf); m->hmask.upload(hm); m->jcol.upload(jc); m->jval.upload(jv);
     if (m->td) {  // the Exact frees' exponents per point (zero in the Schroedinger picture)
       std::vector<double2> z((size_t)n_points * d, make_double2(0., 0.));
       for (int k = 0; k < n_points; ++k) {
@@ -1868,7 +1872,7 @@ static void master_launch(qt_master* m, const double* intervals, int n_int, hipS
     m->dtd.alloc((size_t)m->n * n_int);
     m->dn.alloc((size_t)m->n * n_int);
     qt::MasterDev M{};
-    M.n_points = m->n; M.dim = m->d; M.hfrag = m->hfrag.p;
+    M.n_points = m->n; M.dim = m->d; M.hfrag = m->hfrag.p; M.hmask = m->hmask.p;
     M.nj = m->nj; M.wj = m->wj; M.jcol = m->jcol.p; M.jval = m->jval.p;
     M.no = no; M.wo = m->sys0->dev.wo; M.ocol = m->sys0->dev.ocol; M.oval = m->sys0->dev.oval; M.oimag = m->sys0->dev.oimag; M.owid = m->sys0->dev.owid;
     M.x = m->x.p; M.k = m->k.p; M.t = m->t.p; M.dtDid = m->dtDid.p; M.dtTry = m->dtTry.p;

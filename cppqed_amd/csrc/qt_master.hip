@@ -50,9 +50,19 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
                                           const RemEntry* remL) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB;
   constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wavefront's index (uniform)
   const int d = M.dim;
   const double* const hf = M.hfrag + (M.shared_h ? (size_t)0 : (size_t)p * NB * KC * 64 * 2);
+  // the k-steps whose A fragment is non-zero, per tile (a zero fragment adds exact zeros: skipping
+  // its load and MFMAs leaves the product bit for bit the same)
+  const unsigned* const hm = M.hmask + (M.shared_h ? (size_t)0 : (size_t)p * NB);
+  unsigned mk[TW];
+#pragma unroll
+  for (int tw = 0; tw < TW; ++tw) {
+    const int t = w + kMasterWaves * tw;
+    mk[tw] = t < NT ? hm[t / NB] : 0u;
+  }
   double* const SrL = Sr + (lane >> 4) * LD + (lane & 15);
   double* const SiL = Si + (lane >> 4) * LD + (lane & 15);
   const double* const SrT = Sr + (lane & 15) * LD + (lane >> 4);
@@ -67,7 +77,7 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
 #pragma unroll
     for (int tw = 0; tw < TW; ++tw) {
       const int t = w + kMasterWaves * tw;
-      if (t >= NT || kk >= KC) { a[tw] = make_double2(0., 0.); continue; }
+      if (t >= NT || kk >= KC || !((mk[tw] >> kk) & 1u)) { a[tw] = make_double2(0., 0.); continue; }
       const double* q = hf + ((size_t)((t / NB) * KC + kk) * 64 + lane) * 2;
       a[tw] = make_double2(q[0], q[1]);
     }
@@ -76,7 +86,7 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
 #pragma unroll
     for (int tw = 0; tw < TW; ++tw) {
       const int t = w + kMasterWaves * tw;
-      if (t >= NT) continue;
+      if (t >= NT || !((mk[tw] >> kk) & 1u)) continue;
       const int c = t % NB;
       const double ar = a[tw].x, ai = a[tw].y;
       double br = SrL[4 * kk * LD + 16 * c], bi = SiL[4 * kk * LD + 16 * c];
@@ -539,7 +549,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   if (tid < kCounters) M.counters[(size_t)p * kCounters + tid] = cnt[tid];
 }
 
-bool master_supported(int nb) { return nb >= 1 && nb <= 6; }
+bool master_supported(int nb) { return nb >= 1 && nb <= kMasterMaxNB; }
 
 template <bool TD>
 static hipError_t launch_master_t(int nb, const MasterDev& M, hipStream_t st) {

@@ -6,12 +6,15 @@
 namespace qt {
 
 constexpr int kMaxAutostopAvr = 64;  // averages a point's autostop can watch
+constexpr int kMasterMaxNB = 6;      // the MFMA kernel's largest padded dimension, in 16-row blocks
 
 struct MasterDev {
   int n_points;
   int dim;               // true dimension d
   const double* hfrag;   // [point][NB][DP/4][64] x {re, im}: A fragments of H_nH/i ([1][...] when shared_h)
   int shared_h;          // 1: every point's product uses the one copy (point 0's); its differences are remainder
+  const unsigned* hmask; // [point][NB] ([1][NB] when shared_h): bit kk set when the A fragment of row block
+                         // rb and k-step kk holds a non-zero entry (the product skips the others)
   int nj, wj;
   const int* jcol;       // [point][nj][wj][dim] ELL of the jump operators
   const double2* jval;
