@@ -47,7 +47,7 @@ struct RemEntry { double2 v; int c, f; };
 template <int NB, bool TD>
 __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr, double* Si, double2* Kout,
                                           const double2* ph, const double2* phi, const double2* rph,
-                                          const RemEntry* remL) {
+                                          const RemEntry* remL, const RemEntry* jmpL) {
   constexpr int DP = 16 * NB, KC = DP / 4, LD = DP + 1, NT = NB * NB;
   constexpr int TW = (NT + kMasterWaves - 1) / kMasterWaves;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -177,14 +177,19 @@ __device__ __noinline__ void master_apply(const MasterDev& M, int p, double* Sr,
         double ax = 0., ay = 0.;
         for (int m = 0; m < M.nj; ++m) {
           const size_t base = ((size_t)p * M.nj + m) * M.wj * d;
+          const int lbase = m * M.wj * d;
           for (int aa = 0; aa < M.wj; ++aa) {
-            const double2 ja = M.jval[base + (size_t)aa * d + i];
+            double2 ja;
+            int ca;
+            if (jmpL) { const RemEntry& e = jmpL[lbase + aa * d + i]; ja = e.v; ca = e.c; }
+            else { ja = M.jval[base + (size_t)aa * d + i]; ca = M.jcol[base + (size_t)aa * d + i]; }
             if (ja.x == 0. && ja.y == 0.) continue;
-            const int ca = M.jcol[base + (size_t)aa * d + i];
             for (int b = 0; b < M.wj; ++b) {
-              const double2 jb = M.jval[base + (size_t)b * d + j];
+              double2 jb;
+              int cb;
+              if (jmpL) { const RemEntry& e = jmpL[lbase + b * d + j]; jb = e.v; cb = e.c; }
+              else { jb = M.jval[base + (size_t)b * d + j]; cb = M.jcol[base + (size_t)b * d + j]; }
               if (jb.x == 0. && jb.y == 0.) continue;
-              const int cb = M.jcol[base + (size_t)b * d + j];
               const double sx = Sr[ca * LD + cb], sy = Si[ca * LD + cb];
               const double ux = ja.x * sx - ja.y * sy, uy = ja.x * sy + ja.y * sx;  // J_ia S_{ca,cb}
               ax += ux * jb.x + uy * jb.y;                                          // ... conj(J_jb)
@@ -240,6 +245,10 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   // the point's remainder when it fits (read by every derivative)
   constexpr int kRemBytes = NB <= 5 ? 8192 : 4096;
   __shared__ RemEntry remS[kRemBytes / sizeof(RemEntry)];
+  // the point's jump operators (ELL rows) when they fit, read by every derivative's J S J^dagger
+  // (from LDS instead of a chain of L2 round trips per entry)
+  constexpr int kJmpBytes = NB <= 5 ? 16384 : 0;
+  __shared__ RemEntry jmpS[kJmpBytes ? kJmpBytes / sizeof(RemEntry) : 1];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int p = blockIdx.x;
@@ -252,6 +261,15 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       remS[e] = RemEntry{M.rval[o], M.rcol[o], M.rfi[o]};
     }
     remL = remS;  // ordered before its first use by the derivative's barrier
+  }
+  const RemEntry* jmpL = nullptr;
+  if (kJmpBytes > 0 && M.nj > 0 && (size_t)M.nj * M.wj * d <= kJmpBytes / sizeof(RemEntry)) {
+    const int nje = M.nj * M.wj * d;
+    for (int e = tid; e < nje; e += 64 * kMasterWaves) {
+      const size_t o = (size_t)p * nje + e;
+      jmpS[e] = RemEntry{M.jval[o], M.jcol[o], 0};
+    }
+    jmpL = jmpS;  // ordered before its first use by the derivative's barrier
   }
 
   // rho (X) is kept whole, tile by tile; the RK slots K1..K6 and the step's result XN hold only
@@ -368,7 +386,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       }
     }
     __syncthreads();
-    master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph, remL);
+    master_apply<NB, TD>(M, p, Sr, Si, Kout, ph, phi, rph, remL, jmpL);
   };
   auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
   auto ldX = [&](int uw, int v) { return X[xidx(uw, v)]; };
