@@ -370,19 +370,20 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       (void)tau;
     }
   };
-  // Kout = F(S) at picture time tau; S given on the lower tiles element-wise by stage(uw, v) and
-  // mirrored (conjugated) into the upper tiles
+  // Kout = F(S) at picture time tau; S given on the lower tiles by stage(uw, sv), the four entries
+  // sv[v] of this lane in lower tile uw, and mirrored (conjugated) into the upper tiles
   auto derivative = [&](auto&& stage, double2* Kout, double tau) {
     __syncthreads();
     setPhases(tau);
     for (int uw = 0; uw < UW; ++uw) {
       if (!hasL(uw)) break;
       const bool mirror = lr[uw] != lc[uw];
+      double2 sv[4];
+      stage(uw, sv);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double2 sv = stage(uw, v);
-        SrL[llofs(uw, v)] = sv.x; SiL[llofs(uw, v)] = sv.y;
-        if (mirror) { SrM[lmofs(uw, v)] = sv.x; SiM[lmofs(uw, v)] = -sv.y; }
+        SrL[llofs(uw, v)] = sv[v].x; SiL[llofs(uw, v)] = sv[v].y;
+        if (mirror) { SrM[lmofs(uw, v)] = sv[v].x; SiM[lmofs(uw, v)] = -sv[v].y; }
       }
     }
     __syncthreads();
@@ -390,6 +391,10 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
   };
   auto ld = [&](const double2* A, int tw, int v) { return A[gidx(tw, v)]; };
   auto ldX = [&](int uw, int v) { return X[xidx(uw, v)]; };
+  auto stageX = [&](int uw, double2 (&sv)[4]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sv[v] = ldX(uw, v);
+  };
 
   double t = M.t[p], dtDid = M.dtDid[p], dtTry = M.dtTry[p];
   if (tid < kCounters) cnt[tid] = M.counters[(size_t)p * kCounters + tid];
@@ -411,7 +416,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
         const double nextDtTry = (fabs(Dt) < fabs(dtTry / 10.) ? dtTry : 0.);
         if ((Dt > 0.) != (dtTry > 0.)) dtTry = -dtDid;
         if (fabs(dtTry) > fabs(Dt)) dtTry = Dt;
-        derivative([&](int uw, int v) { return ldX(uw, v); }, K1, t - t0);  // k1 at (x, t); unchanged on retries
+        derivative(stageX, K1, t - t0);  // k1 at (x, t); unchanged on retries
         long long tries = 0, failed = 0;
         for (;;) {
           ++tries;
@@ -419,10 +424,18 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
           // stages 2..6: S = X + dt sum_{i<s} a_si K_i, accumulated in the tableau's order
 #pragma unroll 1
           for (int st = 1; st < 6; ++st) {
-            derivative([&](int uw, int v) {
-              double2 acc = ldX(uw, v);
-              for (int i = 0; i < st; ++i) acc = axpy(ck_a(st, i) * dt, K[(size_t)i * NN + lidx(uw, v)], acc);
-              return acc;
+            // each slot's four entries are loaded together, then added in the tableau's order
+            derivative([&](int uw, double2 (&sv)[4]) {
+              stageX(uw, sv);
+              for (int i = 0; i < st; ++i) {
+                const double c = ck_a(st, i) * dt;
+                const double2* const Ki = K + (size_t)i * NN;
+                double2 kv[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) kv[v] = Ki[lidx(uw, v)];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sv[v] = axpy(c, kv[v], sv[v]);
+              }
             }, K + (size_t)st * NN, (t + ck_c(st) * dt) - t0);
           }
           // the error's maximum over the lower tiles is the whole matrix's (mirrored entries have
@@ -533,7 +546,7 @@ __global__ void __launch_bounds__(64 * kMasterWaves) master_kernel(MasterDev M) 
       const double v = bsum(part);
       if (tid == 0) M.obs_out[((size_t)p * M.n_intervals + kI) * M.no + q] = v;
     }
-    derivative([&](int uw, int v) { return ldX(uw, v); }, K1, 0.);  // t0 = t at a stream point
+    derivative(stageX, K1, 0.);  // t0 = t at a stream point
     __syncthreads();  // K1's tiles come from other wavefronts (as K6's before the error estimate)
     double n2 = 0.;
     for (int uw = 0; uw < UW; ++uw) {
